@@ -1,0 +1,361 @@
+"""Host-side mirror of the chemrust-scf eigensolver surface over libchs_gpu.so.
+
+The names follow the reference (TonyWu20/chemrust-scf): ``Gpu.from_host`` /
+``Gpu.sync_to_host`` (src/device/mod.rs:216-265), ``PcieAccount``
+(src/device/pcie.rs:21-70), ``apply_full_hamiltonian``
+(src/eigensolver/chebyshev.rs:371-403), ``compute_spectral_bounds``
+(:214-254), ``chebyshev_filter`` (:626-816), ``rayleigh_ritz``
+(src/eigensolver/rayleigh_ritz.rs:48-245), ``pw_coords_to_fft_indices``
+(src/scf.rs:387-401) and ``run_scf`` (:358-376). Errors are raised as
+``ChsError`` subclasses named after the reference's ``Error`` variants
+(src/types.rs:225-248).
+
+Every compute call goes to hand-written gfx950 kernels in libchs_gpu.so;
+there is no CPU fallback: if the library or a GPU is missing, the calls
+raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libchs_gpu.so")
+
+C128, C64 = 0, 1
+OK = 0
+_ERR = {
+    -1: "NotImplemented", -2: "ChebyshevDiverged", -3: "RayleighRitzFailed", -4: "Solver",
+    -5: "Blas", -6: "Fft", -7: "Device", -8: "KernelBuild", -9: "InvalidArgument",
+    -10: "OutOfMemory", -11: "NotConverged",
+}
+
+
+class ChsError(RuntimeError):
+    def __init__(self, code, msg, diag=None):
+        super().__init__(f"{_ERR.get(code, code)}: {msg}")
+        self.code = code
+        self.diag = diag
+
+
+class ChebyshevDiverged(ChsError):
+    pass
+
+
+class RayleighRitzFailed(ChsError):
+    pass
+
+
+class Diag(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("norm_previous", C.c_double),
+                ("norm_current", C.c_double), ("growth_factor", C.c_double),
+                ("solver_info", C.c_int32)]
+
+
+class ScfParams(C.Structure):
+    _fields_ = [
+        ("n_rep", C.c_int32), ("a0", C.c_double), ("zval", C.c_double),
+        ("rc_loc", C.c_double), ("ecut", C.c_double),
+        ("n_grid", C.c_int32), ("n_bands", C.c_int32),
+        ("kfrac", C.c_double * 3),
+        ("n_proj_per_atom", C.c_int32),
+        ("beta_rc", C.c_double), ("q_rc", C.c_double), ("q0", C.c_double),
+        ("d_min", C.c_double), ("d_max", C.c_double),
+        ("seed_d", C.c_uint64), ("seed_psi", C.c_uint64), ("seed_lanczos", C.c_uint64),
+        ("cheb_degree", C.c_int32), ("first_passes", C.c_int32),
+        ("lanczos_k", C.c_int32), ("pulay_depth", C.c_int32),
+        ("mix_alpha", C.c_double), ("kerker_q", C.c_double), ("etol", C.c_double),
+        ("smear_kt", C.c_double),
+        ("max_iter", C.c_int32), ("precision", C.c_int32), ("fp64_final", C.c_int32),
+    ]
+
+
+class ScfResult(C.Structure):
+    _fields_ = [("n_iter", C.c_int32), ("converged", C.c_int32), ("n_pw", C.c_int32),
+                ("n_proj", C.c_int32), ("energy", C.c_double), ("mu", C.c_double),
+                ("seconds", C.c_double), ("t_filter", C.c_double), ("t_rr", C.c_double),
+                ("t_density", C.c_double), ("t_veff", C.c_double), ("t_lanczos", C.c_double)]
+
+
+_LIB = None
+vp = C.c_void_p
+
+
+def lib():
+    """Load libchs_gpu.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        i, d, sz, u64 = C.c_int, C.c_double, C.c_size_t, C.c_uint64
+        dp = C.POINTER(C.c_double)
+        sig = {
+            "chs_context_create": [i, C.POINTER(vp)],
+            "chs_context_destroy": [vp],
+            "chs_synchronize": [vp],
+            "chs_diag_get": [vp, C.POINTER(Diag)],
+            "chs_transfer_bytes": [vp, C.POINTER(u64), C.POINTER(u64)],
+            "chs_transfer_reset": [vp],
+            "chs_alloc": [vp, sz, C.POINTER(vp)],
+            "chs_free": [vp, vp],
+            "chs_h2d": [vp, vp, vp, sz],
+            "chs_d2h": [vp, vp, vp, sz],
+            "chs_d2d": [vp, vp, vp, sz],
+            "chs_memset": [vp, vp, i, sz],
+            "chs_pw_coords_to_fft_indices": [vp, vp, i, i, i, i, vp],
+            "chs_gsphere": [vp, d, d, dp, i, i, C.POINTER(i), vp, vp, vp],
+            "chs_fft3d_c2c": [vp, i, vp, i, i, i],
+            "chs_bind_hamiltonian": [vp, i, i, vp, vp, vp],
+            "chs_set_veff": [vp, vp],
+            "chs_bind_projectors": [vp, i, vp, vp, vp],
+            "chs_set_dion": [vp, vp],
+            "chs_apply_h": [vp, i, vp, i, vp, vp],
+            "chs_spectral_bounds": [dp, i, d, dp],
+            "chs_grid_lambda_max": [vp, d, dp],
+            "chs_lanczos_bound": [vp, i, u64, dp, dp],
+            "chs_chebyshev_filter": [vp, i, vp, i, i, dp, vp],
+            "chs_rayleigh_ritz": [vp, vp, i, dp, vp],
+            "chs_zhegvd": [vp, i, vp, vp, dp],
+            "chs_cholqr": [vp, vp, i],
+            "chs_zgemm": [vp, C.c_char, C.c_char, i, i, i, dp, vp, i, vp, i, dp, vp, i],
+            "chs_zdotc": [vp, i, vp, vp, dp],
+            "chs_zaxpy": [vp, i, dp, vp, vp],
+            "chs_zscal": [vp, i, dp, vp],
+            "chs_density": [vp, vp, i, dp, d, vp],
+            "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        L.chs_last_error.restype = C.c_char_p
+        L.chs_last_error.argtypes = [vp]
+        _LIB = L
+    return _LIB
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device context = one HIP stream (CudaContext::new + default stream)."""
+
+    def __init__(self, device=0):
+        h = vp()
+        rc = lib().chs_context_create(device, C.byref(h))
+        if rc != OK:
+            raise ChsError(rc, f"cannot create a context on device {device}")
+        self.h = h
+        self._bufs = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chs_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def check(self, rc):
+        if rc == OK:
+            return
+        msg = lib().chs_last_error(self.h).decode(errors="replace")
+        dg = Diag()
+        lib().chs_diag_get(self.h, C.byref(dg))
+        cls = {-2: ChebyshevDiverged, -3: RayleighRitzFailed}.get(rc, ChsError)
+        raise cls(rc, msg, dg)
+
+    def synchronize(self):
+        self.check(lib().chs_synchronize(self.h))
+
+    def transfer_bytes(self):
+        """PcieAccount {h2d_bytes, d2h_bytes} of this context."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self.check(lib().chs_transfer_bytes(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def transfer_reset(self):
+        self.check(lib().chs_transfer_reset(self.h))
+
+
+class Gpu:
+    """Device array owned by the caller (Gpu<T>, src/device/mod.rs:50-58)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = vp()
+        ctx.check(lib().chs_alloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, arr, dtype=None):
+        arr = np.ascontiguousarray(arr, dtype=dtype)
+        g = cls(ctx, arr.shape, arr.dtype)
+        ctx.check(lib().chs_h2d(ctx.h, g.ptr, arr.ctypes.data_as(vp), g.nbytes))
+        return g
+
+    def sync_to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(lib().chs_d2h(self.ctx.h, out.ctypes.data_as(vp), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        self.ctx.check(lib().chs_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if getattr(self, "ptr", None) and self.ctx.h:
+            lib().chs_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def precision_of(dtype):
+    return C128 if np.dtype(dtype) == np.complex128 else C64
+
+
+# ---- setup ---------------------------------------------------------------
+
+def pw_coords_to_fft_indices(ctx, miller, grid):
+    """src/scf.rs:387-401 on the device: Miller (n_pw, 3) int32 -> int32 indices."""
+    miller = np.ascontiguousarray(miller, np.int32).reshape(-1, 3)
+    n_pw = miller.shape[0]
+    dm = Gpu.from_host(ctx, miller)
+    di = Gpu(ctx, (n_pw,), np.int32)
+    ngx, ngy, ngz = grid
+    ctx.check(lib().chs_pw_coords_to_fft_indices(ctx.h, dm.ptr, n_pw, ngx, ngy, ngz, di.ptr))
+    return di.sync_to_host()
+
+
+def gsphere(ctx, L, ecut, n_grid, kfrac=(0.0, 0.0, 0.0)):
+    """Plane-wave sphere on the device: returns (miller, fft_idx, kin) as device arrays."""
+    kf = np.ascontiguousarray(kfrac, np.float64)
+    n = C.c_int()
+    ctx.check(lib().chs_gsphere(ctx.h, L, ecut, _dptr(kf), n_grid, 0, C.byref(n), None, None, None))
+    n_pw = n.value
+    mil = Gpu(ctx, (n_pw, 3), np.int32)
+    idx = Gpu(ctx, (n_pw,), np.int32)
+    kin = Gpu(ctx, (n_pw,), np.float64)
+    ctx.check(lib().chs_gsphere(ctx.h, L, ecut, _dptr(kf), n_grid, n_pw, C.byref(n), mil.ptr,
+                                idx.ptr, kin.ptr))
+    return mil, idx, kin
+
+
+def fft3d_c2c(ctx, grid, sign):
+    """BatchedFftPlan3d c2c (sign +1 inverse, -1 forward) in place on a device array
+    of shape (batch, n, n, n)."""
+    batch, n = grid.shape[0], grid.shape[-1]
+    ctx.check(lib().chs_fft3d_c2c(ctx.h, precision_of(grid.dtype), grid.ptr, n, batch, sign))
+
+
+# ---- Hamiltonian and eigensolver ----------------------------------------
+
+class Hamiltonian:
+    """Bound plane-wave set + V_eff (+ projectors) on a context."""
+
+    def __init__(self, ctx, n_grid, miller, kin, veff):
+        self.ctx = ctx
+        self.n_grid = n_grid
+        self.miller = miller if isinstance(miller, Gpu) else Gpu.from_host(ctx, np.asarray(miller, np.int32))
+        self.kin = kin if isinstance(kin, Gpu) else Gpu.from_host(ctx, np.asarray(kin, np.float64))
+        self.veff = veff if isinstance(veff, Gpu) else Gpu.from_host(ctx, np.asarray(veff, np.float64))
+        self.n_pw = self.kin.shape[0]
+        ctx.check(lib().chs_bind_hamiltonian(ctx.h, n_grid, self.n_pw, self.miller.ptr,
+                                             self.kin.ptr, self.veff.ptr))
+        self.n_proj = 0
+
+    def set_veff(self, veff):
+        self.veff = veff if isinstance(veff, Gpu) else Gpu.from_host(self.ctx, np.asarray(veff, np.float64))
+        self.ctx.check(lib().chs_set_veff(self.ctx.h, self.veff.ptr))
+
+    def bind_projectors(self, beta, d, q):
+        self.beta = beta if isinstance(beta, Gpu) else Gpu.from_host(self.ctx, np.asarray(beta, np.complex128))
+        self.d = d if isinstance(d, Gpu) else Gpu.from_host(self.ctx, np.asarray(d, np.float64))
+        self.q = q if isinstance(q, Gpu) else Gpu.from_host(self.ctx, np.asarray(q, np.float64))
+        self.n_proj = self.beta.shape[0]
+        self.ctx.check(lib().chs_bind_projectors(self.ctx.h, self.n_proj, self.beta.ptr,
+                                                 self.d.ptr, self.q.ptr))
+
+    def set_dion(self, d):
+        self.d = d if isinstance(d, Gpu) else Gpu.from_host(self.ctx, np.asarray(d, np.float64))
+        self.ctx.check(lib().chs_set_dion(self.ctx.h, self.d.ptr))
+
+
+def apply_full_hamiltonian(ctx, psi, want_s=False):
+    """H|psi> (and S|psi>) for a device block psi of shape (n_b, n_pw)."""
+    hpsi = Gpu(ctx, psi.shape, psi.dtype)
+    spsi = Gpu(ctx, psi.shape, psi.dtype) if want_s else None
+    ctx.check(lib().chs_apply_h(ctx.h, precision_of(psi.dtype), psi.ptr, psi.shape[0], hpsi.ptr,
+                                spsi.ptr if spsi else None))
+    return (hpsi, spsi) if want_s else hpsi
+
+
+def compute_spectral_bounds(eigenvalues, lambda_max):
+    """{lambda_max, eps_cut, center, half_width} (chebyshev.rs:214-254)."""
+    eig = np.ascontiguousarray(eigenvalues if eigenvalues is not None else [], np.float64)
+    out = np.zeros(4)
+    rc = lib().chs_spectral_bounds(_dptr(eig), len(eig), lambda_max, _dptr(out))
+    if rc:
+        raise ChsError(rc, "spectral bounds")
+    return out
+
+
+def grid_lambda_max(ctx, L):
+    out = C.c_double()
+    ctx.check(lib().chs_grid_lambda_max(ctx.h, L, C.byref(out)))
+    return out.value
+
+
+def lanczos_bound(ctx, k=10, seed=4):
+    up, lo = C.c_double(), C.c_double()
+    ctx.check(lib().chs_lanczos_bound(ctx.h, k, seed, C.byref(up), C.byref(lo)))
+    return up.value, lo.value
+
+
+def chebyshev_filter(ctx, psi, degree, bounds, out=None):
+    """Degree-m filter of device block psi (n_b, n_pw); returns the filtered block."""
+    out = out if out is not None else Gpu(ctx, psi.shape, psi.dtype)
+    b4 = np.ascontiguousarray(bounds, np.float64)
+    ctx.check(lib().chs_chebyshev_filter(ctx.h, precision_of(psi.dtype), psi.ptr, psi.shape[0],
+                                         degree, _dptr(b4), out.ptr))
+    return out
+
+
+def zgemm(ctx, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
+    """BlasHandle::gemm_c64 (src/device/blas.rs:24-61), column-major."""
+    al = np.array([complex(alpha).real, complex(alpha).imag])
+    be = np.array([complex(beta).real, complex(beta).imag])
+    ctx.check(lib().chs_zgemm(ctx.h, opa.encode(), opb.encode(), m, n, k, _dptr(al), a.ptr, lda,
+                              b.ptr, ldb, _dptr(be), c.ptr, ldc))
+
+
+def zhegvd(ctx, a, b=None):
+    """SolverHandle::zhegvd (src/device/solver.rs:47-95): itype 1, lower, vectors.
+    a, b: device (n, n) c128 column-major; returns eigenvalues; a <- eigenvectors."""
+    n = a.shape[0]
+    w = np.zeros(n)
+    ctx.check(lib().chs_zhegvd(ctx.h, n, a.ptr, b.ptr if b is not None else None, _dptr(w)))
+    return w
+
+
+def cholqr(ctx, y):
+    """Cholesky-QR orthonormalisation of a c128 device block (in place)."""
+    ctx.check(lib().chs_cholqr(ctx.h, y.ptr, y.shape[0]))
+
+
+def rayleigh_ritz(ctx, y, want_hy=False):
+    """In-place Rayleigh-Ritz of a c128 device block; returns (eps, hy or None)."""
+    eps = np.zeros(y.shape[0])
+    hy = Gpu(ctx, y.shape, y.dtype) if want_hy else None
+    ctx.check(lib().chs_rayleigh_ritz(ctx.h, y.ptr, y.shape[0], _dptr(eps), hy.ptr if hy else None))
+    return eps, hy

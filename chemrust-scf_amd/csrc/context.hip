@@ -1,0 +1,211 @@
+// Context, memory, transfer accounting and status reporting.
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+
+#include "impl.hpp"
+
+namespace chs {
+
+int set_error(chs_context* ctx, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->last_error = buf;
+  return code;
+}
+
+int ensure(chs_context* ctx, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return CHS_OK;
+  if (b.p) {
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    CHS_HIP(ctx, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  CHS_HIP(ctx, hipMalloc(&b.p, want));
+  b.bytes = want;
+  return CHS_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int twiddles(chs_context* ctx, int n, int precision, const void** out) {
+  auto key = std::make_pair(n, precision);
+  auto it = ctx->impl->twiddles.find(key);
+  if (it != ctx->impl->twiddles.end()) {
+    *out = it->second;
+    return CHS_OK;
+  }
+  void* d = nullptr;
+  if (precision == CHS_C128) {
+    std::vector<double> h(2 * n);
+    for (int t = 0; t < n; ++t) {
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / n;
+      h[2 * t] = (double)cosl(a);
+      h[2 * t + 1] = (double)sinl(a);
+    }
+    CHS_HIP(ctx, hipMalloc(&d, h.size() * sizeof(double)));
+    CHS_HIP(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(2 * n);
+    for (int t = 0; t < n; ++t) {
+      long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)t / n;
+      h[2 * t] = (float)cosl(a);
+      h[2 * t + 1] = (float)sinl(a);
+    }
+    CHS_HIP(ctx, hipMalloc(&d, h.size() * sizeof(float)));
+    CHS_HIP(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+  ctx->impl->twiddles[key] = d;
+  *out = d;
+  return CHS_OK;
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" {
+
+int chs_context_create(int device, chs_context** out) {
+  if (!out) return CHS_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return CHS_ERR_DEVICE;
+  if (device < 0 || device >= n) return CHS_ERR_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return CHS_ERR_DEVICE;
+  auto* ctx = new chs_context;
+  ctx->device = device;
+  ctx->impl = new chs_context::Impl;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&ctx->impl->ev0) != hipSuccess ||
+      hipEventCreate(&ctx->impl->ev1) != hipSuccess) {
+    delete ctx->impl;
+    delete ctx;
+    return CHS_ERR_DEVICE;
+  }
+  *out = ctx;
+  return CHS_OK;
+}
+
+int chs_context_destroy(chs_context* ctx) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  auto* im = ctx->impl;
+  for (auto& kv : im->twiddles) (void)hipFree(kv.second);
+  DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
+                    &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
+                    &im->ham.st_mem, &im->ham.veff_f, &im->ham.beta_f, &im->ham.sinv_m,
+                    &im->ham.sinv_m_f};
+  for (DevBuf* b : bufs) release(*b);
+  (void)hipEventDestroy(im->ev0);
+  (void)hipEventDestroy(im->ev1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete im;
+  delete ctx;
+  return CHS_OK;
+}
+
+int chs_synchronize(chs_context* ctx) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CHS_OK;
+}
+
+const char* chs_last_error(chs_context* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+int chs_diag_get(chs_context* ctx, chs_diag* out) {
+  if (!ctx || !out) return CHS_ERR_INVALID_ARG;
+  *out = ctx->diag;
+  return CHS_OK;
+}
+
+int chs_transfer_bytes(chs_context* ctx, uint64_t* h2d, uint64_t* d2h) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  if (h2d) *h2d = ctx->h2d;
+  if (d2h) *d2h = ctx->d2h;
+  return CHS_OK;
+}
+
+int chs_transfer_reset(chs_context* ctx) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  ctx->h2d = ctx->d2h = 0;
+  return CHS_OK;
+}
+
+int chs_alloc(chs_context* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return CHS_ERR_INVALID_ARG;
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 1;
+  CHS_HIP(ctx, hipMalloc(dptr, bytes));
+  return CHS_OK;
+}
+
+int chs_free(chs_context* ctx, void* dptr) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  if (!dptr) return CHS_OK;
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  CHS_HIP(ctx, hipFree(dptr));
+  return CHS_OK;
+}
+
+int chs_h2d(chs_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return CHS_ERR_INVALID_ARG;
+  if (!bytes) return CHS_OK;
+  CHS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->h2d += bytes;
+  return CHS_OK;
+}
+
+int chs_d2h(chs_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return CHS_ERR_INVALID_ARG;
+  if (!bytes) return CHS_OK;
+  CHS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += bytes;
+  return CHS_OK;
+}
+
+int chs_d2d(chs_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes)) return CHS_ERR_INVALID_ARG;
+  if (!bytes) return CHS_OK;
+  CHS_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return CHS_OK;
+}
+
+int chs_memset(chs_context* ctx, void* dptr, int value, size_t bytes) {
+  if (!ctx || (!dptr && bytes)) return CHS_ERR_INVALID_ARG;
+  if (!bytes) return CHS_OK;
+  CHS_HIP(ctx, hipMemsetAsync(dptr, value, bytes, ctx->stream));
+  return CHS_OK;
+}
+
+int chs_spectral_bounds(const double* eig, int n_eig, double lambda_max, double* out4) {
+  // src/eigensolver/chebyshev.rs:214-254
+  if (!out4 || (n_eig > 0 && !eig)) return CHS_ERR_INVALID_ARG;
+  double eps_cut;
+  if (n_eig <= 0) {
+    eps_cut = lambda_max / 3.0;
+  } else {
+    const double e0 = eig[0], el = eig[n_eig - 1];
+    const double raw = el + 0.2 * (el - e0);
+    eps_cut = raw < lambda_max * 0.95 ? raw : lambda_max * 0.95;
+  }
+  out4[0] = lambda_max;
+  out4[1] = eps_cut;
+  out4[2] = 0.5 * (lambda_max + eps_cut);
+  out4[3] = 0.5 * (lambda_max - eps_cut);
+  return CHS_OK;
+}
+
+}  // extern "C"

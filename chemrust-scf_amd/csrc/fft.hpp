@@ -1,0 +1,222 @@
+// Register codelets and LDS-staged line FFTs for gfx950.
+//
+// A length-N line transform (N = N1 * N2) runs as two passes over LDS:
+//   pass 1: per (line, n2) one thread loads x[n1*N2 + n2] (n1 < N1) into
+//           registers, does a DFT_N1, multiplies by w_N^{n2*k1} and stores
+//           back in place;
+//   pass 2: per (line, k1) one thread loads x[k1*N2 + n2] (n2 < N2), does a
+//           DFT_N2 and stores X[k1 + N1*k2] (natural order).
+// Codelets DFT_R (R = 2, 3, 4 directly; 6, 8, 9, 12, 16 as R = A*B
+// Cooley-Tukey with constant twiddles) are fully unrolled in registers.
+// S = +1 is the inverse transform e^{+i 2 pi jk/N}, S = -1 the forward
+// e^{-i 2 pi jk/N}; neither is normalised.
+#pragma once
+#include "common.hpp"
+#include "roots.inc"
+
+namespace chs {
+
+template <int R> struct Roots;
+#define CHS_ROOTS(R)                                                                     \
+  template <> struct Roots<R> {                                                          \
+    static __device__ __forceinline__ double re(int j) { return kRootRe##R[j]; }         \
+    static __device__ __forceinline__ double im(int j) { return kRootIm##R[j]; }         \
+  };
+CHS_ROOTS(6)
+CHS_ROOTS(8)
+CHS_ROOTS(9)
+CHS_ROOTS(12)
+CHS_ROOTS(16)
+#undef CHS_ROOTS
+
+// i * S * z
+template <int S, typename T> __device__ __forceinline__ cplx<T> rot_i(cplx<T> z) {
+  return S > 0 ? mk(-z.y, z.x) : mk(z.y, -z.x);
+}
+
+template <int R, int S, typename T> struct Dft;
+
+template <int S, typename T> struct Dft<1, S, T> {
+  static __device__ __forceinline__ void run(cplx<T>* a) {}
+};
+
+template <int S, typename T> struct Dft<2, S, T> {
+  static __device__ __forceinline__ void run(cplx<T>* a) {
+    cplx<T> t = a[0];
+    a[0] = t + a[1];
+    a[1] = t - a[1];
+  }
+};
+
+template <int S, typename T> struct Dft<3, S, T> {
+  static __device__ __forceinline__ void run(cplx<T>* a) {
+    const T h = (T)0.8660254037844386;  // sqrt(3)/2
+    cplx<T> t1 = a[1] + a[2];
+    cplx<T> t2 = mk(a[0].x - (T)0.5 * t1.x, a[0].y - (T)0.5 * t1.y);
+    cplx<T> d = a[1] - a[2];
+    cplx<T> t3 = rot_i<S>(mk(h * d.x, h * d.y));
+    a[0] = a[0] + t1;
+    a[1] = t2 + t3;
+    a[2] = t2 - t3;
+  }
+};
+
+template <int S, typename T> struct Dft<4, S, T> {
+  static __device__ __forceinline__ void run(cplx<T>* a) {
+    cplx<T> s02 = a[0] + a[2], d02 = a[0] - a[2];
+    cplx<T> s13 = a[1] + a[3], d13 = rot_i<S>(a[1] - a[3]);
+    a[0] = s02 + s13;
+    a[2] = s02 - s13;
+    a[1] = d02 + d13;
+    a[3] = d02 - d13;
+  }
+};
+
+// R = A * B: X[k1 + A*k2] = sum_{n2} w_B^{n2 k2} w_R^{n2 k1} sum_{n1} w_A^{n1 k1} x[B n1 + n2]
+template <int A, int B, int S, typename T> struct DftCT {
+  static __device__ __forceinline__ void run(cplx<T>* a) {
+    constexpr int R = A * B;
+    cplx<T> y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+      cplx<T> t[A];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) t[n1] = a[B * n1 + n2];
+      Dft<A, S, T>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < A; ++k1) {
+        const int e = (n2 * k1) % R;
+        if (e == 0) {
+          y[k1 * B + n2] = t[k1];
+        } else {
+          const cplx<T> w = mk((T)Roots<R>::re(e), (T)(S * Roots<R>::im(e)));
+          y[k1 * B + n2] = t[k1] * w;
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      cplx<T> t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = y[k1 * B + n2];
+      Dft<B, S, T>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) a[k1 + A * k2] = t[k2];
+    }
+  }
+};
+
+template <int S, typename T> struct Dft<6, S, T> : DftCT<2, 3, S, T> {};
+template <int S, typename T> struct Dft<8, S, T> : DftCT<2, 4, S, T> {};
+template <int S, typename T> struct Dft<9, S, T> : DftCT<3, 3, S, T> {};
+template <int S, typename T> struct Dft<12, S, T> : DftCT<4, 3, S, T> {};
+template <int S, typename T> struct Dft<16, S, T> : DftCT<4, 4, S, T> {};
+
+// Factorisation N = N1 * N2 used for each supported line length.
+template <int N> struct Split;
+#define CHS_SPLIT(N, A, B)            \
+  template <> struct Split<N> {       \
+    static constexpr int N1 = A;      \
+    static constexpr int N2 = B;      \
+  };
+CHS_SPLIT(8, 2, 4)
+CHS_SPLIT(12, 3, 4)
+CHS_SPLIT(16, 4, 4)
+CHS_SPLIT(18, 2, 9)
+CHS_SPLIT(24, 4, 6)
+CHS_SPLIT(32, 4, 8)
+CHS_SPLIT(36, 4, 9)
+CHS_SPLIT(48, 4, 12)
+CHS_SPLIT(64, 8, 8)
+CHS_SPLIT(72, 8, 9)
+CHS_SPLIT(96, 8, 12)
+CHS_SPLIT(108, 12, 9)
+CHS_SPLIT(128, 8, 16)
+CHS_SPLIT(144, 12, 12)
+#undef CHS_SPLIT
+
+// Twiddle w_N^t = e^{S 2 pi i t / N}; tw holds the S = +1 table (t < N).
+template <int S, typename T>
+__device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int t) {
+  cplx<T> w = tw[t];
+  return S > 0 ? w : mk(w.x, -w.y);
+}
+
+// FFT of `nlines` lines of length N held in LDS. Element j of line l lives at
+// lds[off(l) + j * es]. Called by every thread of the block (has barriers);
+// starts and ends with a barrier-consistent state. tw: length-N root table
+// (global memory or LDS).
+template <int N, int S, typename T, typename Off>
+__device__ __forceinline__ void lds_fft_lines(cplx<T>* lds, int nlines, Off off, int es,
+                                              const cplx<T>* tw) {
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  const int nt = blockDim.x;
+  __syncthreads();
+  for (int t = threadIdx.x; t < nlines * N2; t += nt) {
+    const int l = t / N2, n2 = t - l * N2;
+    cplx<T>* base = lds + off(l) + n2 * es;
+    cplx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = base[n1 * N2 * es];
+    Dft<N1, S, T>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * twiddle<S>(tw, n2 * k1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2 * es] = a[k1];
+  }
+  __syncthreads();
+  // pass 2 in rounds of whole lines, so that no round overwrites inputs of
+  // a later round of the same line
+  const int lines_per_round = nt / N1 > 0 ? nt / N1 : 1;
+  for (int l0 = 0; l0 < nlines; l0 += lines_per_round) {
+    const int t = threadIdx.x;
+    const int l = l0 + t / N1, k1 = t % N1;
+    const bool act = (t < lines_per_round * N1) && (l < nlines);
+    cplx<T> a[N2];
+    cplx<T>* base = act ? lds + off(l) : lds;
+    if (act) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) a[n2] = base[(k1 * N2 + n2) * es];
+      Dft<N2, S, T>::run(a);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) base[(k1 + N1 * k2) * es] = a[k2];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+}
+
+// Dispatch a functor templated on the supported line length N.
+#define CHS_DISPATCH_N(n, ...)                 \
+  switch (n) {                                  \
+    case 8: { constexpr int NN = 8; __VA_ARGS__; } break;     \
+    case 12: { constexpr int NN = 12; __VA_ARGS__; } break;   \
+    case 16: { constexpr int NN = 16; __VA_ARGS__; } break;   \
+    case 18: { constexpr int NN = 18; __VA_ARGS__; } break;   \
+    case 24: { constexpr int NN = 24; __VA_ARGS__; } break;   \
+    case 32: { constexpr int NN = 32; __VA_ARGS__; } break;   \
+    case 36: { constexpr int NN = 36; __VA_ARGS__; } break;   \
+    case 48: { constexpr int NN = 48; __VA_ARGS__; } break;   \
+    case 64: { constexpr int NN = 64; __VA_ARGS__; } break;   \
+    case 72: { constexpr int NN = 72; __VA_ARGS__; } break;   \
+    case 96: { constexpr int NN = 96; __VA_ARGS__; } break;   \
+    case 108: { constexpr int NN = 108; __VA_ARGS__; } break; \
+    case 128: { constexpr int NN = 128; __VA_ARGS__; } break; \
+    case 144: { constexpr int NN = 144; __VA_ARGS__; } break; \
+    default: return set_error(ctx, CHS_ERR_FFT, "unsupported FFT length %d", (int)(n)); \
+  }
+
+inline bool fft_length_supported(int n) {
+  switch (n) {
+    case 8: case 12: case 16: case 18: case 24: case 32: case 36: case 48:
+    case 64: case 72: case 96: case 108: case 128: case 144:
+      return true;
+    default:
+      return false;
+  }
+}
+
+}  // namespace chs

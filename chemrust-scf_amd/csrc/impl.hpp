@@ -1,0 +1,82 @@
+// Context internals shared by the translation units of libchs_gpu.
+#pragma once
+#include <map>
+#include <utility>
+
+#include "common.hpp"
+
+namespace chs {
+
+// Column ("stick") decomposition of the plane-wave sphere: one stick per
+// (ix, iy) grid column that holds at least one plane wave. Plane waves are
+// stored lexicographically in (h, k, l), so the plane waves of a stick are a
+// contiguous index range with consecutive l.
+struct StickTable {
+  int n_st = 0;    // sticks
+  int n_act = 0;   // distinct ix among sticks ("active" x columns)
+  int n_grid = 0;
+  int n_pw = 0;
+  // device arrays (int32), length n_st unless noted
+  int32_t* iy = nullptr;     // FFT y index of the stick
+  int32_t* act = nullptr;    // index of the stick's ix in the active list
+  int32_t* pw0 = nullptr;    // first plane wave of the stick
+  int32_t* cnt = nullptr;    // plane waves in the stick
+  int32_t* iz0 = nullptr;    // FFT z index of the first plane wave
+  int32_t* act_ix = nullptr; // n_act: ix of each active column
+  std::vector<int32_t> h_act_ix;
+};
+
+struct Hamiltonian {
+  bool bound = false;
+  int n_grid = 0, n_pw = 0;
+  const int32_t* miller = nullptr;
+  const double* kin = nullptr;
+  const double* veff = nullptr;
+  DevBuf veff_f;      // float copy of V_eff for the c64 path
+  bool veff_f_valid = false;
+  StickTable st;
+  DevBuf st_mem;
+  // projectors
+  int n_proj = 0;
+  const double* beta = nullptr;  // c128, P x n_pw
+  const double* dion = nullptr;  // P
+  const double* qion = nullptr;  // P
+  bool has_s = false;
+  DevBuf beta_f;                 // c64 copy
+  DevBuf sinv_m;                 // c128 P x P (Q^-1 + beta^H beta)^-1
+  DevBuf sinv_m_f;               // c64 copy
+};
+
+}  // namespace chs
+
+struct chs_context::Impl {
+  std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
+  chs::Hamiltonian ham;
+  // workspaces
+  chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
+  chs::DevBuf gemm_ws, lin_ws, scf_ws;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace chs {
+
+// Length-N root table e^{+2 pi i t/N} (t < N) in the given precision.
+int twiddles(chs_context* ctx, int n, int precision, const void** out);
+// Batched in-place full-grid 3D FFT (fftgrid.hip).
+int fft3d(chs_context* ctx, int precision, void* grid, int n, int batch, int sign);
+// Stick tables for the bound plane-wave set.
+int build_sticks(chs_context* ctx);
+
+// Local-potential H|psi> pipeline pieces (hamiltonian.hip).
+int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
+                  const void* aux0, const void* aux1, double c, double inv_e, double* d_norm_part,
+                  int* n_part);
+
+// Modes of the stick epilogue.
+enum EpiMode {
+  EPI_HPSI = 0,    // out = T psi + V psi
+  EPI_CHEB1 = 1,   // out = (H psi - c psi) * inv_e
+  EPI_CHEBK = 2,   // out = 2 (H psi - c psi) * inv_e - aux0
+};
+
+}  // namespace chs

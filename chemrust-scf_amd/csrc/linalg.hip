@@ -1,0 +1,618 @@
+// FP64 dense linear algebra on the device: complex GEMM (split-K for the
+// tall-skinny Gram products), Hermitisation, Cholesky, triangular inverse,
+// a parallel cyclic Jacobi Hermitian eigensolver, and the Rayleigh-Ritz /
+// Cholesky-QR drivers built from them.
+//
+// Reference: rayleigh_ritz (src/eigensolver/rayleigh_ritz.rs:48-245) calls
+// cuBLAS ZGEMM for H_sub and S_sub and cuSOLVER ZHEGVD; the semantics
+// matched here are diagonalize_and_rotate (src/eigensolver/
+// subspace_diag.rs:229-391): S = L L^H, B = L^{-1} H L^{-H} (Hermitised),
+// B U = U diag(eps) ascending, V = L^{-H} U, psi <- psi V.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "blas1.hpp"
+#include "impl.hpp"
+#include "linalg.hpp"
+#include "ops.hpp"
+
+namespace chs {
+
+// ---------------------------------------------------------------------------
+// ZGEMM: C = alpha op(A) op(B) + beta C, column-major, op in {N, T, C}.
+// 64 x 64 output tile per block of 256 threads (4 x 4 per thread), K tiles
+// of 16 staged in LDS. With ksplit > 1 every z-slice writes its partial
+// tile to `part` and k_zgemm_reduce combines them in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+template <int OP>
+__device__ __forceinline__ z64 load_op(const z64* __restrict__ A, int lda, int r, int c) {
+  // element (r, c) of op(A)
+  if (OP == 0) return A[(size_t)r + (size_t)c * lda];
+  z64 v = A[(size_t)c + (size_t)r * lda];
+  return OP == 2 ? conj(v) : v;
+}
+
+template <int OPA, int OPB>
+__global__ void __launch_bounds__(256) k_zgemm(int m, int n, int k, z64 alpha, const z64* __restrict__ A,
+                                               int lda, const z64* __restrict__ B, int ldb, z64 beta,
+                                               z64* __restrict__ C, int ldc, int klen,
+                                               z64* __restrict__ part) {
+  __shared__ z64 As[GBK][GBM + 1];
+  __shared__ z64 Bs[GBK][GBN + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+  const int kb = blockIdx.z * klen, ke = min(k, kb + klen);
+  z64 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0, 0.0);
+  for (int k0 = kb; k0 < ke; k0 += GBK) {
+    // A tile: GBM x GBK of op(A); B tile: GBK x GBN of op(B)
+    for (int t = threadIdx.x; t < GBM * GBK; t += 256) {
+      int r, kk;
+      if (OPA == 0) { r = t % GBM; kk = t / GBM; } else { kk = t % GBK; r = t / GBK; }
+      const int gr = m0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < m && gk < ke) ? load_op<OPA>(A, lda, gr, gk) : mk(0.0, 0.0);
+    }
+    for (int t = threadIdx.x; t < GBK * GBN; t += 256) {
+      int c, kk;
+      if (OPB == 0) { kk = t % GBK; c = t / GBK; } else { c = t % GBN; kk = t / GBN; }
+      const int gc = n0 + c, gk = k0 + kk;
+      Bs[kk][c] = (gc < n && gk < ke) ? load_op<OPB>(B, ldb, gk, gc) : mk(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; ++kk) {
+      z64 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[i][j].x = fma(a[i].x, b[j].x, acc[i][j].x);
+          acc[i][j].x = fma(-a[i].y, b[j].y, acc[i][j].x);
+          acc[i][j].y = fma(a[i].x, b[j].y, acc[i][j].y);
+          acc[i][j].y = fma(a[i].y, b[j].x, acc[i][j].y);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = m0 + tx + 16 * i, c = n0 + ty + 16 * j;
+      if (r < m && c < n) {
+        if (part) {
+          part[(size_t)blockIdx.z * m * n + (size_t)r + (size_t)c * m] = acc[i][j];
+        } else {
+          z64 v = alpha * acc[i][j];
+          if (beta.x != 0.0 || beta.y != 0.0) v = v + beta * C[(size_t)r + (size_t)c * ldc];
+          C[(size_t)r + (size_t)c * ldc] = v;
+        }
+      }
+    }
+}
+
+__global__ void k_zgemm_reduce(int m, int n, int nz, const z64* __restrict__ part, z64 alpha,
+                               z64 beta, z64* __restrict__ C, int ldc) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)m * n) return;
+  z64 s = mk(0.0, 0.0);
+  for (int z = 0; z < nz; ++z) s = s + part[(size_t)z * m * n + idx];
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  z64 v = alpha * s;
+  if (beta.x != 0.0 || beta.y != 0.0) v = v + beta * C[(size_t)r + (size_t)c * ldc];
+  C[(size_t)r + (size_t)c * ldc] = v;
+}
+
+static int opcode(char op) {
+  switch (op) {
+    case 'N': case 'n': return 0;
+    case 'T': case 't': return 1;
+    case 'C': case 'c': return 2;
+    default: return -1;
+  }
+}
+
+int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
+          int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc) {
+  const int oa = opcode(opa), ob = opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (m == 0 || n == 0) return CHS_OK;
+  const int tiles = (int)(ceil_div(m, GBM) * ceil_div(n, GBN));
+  // split K when the output tiles alone cannot fill the device
+  int ksplit = 1;
+  if (tiles < 512 && k > 4 * GBK) {
+    ksplit = std::min(512 / tiles + 1, (k + 255) / 256);
+    ksplit = std::max(ksplit, 1);
+  }
+  int klen = (k + ksplit - 1) / ksplit;
+  klen = ((klen + GBK - 1) / GBK) * GBK;
+  if (klen == 0) klen = GBK;
+  ksplit = std::max(1, (k + klen - 1) / klen);
+  z64* part = nullptr;
+  if (ksplit > 1) {
+    CHS_TRY(ensure(ctx, ctx->impl->gemm_ws, (size_t)ksplit * m * n * sizeof(z64)));
+    part = (z64*)ctx->impl->gemm_ws.p;
+  }
+  dim3 grid(ceil_div(m, GBM), ceil_div(n, GBN), ksplit);
+#define CHS_GEMM_CASE(OA, OB)                                                                \
+  if (oa == OA && ob == OB)                                                                  \
+    hipLaunchKernelGGL((k_zgemm<OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k, alpha, A, \
+                       lda, B, ldb, beta, C, ldc, klen, part);
+  CHS_GEMM_CASE(0, 0) CHS_GEMM_CASE(0, 1) CHS_GEMM_CASE(0, 2)
+  CHS_GEMM_CASE(1, 0) CHS_GEMM_CASE(1, 1) CHS_GEMM_CASE(1, 2)
+  CHS_GEMM_CASE(2, 0) CHS_GEMM_CASE(2, 1) CHS_GEMM_CASE(2, 2)
+#undef CHS_GEMM_CASE
+  CHS_LAUNCH_CHECK(ctx);
+  if (ksplit > 1) {
+    hipLaunchKernelGGL(k_zgemm_reduce, dim3(ceil_div((size_t)m * n, 256)), dim3(256), 0,
+                       ctx->stream, m, n, ksplit, part, alpha, beta, C, ldc);
+    CHS_LAUNCH_CHECK(ctx);
+  }
+  return CHS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Small dense kernels on n x n column-major matrices
+// ---------------------------------------------------------------------------
+
+// M <- (M + M^H) / 2 (subspace_diag.rs:117-125)
+__global__ void k_hermitise(z64* __restrict__ M, int n) {
+  const int j = blockIdx.x, i0 = threadIdx.x;
+  for (int i = i0; i <= j; i += blockDim.x) {
+    z64 a = M[(size_t)i + (size_t)j * n], b = M[(size_t)j + (size_t)i * n];
+    z64 v = mk(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    if (i == j) v.y = 0.0;
+    M[(size_t)i + (size_t)j * n] = v;
+    M[(size_t)j + (size_t)i * n] = conj(v);
+  }
+}
+
+// Right-looking lower Cholesky in one block; info[0] = failing column + 1.
+__global__ void __launch_bounds__(1024) k_cholesky(z64* __restrict__ A, int n, int* __restrict__ info) {
+  __shared__ double piv;
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (threadIdx.x == 0) {
+      const double d = A[(size_t)j + (size_t)j * n].x;
+      if (!(d > 0.0)) {
+        bad = j + 1;
+      } else {
+        piv = sqrt(d);
+        A[(size_t)j + (size_t)j * n] = mk(piv, 0.0);
+      }
+    }
+    __syncthreads();
+    if (bad) break;
+    const double inv = 1.0 / piv;
+    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
+      z64 v = A[(size_t)i + (size_t)j * n];
+      A[(size_t)i + (size_t)j * n] = mk(v.x * inv, v.y * inv);
+    }
+    __syncthreads();
+    // trailing lower triangle: A[i][c] -= L[i][j] conj(L[c][j]) for j < c <= i
+    const int m = n - j - 1;
+    const long tot = (long)m * (m + 1) / 2;
+    for (long t = threadIdx.x; t < tot; t += blockDim.x) {
+      // map t -> (ii, cc) with 0 <= cc <= ii < m, column-major over cc
+      int cc = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
+      while ((long)cc * (cc + 1) / 2 > t) --cc;
+      while ((long)(cc + 1) * (cc + 2) / 2 <= t) ++cc;
+      const int rem = (int)(t - (long)cc * (cc + 1) / 2);
+      // (cc, rem) enumerates a lower triangle by rows: row = cc, col = rem
+      const int i = j + 1 + cc, c = j + 1 + rem;
+      const z64 lij = A[(size_t)i + (size_t)j * n], lcj = A[(size_t)c + (size_t)j * n];
+      z64 a = A[(size_t)i + (size_t)c * n];
+      a.x -= lij.x * lcj.x + lij.y * lcj.y;
+      a.y -= lij.y * lcj.x - lij.x * lcj.y;
+      A[(size_t)i + (size_t)c * n] = a;
+    }
+    __syncthreads();
+  }
+  // zero the strict upper triangle
+  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
+    const int i = (int)(t % n), c = (int)(t / n);
+    if (i < c) A[t] = mk(0.0, 0.0);
+  }
+  if (threadIdx.x == 0) info[0] = bad;
+}
+
+// X = L^{-1} (lower), rows in sequence, columns in parallel.
+__global__ void __launch_bounds__(1024) k_tri_inv(const z64* __restrict__ L, z64* __restrict__ X, int n) {
+  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) X[t] = mk(0.0, 0.0);
+  __syncthreads();
+  for (int i = 0; i < n; ++i) {
+    const z64 lii = L[(size_t)i + (size_t)i * n];
+    const double inv = 1.0 / lii.x;  // real positive diagonal
+    for (int c = threadIdx.x; c <= i; c += blockDim.x) {
+      z64 s = (c == i) ? mk(1.0, 0.0) : mk(0.0, 0.0);
+      for (int k = c; k < i; ++k) s = s - L[(size_t)i + (size_t)k * n] * X[(size_t)k + (size_t)c * n];
+      X[(size_t)i + (size_t)c * n] = mk(s.x * inv, s.y * inv);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Parallel cyclic Jacobi (round-robin ordering). Per round, k_jacobi_cols
+// computes each pair's rotation U = [[c, s e], [-s e*, c]] from its own
+// 2 x 2 block and applies A <- A U and V <- V U on columns p, q;
+// k_jacobi_rows then applies A <- U^H A on rows p, q.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_jacobi_cols(z64* __restrict__ A, z64* __restrict__ V, int n,
+                                                     const int2* __restrict__ pairs,
+                                                     double4* __restrict__ rot) {
+  const int2 pq = pairs[blockIdx.x];
+  const int p = pq.x, q = pq.y;
+  __shared__ double4 r;
+  if (threadIdx.x == 0) {
+    double c = 1.0, s = 0.0, er = 1.0, ei = 0.0;
+    if (q < n && p < n) {
+      const z64 apq = A[(size_t)p + (size_t)q * n];
+      const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
+      const double app = A[(size_t)p + (size_t)p * n].x, aqq = A[(size_t)q + (size_t)q * n].x;
+      if (mag > 1e-300 && mag > 1e-18 * sqrt(fabs(app * aqq))) {
+        er = apq.x / mag;
+        ei = apq.y / mag;
+        const double tau = (aqq - app) / (2.0 * mag);
+        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = t * c;
+      }
+    }
+    r = make_double4(c, s, er, ei);
+    rot[blockIdx.x] = r;
+  }
+  __syncthreads();
+  if (p >= n || q >= n || r.y == 0.0) return;
+  const double c = r.x, s = r.y;
+  const z64 e = mk(r.z, r.w), ec = mk(r.z, -r.w);
+  z64* cp = A + (size_t)p * n;
+  z64* cq = A + (size_t)q * n;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const z64 akp = cp[k], akq = cq[k];
+    cp[k] = c * akp - s * (ec * akq);
+    cq[k] = s * (e * akp) + c * akq;
+  }
+  z64* vp = V + (size_t)p * n;
+  z64* vq = V + (size_t)q * n;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const z64 vkp = vp[k], vkq = vq[k];
+    vp[k] = c * vkp - s * (ec * vkq);
+    vq[k] = s * (e * vkp) + c * vkq;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_jacobi_rows(z64* __restrict__ A, int n,
+                                                     const int2* __restrict__ pairs,
+                                                     const double4* __restrict__ rot) {
+  const int2 pq = pairs[blockIdx.x];
+  const int p = pq.x, q = pq.y;
+  const double4 r = rot[blockIdx.x];
+  if (p >= n || q >= n || r.y == 0.0) return;
+  const double c = r.x, s = r.y;
+  const z64 e = mk(r.z, r.w), ec = mk(r.z, -r.w);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const z64 apk = A[(size_t)p + (size_t)k * n], aqk = A[(size_t)q + (size_t)k * n];
+    A[(size_t)p + (size_t)k * n] = c * apk - s * (e * aqk);
+    A[(size_t)q + (size_t)k * n] = s * (ec * apk) + c * aqk;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A[(size_t)p + (size_t)q * n] = mk(0.0, 0.0);
+    A[(size_t)q + (size_t)p * n] = mk(0.0, 0.0);
+    A[(size_t)p + (size_t)p * n].y = 0.0;
+    A[(size_t)q + (size_t)q * n].y = 0.0;
+  }
+}
+
+// out[0] = sum_{i != j} |A_ij|^2, out[1] = sum |A_ij|^2 (one block).
+__global__ void k_offnorm(const z64* __restrict__ A, int n, double* __restrict__ out) {
+  __shared__ double sh[8];
+  double off = 0, all = 0;
+  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
+    const int i = (int)(t % n), j = (int)(t / n);
+    const double v = norm2(A[t]);
+    all += v;
+    if (i != j) off += v;
+  }
+  double a = block_sum(off, sh);
+  double b = block_sum(all, sh);
+  if (threadIdx.x == 0) {
+    out[0] = a;
+    out[1] = b;
+  }
+}
+
+__global__ void k_set_identity(z64* __restrict__ V, int n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < (size_t)n * n) V[t] = mk((t % n) == (t / n) ? 1.0 : 0.0, 0.0);
+}
+
+// Ascending order of the diagonal (ties by index); eigenvalues to w, columns
+// of V permuted into U.
+__global__ void __launch_bounds__(1024) k_sort_eig(const z64* __restrict__ A, const z64* __restrict__ V,
+                                                   int n, double* __restrict__ w,
+                                                   z64* __restrict__ U) {
+  __shared__ int rank_of[1024];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double di = A[(size_t)i + (size_t)i * n].x;
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = A[(size_t)j + (size_t)j * n].x;
+      r += (dj < di) || (dj == di && j < i);
+    }
+    rank_of[i] = r;
+    w[r] = di;
+  }
+  __syncthreads();
+  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
+    const int k = (int)(t % n), col = (int)(t / n);
+    U[(size_t)k + (size_t)rank_of[col] * n] = V[t];
+  }
+}
+
+struct JacobiPlan {
+  int n = 0;
+  int rounds = 0, pairs = 0;
+  DevBuf sched;  // int2 [rounds][pairs]
+  DevBuf rot;    // double4 [pairs]
+};
+static JacobiPlan g_plan;  // guarded per context device by rebuild on size change
+
+static int jacobi_plan(chs_context* ctx, int n, JacobiPlan& P) {
+  if (P.n == n && P.sched.p) return CHS_OK;
+  const int m = (n % 2) ? n + 1 : n;  // add a dummy index for odd n
+  P.n = n;
+  P.rounds = m - 1;
+  P.pairs = m / 2;
+  std::vector<int> ring(m);
+  std::iota(ring.begin(), ring.end(), 0);
+  std::vector<int2> s((size_t)P.rounds * P.pairs);
+  for (int r = 0; r < P.rounds; ++r) {
+    for (int k = 0; k < P.pairs; ++k) {
+      int a = ring[k], b = ring[m - 1 - k];
+      if (a > b) std::swap(a, b);
+      s[(size_t)r * P.pairs + k] = make_int2(a, b);  // b == n marks the dummy
+    }
+    // circle method: keep ring[0], rotate the rest
+    const int last = ring[m - 1];
+    for (int k = m - 1; k > 1; --k) ring[k] = ring[k - 1];
+    ring[1] = last;
+  }
+  CHS_TRY(ensure(ctx, P.sched, s.size() * sizeof(int2)));
+  CHS_HIP(ctx, hipMemcpyAsync(P.sched.p, s.data(), s.size() * sizeof(int2), hipMemcpyHostToDevice,
+                              ctx->stream));
+  CHS_TRY(ensure(ctx, P.rot, (size_t)P.pairs * sizeof(double4)));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CHS_OK;
+}
+
+// Hermitian eigensolver: A (n x n) is destroyed; eigenvalues ascending in
+// device w, eigenvectors in U. Work: V (n x n).
+int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int max_sweeps,
+                int* sweeps_out) {
+  JacobiPlan& P = g_plan;
+  CHS_TRY(jacobi_plan(ctx, n, P));
+  hipLaunchKernelGGL(k_set_identity, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0, ctx->stream,
+                     V, n);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
+  double* dn = (double*)ctx->impl->small.p + 48;
+  int sweeps = 0;
+  for (; sweeps < max_sweeps; ++sweeps) {
+    hipLaunchKernelGGL(k_offnorm, dim3(1), dim3(256), 0, ctx->stream, A, n, dn);
+    CHS_LAUNCH_CHECK(ctx);
+    double h[2];
+    CHS_HIP(ctx, hipMemcpyAsync(h, dn, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (h[0] <= 1e-30 * h[1] || h[0] == 0.0) break;
+    for (int r = 0; r < P.rounds; ++r) {
+      const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
+      hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, ctx->stream, A, V, n, pr,
+                         (double4*)P.rot.p);
+      hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, ctx->stream, A, n, pr,
+                         (const double4*)P.rot.p);
+    }
+    CHS_LAUNCH_CHECK(ctx);
+  }
+  if (sweeps_out) *sweeps_out = sweeps;
+  if (n > 1024) return set_error(ctx, CHS_ERR_SOLVER, "eigensolver limited to n <= 1024");
+  hipLaunchKernelGGL(k_sort_eig, dim3(1), dim3(1024), 0, ctx->stream, A, V, n, w, U);
+  CHS_LAUNCH_CHECK(ctx);
+  if (sweeps == max_sweeps)
+    return set_error(ctx, CHS_ERR_SOLVER, "Jacobi eigensolver did not converge in %d sweeps",
+                     max_sweeps);
+  return CHS_OK;
+}
+
+int cholesky(chs_context* ctx, z64* A, int n, int* info_host) {
+  if (n > 4096) return set_error(ctx, CHS_ERR_INVALID_ARG, "Cholesky limited to n <= 4096");
+  CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
+  int* dinfo = (int*)((double*)ctx->impl->small.p + 56);
+  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), 0, ctx->stream, A, n, dinfo);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_HIP(ctx, hipMemcpyAsync(info_host, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CHS_OK;
+}
+
+int tri_inv(chs_context* ctx, const z64* L, z64* X, int n) {
+  hipLaunchKernelGGL(k_tri_inv, dim3(1), dim3(1024), 0, ctx->stream, L, X, n);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int hermitise(chs_context* ctx, z64* M, int n) {
+  hipLaunchKernelGGL(k_hermitise, dim3(n), dim3(256), 0, ctx->stream, M, n);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+// Generalized (B != nullptr) or standard Hermitian eigensolve of device
+// matrices (n x n column-major); eigenvectors overwrite A, eigenvalues in
+// device w. B is overwritten by its Cholesky factor.
+int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w) {
+  auto* im = ctx->impl;
+  const size_t nn = (size_t)n * n;
+  CHS_TRY(ensure(ctx, im->lin_ws, 4 * nn * sizeof(z64)));
+  z64* Li = (z64*)im->lin_ws.p;
+  z64* T = Li + nn;
+  z64* U = T + nn;
+  z64* V = U + nn;
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  if (B) {
+    int info = 0;
+    CHS_TRY(cholesky(ctx, B, n, &info));
+    if (info) {
+      ctx->diag.solver_info = info;
+      return set_error(ctx, CHS_ERR_RAYLEIGH_RITZ_FAILED,
+                       "Cholesky of S_sub failed at column %d (not positive definite)", info);
+    }
+    CHS_TRY(tri_inv(ctx, B, Li, n));
+    // T = Li A ; A <- T Li^H
+    CHS_TRY(zgemm(ctx, 'N', 'N', n, n, n, one, Li, n, A, n, zero, T, n));
+    CHS_TRY(zgemm(ctx, 'N', 'C', n, n, n, one, T, n, Li, n, zero, A, n));
+    CHS_TRY(hermitise(ctx, A, n));
+  }
+  int sweeps = 0;
+  CHS_TRY(heev_jacobi(ctx, A, n, w, U, V, 60, &sweeps));
+  if (B) {
+    // A <- Li^H U
+    CHS_TRY(zgemm(ctx, 'C', 'N', n, n, n, one, Li, n, U, n, zero, A, n));
+  } else {
+    CHS_HIP(ctx, hipMemcpyAsync(A, U, nn * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return CHS_OK;
+}
+
+// Rayleigh-Ritz on a c128 block Y (n_b x n_pw band-major): see file header.
+int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
+                       double* eps_dev_out) {
+  CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const int n_pw = H.n_pw;
+  const size_t ne = (size_t)n_b * n_pw;
+  CHS_TRY(ensure(ctx, im->ws4, ne * sizeof(z64)));
+  CHS_TRY(ensure(ctx, im->ws5, ne * sizeof(z64)));
+  z64* HY = (z64*)im->ws4.p;
+  z64* SY = (z64*)im->ws5.p;
+  const bool gen = H.has_s;
+  CHS_TRY(apply_h_impl(ctx, CHS_C128, Y, n_b, HY, gen ? SY : nullptr));
+  const size_t nn = (size_t)n_b * n_b;
+  CHS_TRY(ensure(ctx, im->scf_ws, (2 * nn + n_b) * sizeof(z64) + 64));
+  z64* Hs = (z64*)im->scf_ws.p;
+  z64* Ss = Hs + nn;
+  double* w = (double*)(Ss + nn);
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, HY, n_pw, zero, Hs, n_b));
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen ? SY : Y, n_pw, zero, Ss, n_b));
+  CHS_TRY(hermitise(ctx, Hs, n_b));
+  CHS_TRY(hermitise(ctx, Ss, n_b));
+  CHS_TRY(zhegv(ctx, Hs, Ss, n_b, w));
+  // Y <- Y V (V in Hs); HY <- HY V
+  CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, one, Y, n_pw, Hs, n_b, zero, SY, n_pw));
+  CHS_HIP(ctx, hipMemcpyAsync(Y, SY, ne * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
+  if (HY_out) CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, one, HY, n_pw, Hs, n_b, zero, HY_out, n_pw));
+  if (eps_dev_out)
+    CHS_HIP(ctx, hipMemcpyAsync(eps_dev_out, w, n_b * sizeof(double), hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  if (eps_host) {
+    CHS_HIP(ctx, hipMemcpyAsync(eps_host, w, n_b * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->d2h += n_b * sizeof(double);
+  }
+  return CHS_OK;
+}
+
+// Y <- Y L^{-H} with S_sub = Y^H S Y = L L^H.
+int cholqr_impl(chs_context* ctx, z64* Y, int n_b) {
+  CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const int n_pw = H.n_pw;
+  const size_t ne = (size_t)n_b * n_pw, nn = (size_t)n_b * n_b;
+  CHS_TRY(ensure(ctx, im->ws5, ne * sizeof(z64)));
+  z64* SY = (z64*)im->ws5.p;
+  const bool gen = H.has_s;
+  if (gen) {
+    CHS_TRY(ensure(ctx, im->ws4, ne * sizeof(z64)));
+    CHS_TRY(apply_h_impl(ctx, CHS_C128, Y, n_b, im->ws4.p, SY));
+  }
+  CHS_TRY(ensure(ctx, im->scf_ws, 2 * nn * sizeof(z64)));
+  z64* Ss = (z64*)im->scf_ws.p;
+  z64* Li = Ss + nn;
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen ? SY : Y, n_pw, zero, Ss, n_b));
+  CHS_TRY(hermitise(ctx, Ss, n_b));
+  int info = 0;
+  CHS_TRY(cholesky(ctx, Ss, n_b, &info));
+  if (info) {
+    ctx->diag.solver_info = info;
+    return set_error(ctx, CHS_ERR_RAYLEIGH_RITZ_FAILED, "Cholesky-QR failed at column %d", info);
+  }
+  CHS_TRY(tri_inv(ctx, Ss, Li, n_b));
+  CHS_TRY(zgemm(ctx, 'N', 'C', n_pw, n_b, n_b, one, Y, n_pw, Li, n_b, zero, SY, n_pw));
+  CHS_HIP(ctx, hipMemcpyAsync(Y, SY, ne * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
+  return CHS_OK;
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" {
+
+int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const double alpha[2],
+              const double* d_a, int lda, const double* d_b, int ldb, const double beta[2],
+              double* d_c, int ldc) {
+  if (!ctx || !alpha || !beta || (m > 0 && n > 0 && (!d_c || (k > 0 && (!d_a || !d_b)))))
+    return CHS_ERR_INVALID_ARG;
+  const int oa = opcode(opa), ob = opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (ldc < std::max(1, m) || lda < std::max(1, oa == 0 ? m : k) ||
+      ldb < std::max(1, ob == 0 ? k : n))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "zgemm: leading dimension too small");
+  return zgemm(ctx, opa, opb, m, n, k, mk(alpha[0], alpha[1]), (const z64*)d_a, lda,
+               (const z64*)d_b, ldb, mk(beta[0], beta[1]), (z64*)d_c, ldc);
+}
+
+int chs_zhegvd(chs_context* ctx, int n, double* d_a, double* d_b, double* w) {
+  if (!ctx || n <= 0 || !d_a || !w) return CHS_ERR_INVALID_ARG;
+  if (n > 1024) return set_error(ctx, CHS_ERR_INVALID_ARG, "zhegvd limited to n <= 1024");
+  auto* im = ctx->impl;
+  CHS_TRY(ensure(ctx, im->ws6, (size_t)n * sizeof(double) + 64));
+  double* dw = (double*)im->ws6.p;
+  CHS_TRY(zhegv(ctx, (z64*)d_a, (z64*)d_b, n, dw));
+  CHS_HIP(ctx, hipMemcpyAsync(w, dw, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += n * sizeof(double);
+  return CHS_OK;
+}
+
+int chs_rayleigh_ritz(chs_context* ctx, double* d_y, int n_b, double* eps, double* d_hy) {
+  if (!ctx || !d_y || n_b <= 0 || !eps) return CHS_ERR_INVALID_ARG;
+  if (n_b > 1024) return set_error(ctx, CHS_ERR_INVALID_ARG, "n_bands limited to 1024");
+  return rayleigh_ritz_impl(ctx, (z64*)d_y, n_b, eps, (z64*)d_hy, nullptr);
+}
+
+int chs_cholqr(chs_context* ctx, double* d_y, int n_b) {
+  if (!ctx || !d_y || n_b <= 0) return CHS_ERR_INVALID_ARG;
+  return cholqr_impl(ctx, (z64*)d_y, n_b);
+}
+
+}  // extern "C"

@@ -1,0 +1,19 @@
+// Internal FP64 dense linear algebra (linalg.hip).
+#pragma once
+#include "impl.hpp"
+
+namespace chs {
+
+int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
+          int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc);
+int hermitise(chs_context* ctx, z64* M, int n);
+int cholesky(chs_context* ctx, z64* A, int n, int* info_host);
+int tri_inv(chs_context* ctx, const z64* L, z64* X, int n);
+int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int max_sweeps,
+                int* sweeps_out);
+int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w);
+int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
+                       double* eps_dev_out);
+int cholqr_impl(chs_context* ctx, z64* Y, int n_b);
+
+}  // namespace chs

@@ -1,0 +1,485 @@
+// Operator entry points: H|psi>, the Chebyshev filter, spectral bounds and
+// the Lanczos upper bound; level-1 primitives.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "blas1.hpp"
+#include "impl.hpp"
+#include "ops.hpp"
+
+namespace chs {
+
+int ensure_precision_ready(chs_context* ctx, int precision);
+
+static size_t csize(int precision) { return precision == CHS_C128 ? 16 : 8; }
+
+// H|psi> = T psi + V_loc psi (+ nonlocal), S|psi> when spsi != nullptr.
+int apply_h_impl(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
+                 void* spsi) {
+  CHS_TRY(ensure_precision_ready(ctx, precision));
+  Hamiltonian& H = ctx->impl->ham;
+  CHS_TRY(vloc_pipeline(ctx, precision, psi, n_b, hpsi, EPI_HPSI, nullptr, nullptr, 0, 0, nullptr,
+                        nullptr));
+  if (H.n_proj > 0) {
+    CHS_TRY(nonlocal_apply(ctx, precision, psi, n_b, hpsi, spsi));
+  } else if (spsi && spsi != psi) {
+    CHS_HIP(ctx, hipMemcpyAsync(spsi, psi, (size_t)n_b * H.n_pw * csize(precision),
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return CHS_OK;
+}
+
+template <typename T>
+__global__ void k_cheb_combine(const cplx<T>* __restrict__ ah, const cplx<T>* __restrict__ p,
+                               cplx<T>* out, const cplx<T>* aux0, size_t n, T c, T inv_e,
+                               int first, double* __restrict__ part) {
+  __shared__ double sh[8];
+  double so = 0, sp = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const cplx<T> h = ah[i], x = p[i];
+    cplx<T> o;
+    if (first) {
+      o = mk((h.x - c * x.x) * inv_e, (h.y - c * x.y) * inv_e);
+    } else {
+      const cplx<T> q = aux0[i];
+      o = mk((T)2 * (h.x - c * x.x) * inv_e - q.x, (T)2 * (h.y - c * x.y) * inv_e - q.y);
+    }
+    out[i] = o;
+    so += (double)o.x * o.x + (double)o.y * o.y;
+    sp += (double)x.x * x.x + (double)x.y * x.y;
+  }
+  double a = block_sum(so, sh);
+  double b = block_sum(sp, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = a;
+    part[2 * blockIdx.x + 1] = b;
+  }
+}
+
+__global__ void k_norm_guard2(const double* __restrict__ part, int n_part, int k,
+                              double* __restrict__ flag) {
+  __shared__ double sh[8];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  double r = block_sum(a, sh);
+  double s = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    const double nc = sqrt(r), np = sqrt(s);
+    const double growth = np > 0.0 ? nc / np : 1.0;
+    if (flag[0] == 0.0 && growth > 10.0) {
+      flag[0] = k;
+      flag[1] = np;
+      flag[2] = nc;
+      flag[3] = growth;
+    }
+  }
+}
+
+// One Chebyshev step Y_out = [2] (A Y_in - c Y_in)/e [- Y_prev] with the
+// growth guard; A = H (fused epilogue) or S^{-1} H (projector path).
+static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, void* yout,
+                     const void* yprev, bool first, double c, double inv_e, int k, double* flag) {
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const size_t ne = (size_t)n_b * H.n_pw;
+  if (H.n_proj == 0) {
+    const int gz = ceil_div(H.st.n_st, 32);
+    CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * (size_t)gz * n_b + 64));
+    int n_part = 0;
+    CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
+                          nullptr, c, inv_e, (double*)im->part.p, &n_part));
+    if (!first) {
+      hipLaunchKernelGGL(k_norm_guard2, dim3(1), dim3(256), 0, ctx->stream,
+                         (const double*)im->part.p, n_part, k, flag);
+      CHS_LAUNCH_CHECK(ctx);
+    }
+    return CHS_OK;
+  }
+  // projector path: AY into a scratch block, then the combine kernel
+  CHS_TRY(ensure(ctx, im->ws3, ne * csize(precision)));
+  CHS_TRY(apply_h_impl(ctx, precision, yin, n_b, im->ws3.p, nullptr));
+  if (H.has_s) CHS_TRY(sinv_apply(ctx, precision, im->ws3.p, n_b));
+  CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * kRedBlocks + 64));
+  if (precision == CHS_C128)
+    hipLaunchKernelGGL((k_cheb_combine<double>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                       ctx->stream, (const z64*)im->ws3.p, (const z64*)yin, (z64*)yout,
+                       (const z64*)yprev, ne, c, inv_e, (int)first, (double*)im->part.p);
+  else
+    hipLaunchKernelGGL((k_cheb_combine<float>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                       ctx->stream, (const c32*)im->ws3.p, (const c32*)yin, (c32*)yout,
+                       (const c32*)yprev, ne, (float)c, (float)inv_e, (int)first,
+                       (double*)im->part.p);
+  CHS_LAUNCH_CHECK(ctx);
+  if (!first) {
+    hipLaunchKernelGGL(k_norm_guard2, dim3(1), dim3(256), 0, ctx->stream,
+                       (const double*)im->part.p, kRedBlocks, k, flag);
+    CHS_LAUNCH_CHECK(ctx);
+  }
+  return CHS_OK;
+}
+
+// Degree-m filter; reference src/eigensolver/chebyshev.rs:626-816. Buffers
+// rotate without copies: a = Y_{k-2}, b = Y_{k-1}; Y_k overwrites a.
+int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int n_b, int degree,
+                          const double* b4, void* out) {
+  CHS_TRY(ensure_precision_ready(ctx, precision));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const size_t bytes = (size_t)n_b * H.n_pw * csize(precision);
+  if (degree == 0) {
+    if (out != psi)
+      CHS_HIP(ctx, hipMemcpyAsync(out, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return CHS_OK;
+  }
+  const double c = b4[2], inv_e = 1.0 / b4[3];
+  CHS_TRY(ensure(ctx, im->ws1, bytes));
+  CHS_TRY(ensure(ctx, im->ws2, bytes));
+  CHS_TRY(ensure(ctx, im->small, 64 * sizeof(double)));
+  double* flag = (double*)im->small.p;
+  CHS_HIP(ctx, hipMemsetAsync(flag, 0, 8 * sizeof(double), ctx->stream));
+  // a <- Y0, b <- Y1
+  void* a = im->ws1.p;
+  void* b = im->ws2.p;
+  CHS_HIP(ctx, hipMemcpyAsync(a, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  CHS_TRY(cheb_step(ctx, precision, a, n_b, b, nullptr, true, c, inv_e, 1, flag));
+  for (int k = 2; k <= degree; ++k) {
+    CHS_TRY(cheb_step(ctx, precision, b, n_b, a, a, false, c, inv_e, k, flag));
+    std::swap(a, b);
+  }
+  CHS_HIP(ctx, hipMemcpyAsync(out, b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  double h_flag[4];
+  CHS_HIP(ctx, hipMemcpyAsync(h_flag, flag, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h_flag[0] != 0.0) {
+    ctx->diag.iteration = (int)h_flag[0];
+    ctx->diag.norm_previous = h_flag[1];
+    ctx->diag.norm_current = h_flag[2];
+    ctx->diag.growth_factor = h_flag[3];
+    return set_error(ctx, CHS_ERR_CHEBYSHEV_DIVERGED,
+                     "Chebyshev filter diverged at iteration %d: norm %.6e -> %.6e (x%.3f)",
+                     (int)h_flag[0], h_flag[1], h_flag[2], h_flag[3]);
+  }
+  return CHS_OK;
+}
+
+// ---- Lanczos -----------------------------------------------------------
+// f = w - alpha v - beta vp ; sf = u - alpha sv - beta svp  (S = I: sv = v)
+__global__ void k_lanczos_f(const z64* __restrict__ w, const z64* __restrict__ u,
+                            const z64* __restrict__ v, const z64* __restrict__ sv,
+                            const z64* __restrict__ vp, const z64* __restrict__ svp,
+                            const double* __restrict__ ab, int j, int gen, size_t n,
+                            z64* __restrict__ f, z64* __restrict__ sf) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double al = ab[2 * j];
+  const double be = j > 0 ? ab[2 * (j - 1) + 1] : 0.0;
+  const z64 vi = v[i], vpi = vp[i];
+  z64 fi = mk(w[i].x - al * vi.x - be * vpi.x, w[i].y - al * vi.y - be * vpi.y);
+  f[i] = fi;
+  if (gen) {
+    const z64 s = sv[i], sp = svp[i];
+    sf[i] = mk(u[i].x - al * s.x - be * sp.x, u[i].y - al * s.y - be * sp.y);
+  }
+}
+
+// alpha_j = Re<v|u> from the two-stage reduction result in red[0]
+__global__ void k_store_alpha(const double* __restrict__ red, double* __restrict__ ab, int j) {
+  ab[2 * j] = red[0];
+}
+__global__ void k_store_beta(const double* __restrict__ red, double* __restrict__ ab, int j) {
+  const double x = red[0];
+  ab[2 * j + 1] = x > 0 ? sqrt(x) : 0.0;
+}
+
+// v <- f / beta_j (and sv <- sf / beta_j); vp, svp take the old v, sv.
+__global__ void k_lanczos_next(z64* __restrict__ v, z64* __restrict__ sv, z64* __restrict__ vp,
+                               z64* __restrict__ svp, const z64* __restrict__ f,
+                               const z64* __restrict__ sf, const double* __restrict__ ab, int j,
+                               int gen, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double be = ab[2 * j + 1];
+  const double s = be > 0 ? 1.0 / be : 0.0;
+  vp[i] = v[i];
+  if (gen) svp[i] = sv[i];
+  if (be > 0) {
+    v[i] = mk(f[i].x * s, f[i].y * s);
+    if (gen) sv[i] = mk(sf[i].x * s, sf[i].y * s);
+  }
+}
+
+__global__ void k_lanczos_scale(z64* v, z64* sv, const double* __restrict__ ab, int j, int gen,
+                                size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double be = ab[2 * j + 1];
+  const double s = be > 0 ? 1.0 / be : 1.0;
+  v[i] = mk(v[i].x * s, v[i].y * s);
+  if (gen) sv[i] = mk(sv[i].x * s, sv[i].y * s);
+}
+
+// Eigenvalues of the k x k tridiagonal T (alpha, beta) by cyclic Jacobi in
+// one thread; out = {max + beta_{k-1}, min}.
+__global__ void k_lanczos_finish(const double* __restrict__ ab, int k, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double t[16][16];
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) t[i][j] = 0;
+  for (int i = 0; i < k; ++i) {
+    t[i][i] = ab[2 * i];
+    if (i + 1 < k) t[i][i + 1] = t[i + 1][i] = ab[2 * i + 1];
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0;
+    for (int i = 0; i < k; ++i)
+      for (int j = i + 1; j < k; ++j) off += t[i][j] * t[i][j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < k - 1; ++p)
+      for (int q = p + 1; q < k; ++q) {
+        const double apq = t[p][q];
+        if (fabs(apq) < 1e-300) continue;
+        const double tau = (t[q][q] - t[p][p]) / (2.0 * apq);
+        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
+        for (int r = 0; r < k; ++r) {
+          const double a = t[r][p], b = t[r][q];
+          t[r][p] = c * a - s * b;
+          t[r][q] = s * a + c * b;
+        }
+        for (int r = 0; r < k; ++r) {
+          const double a = t[p][r], b = t[q][r];
+          t[p][r] = c * a - s * b;
+          t[q][r] = s * a + c * b;
+        }
+      }
+  }
+  double mx = t[0][0], mn = t[0][0];
+  for (int i = 1; i < k; ++i) {
+    mx = fmax(mx, t[i][i]);
+    mn = fmin(mn, t[i][i]);
+  }
+  out[0] = mx + ab[2 * (k - 1) + 1];
+  out[1] = mn;
+}
+
+static int dot_re_into(chs_context* ctx, const z64* a, const z64* b, size_t n, double* dst) {
+  auto* im = ctx->impl;
+  CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * kRedBlocks + 64));
+  hipLaunchKernelGGL((k_dot_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream, a,
+                     b, n, (double*)im->part.p);
+  CHS_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                     (const double*)im->part.p, kRedBlocks, dst);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+// Lanczos bound in the S inner product (S = I without projectors); start
+// vector complex N(0,1) per plane wave from the LCG with `seed`.
+int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min) {
+  CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const int n = H.n_pw;
+  const bool gen = H.has_s;
+  if (k < 1 || k > 16) return set_error(ctx, CHS_ERR_INVALID_ARG, "Lanczos steps must be 1..16");
+  // start vector from the LCG (host, once per call; n_pw complex)
+  std::vector<double> h0(2 * (size_t)n);
+  {
+    uint64_t s = seed;
+    auto uni = [&s]() {
+      s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
+      return (double)s / 281474976710656.0;
+    };
+    auto nrm = [&]() {
+      double u1 = 1.0 - uni();
+      double u2 = uni();
+      return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    };
+    for (int g = 0; g < n; ++g) {
+      h0[2 * g] = nrm();
+      h0[2 * g + 1] = nrm();
+    }
+  }
+  const size_t vb = (size_t)n * sizeof(z64);
+  CHS_TRY(ensure(ctx, im->lin_ws, 9 * vb + 64 * sizeof(double)));
+  z64* base = (z64*)im->lin_ws.p;
+  z64 *v = base, *sv = base + n, *vp = base + 2 * (size_t)n, *svp = base + 3 * (size_t)n,
+      *u = base + 4 * (size_t)n, *w = base + 5 * (size_t)n, *f = base + 6 * (size_t)n,
+      *sf = base + 7 * (size_t)n;
+  double* ab = (double*)(base + 9 * (size_t)n);  // alpha/beta pairs + scratch
+  double* red = ab + 40;
+  CHS_HIP(ctx, hipMemcpyAsync(v, h0.data(), vb, hipMemcpyHostToDevice, ctx->stream));
+  ctx->h2d += vb;
+  CHS_HIP(ctx, hipMemsetAsync(vp, 0, vb, ctx->stream));
+  CHS_HIP(ctx, hipMemsetAsync(svp, 0, vb, ctx->stream));
+  CHS_HIP(ctx, hipMemsetAsync(ab, 0, 40 * sizeof(double), ctx->stream));
+  // normalise: v /= sqrt(<v|Sv>)
+  if (gen) {
+    CHS_TRY(apply_h_impl(ctx, CHS_C128, v, 1, u, sv));
+  } else {
+    CHS_HIP(ctx, hipMemcpyAsync(sv, v, vb, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  CHS_TRY(dot_re_into(ctx, v, sv, n, red));
+  hipLaunchKernelGGL(k_store_beta, dim3(1), dim3(1), 0, ctx->stream, red, ab, 16);
+  CHS_LAUNCH_CHECK(ctx);
+  // v <- v / sqrt(<v|Sv>) (divisor in the ab[33] slot)
+  hipLaunchKernelGGL(k_lanczos_scale, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, v, sv,
+                     ab, 16, (int)gen, (size_t)n);
+  CHS_LAUNCH_CHECK(ctx);
+  for (int j = 0; j < k; ++j) {
+    CHS_TRY(apply_h_impl(ctx, CHS_C128, v, 1, u, nullptr));
+    CHS_HIP(ctx, hipMemcpyAsync(w, u, vb, hipMemcpyDeviceToDevice, ctx->stream));
+    if (gen) CHS_TRY(sinv_apply(ctx, CHS_C128, w, 1));
+    CHS_TRY(dot_re_into(ctx, v, u, n, red));
+    hipLaunchKernelGGL(k_store_alpha, dim3(1), dim3(1), 0, ctx->stream, red, ab, j);
+    CHS_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_lanczos_f, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, w, u, v, sv,
+                       vp, svp, ab, j, (int)gen, (size_t)n, f, sf);
+    CHS_LAUNCH_CHECK(ctx);
+    CHS_TRY(dot_re_into(ctx, f, gen ? sf : f, n, red));
+    hipLaunchKernelGGL(k_store_beta, dim3(1), dim3(1), 0, ctx->stream, red, ab, j);
+    CHS_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_lanczos_next, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, v, sv,
+                       vp, svp, f, sf, ab, j, (int)gen, (size_t)n);
+    CHS_LAUNCH_CHECK(ctx);
+  }
+  hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(1), 0, ctx->stream, ab, k, red + 4);
+  CHS_LAUNCH_CHECK(ctx);
+  double hr[2];
+  CHS_HIP(ctx, hipMemcpyAsync(hr, red + 4, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += 2 * sizeof(double);
+  *upper = hr[0];
+  if (ritz_min) *ritz_min = hr[1];
+  return CHS_OK;
+}
+
+// ---- level-1 primitives ------------------------------------------------
+__global__ void k_zaxpy(size_t n, double ar, double ai, const z64* __restrict__ x, z64* y) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const z64 a = mk(ar, ai);
+    y[i] = y[i] + a * x[i];
+  }
+}
+__global__ void k_zscal(size_t n, double ar, double ai, z64* x) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = mk(ar, ai) * x[i];
+}
+
+__global__ void k_minmax(const double* __restrict__ v, size_t n, double* __restrict__ out) {
+  __shared__ double smin[256], smax[256];
+  double mn = INFINITY, mx = -INFINITY;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+    mn = fmin(mn, v[i]);
+    mx = fmax(mx, v[i]);
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smin[0];
+    out[1] = smax[0];
+  }
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" {
+
+int chs_apply_h(chs_context* ctx, int precision, const void* d_psi, int n_b, void* d_hpsi,
+                void* d_spsi) {
+  if (!ctx || !d_psi || !d_hpsi || n_b <= 0 || (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  return apply_h_impl(ctx, precision, d_psi, n_b, d_hpsi, d_spsi);
+}
+
+int chs_chebyshev_filter(chs_context* ctx, int precision, const void* d_psi, int n_b, int degree,
+                         const double* bounds4, void* d_out) {
+  if (!ctx || !d_psi || !d_out || !bounds4 || n_b <= 0 || degree < 0 ||
+      (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  if (!(bounds4[3] > 0.0))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "filter half width must be positive");
+  return chebyshev_filter_impl(ctx, precision, d_psi, n_b, degree, bounds4, d_out);
+}
+
+int chs_lanczos_bound(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min) {
+  if (!ctx || !upper) return CHS_ERR_INVALID_ARG;
+  return lanczos_impl(ctx, k, seed, upper, ritz_min);
+}
+
+int chs_grid_lambda_max(chs_context* ctx, double L, double* out) {
+  if (!ctx || !out || !(L > 0)) return CHS_ERR_INVALID_ARG;
+  Hamiltonian& H = ctx->impl->ham;
+  if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "no Hamiltonian bound");
+  CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
+  double* d = (double*)ctx->impl->small.p + 16;
+  const size_t n3 = (size_t)H.n_grid * H.n_grid * H.n_grid;
+  hipLaunchKernelGGL(k_minmax, dim3(1), dim3(256), 0, ctx->stream, H.veff, n3, d);
+  CHS_LAUNCH_CHECK(ctx);
+  double mm[2];
+  CHS_HIP(ctx, hipMemcpyAsync(mm, d, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += 2 * sizeof(double);
+  // gmax: the grid corner, as GVectorGrid::gmax in the reference (D6)
+  const double b = 2.0 * M_PI / L;
+  const double gmax = b * std::sqrt(3.0) * (H.n_grid / 2);
+  *out = 0.5 * gmax * gmax + (mm[1] - mm[0]);
+  return CHS_OK;
+}
+
+int chs_zdotc(chs_context* ctx, int n, const double* d_x, const double* d_y, double out[2]) {
+  if (!ctx || n < 0 || !out || (n && (!d_x || !d_y))) return CHS_ERR_INVALID_ARG;
+  auto* im = ctx->impl;
+  CHS_TRY(ensure(ctx, im->small, 64 * sizeof(double)));
+  double* dst = (double*)im->small.p + 32;
+  CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * kRedBlocks + 64));
+  hipLaunchKernelGGL((k_dot_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
+                     (const z64*)d_x, (const z64*)d_y, (size_t)n, (double*)im->part.p);
+  CHS_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kRedThreads), 0, ctx->stream,
+                     (const double*)im->part.p, kRedBlocks, dst);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_HIP(ctx, hipMemcpyAsync(out, dst, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += 2 * sizeof(double);
+  return CHS_OK;
+}
+
+int chs_zaxpy(chs_context* ctx, int n, const double alpha[2], const double* d_x, double* d_y) {
+  if (!ctx || n < 0 || !alpha || (n && (!d_x || !d_y))) return CHS_ERR_INVALID_ARG;
+  if (!n) return CHS_OK;
+  hipLaunchKernelGGL(k_zaxpy, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, (size_t)n,
+                     alpha[0], alpha[1], (const z64*)d_x, (z64*)d_y);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x) {
+  if (!ctx || n < 0 || !alpha || (n && !d_x)) return CHS_ERR_INVALID_ARG;
+  if (!n) return CHS_OK;
+  hipLaunchKernelGGL(k_zscal, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, (size_t)n,
+                     alpha[0], alpha[1], (z64*)d_x);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,19 @@
+// Internal operator entry points shared across translation units.
+#pragma once
+#include "impl.hpp"
+
+namespace chs {
+
+int ensure_precision_ready(chs_context* ctx, int precision);
+int apply_h_impl(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi, void* spsi);
+int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int n_b, int degree,
+                          const double* b4, void* out);
+int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min);
+// Nonlocal projector terms (projectors.hip): hpsi += beta D beta^H psi and,
+// when spsi != nullptr, spsi = psi + beta Q beta^H psi.
+int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
+                   void* spsi);
+// x <- S^{-1} x = x - beta M beta^H x (Woodbury).
+int sinv_apply(chs_context* ctx, int precision, void* x, int n_b);
+
+}  // namespace chs

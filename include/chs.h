@@ -1,0 +1,224 @@
+/* chs.h -- C ABI of libchs_gpu.so, the MI355X (gfx950) implementation of the
+ * chemrust-scf SCF eigensolver hot path.
+ *
+ * Every entry point replaces a piece of the reference's device layer or
+ * eigensolver surface; the reference file:line each one stands in for is
+ * given beside it (paths relative to the reference repository root).
+ *
+ * Conventions (identical to the reference):
+ *  - complex numbers are interleaved (re, im): double pairs for CHS_C128,
+ *    float pairs for CHS_C64 (src/device/mod.rs:24-43);
+ *  - a wavefunction block is band-major ("ColumnDistributed",
+ *    src/layout.rs:48-53): psi[b * n_pw + g], i.e. an (n_pw x n_b)
+ *    column-major matrix with leading dimension n_pw;
+ *  - FFT grids are ix-fastest: idx = ix + ngx * (iy + ngy * iz)
+ *    (src/scf.rs:385-399); the inverse transform is e^{+iG.r}, the forward
+ *    e^{-iG.r}, both unnormalised (src/eigensolver/chebyshev.rs:331,346);
+ *  - eigenvalues ascend (ZHEGVD, src/device/solver.rs:47-95).
+ *
+ * Every function returns an int32 status: CHS_OK (0) or a negative code, one
+ * per variant of the reference's Error enum (src/types.rs:225-248) plus
+ * argument and allocation errors. Nothing is thrown or aborts across the
+ * ABI. chs_last_error() gives a message; chs_diag() the structured fields
+ * (ChebyshevDiverged, solver info, failing Cholesky column).
+ *
+ * Threading: one context owns one HIP stream; calls on a context are
+ * stream-ordered and must not be made from two threads at once. Separate
+ * contexts are independent (src/scf.rs:173-174, CONTEXT.md:160).
+ * Device buffers passed in are owned by the caller; the library never frees
+ * them. Workspaces are owned by the context and reused across calls. */
+#ifndef CHS_H
+#define CHS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (src/types.rs:225-248) ---------------------------- */
+enum {
+  CHS_OK = 0,
+  CHS_ERR_NOT_IMPLEMENTED = -1,      /* Error::NotImplemented */
+  CHS_ERR_CHEBYSHEV_DIVERGED = -2,   /* Error::ChebyshevDiverged{..} */
+  CHS_ERR_RAYLEIGH_RITZ_FAILED = -3, /* Error::RayleighRitzFailed{info} */
+  CHS_ERR_SOLVER = -4,               /* Error::Solver */
+  CHS_ERR_BLAS = -5,                 /* Error::Blas */
+  CHS_ERR_FFT = -6,                  /* Error::Fft */
+  CHS_ERR_DEVICE = -7,               /* Error::Cuda (HIP runtime error) */
+  CHS_ERR_KERNEL_BUILD = -8,         /* Error::Nvrtc (no runtime compile here: unused) */
+  CHS_ERR_INVALID_ARG = -9,
+  CHS_ERR_OUT_OF_MEMORY = -10,
+  CHS_ERR_NOT_CONVERGED = -11        /* SCF hit max_iter */
+};
+
+/* precision of a compute call */
+enum { CHS_C128 = 0, CHS_C64 = 1 };
+
+typedef struct chs_diag {
+  int32_t iteration;          /* ChebyshevDiverged.iteration */
+  double norm_previous;       /* ChebyshevDiverged.norm_previous */
+  double norm_current;        /* ChebyshevDiverged.norm_current */
+  double growth_factor;       /* ChebyshevDiverged.growth_factor */
+  int32_t solver_info;        /* RayleighRitzFailed.info / Cholesky column+1 */
+} chs_diag;
+
+typedef struct chs_context chs_context;
+
+/* ---- context (src/scf.rs:173-177: CudaContext::new(0), default stream,
+ * BLAS/solver handles, CudaKernelSet::new) ------------------------------ */
+int chs_context_create(int device, chs_context** out);
+int chs_context_destroy(chs_context* ctx);
+int chs_synchronize(chs_context* ctx);
+const char* chs_last_error(chs_context* ctx);
+int chs_diag_get(chs_context* ctx, chs_diag* out);
+/* PcieAccount (src/device/pcie.rs:21-70): bytes moved host<->device by this
+ * context since creation or the last reset. */
+int chs_transfer_bytes(chs_context* ctx, uint64_t* h2d, uint64_t* d2h);
+int chs_transfer_reset(chs_context* ctx);
+
+/* ---- memory (Gpu<T>::from_host / sync_to_host, src/device/mod.rs:216-265;
+ * Gpu::from_host_with / sync_to_host_with, src/device/pcie.rs:44-70) ---- */
+int chs_alloc(chs_context* ctx, size_t bytes, void** dptr);
+int chs_free(chs_context* ctx, void* dptr);
+int chs_h2d(chs_context* ctx, void* dst, const void* src, size_t bytes);
+int chs_d2h(chs_context* ctx, void* dst, const void* src, size_t bytes);
+int chs_d2d(chs_context* ctx, void* dst, const void* src, size_t bytes);
+int chs_memset(chs_context* ctx, void* dptr, int value, size_t bytes);
+
+/* ---- setup ------------------------------------------------------------ */
+/* pw_coords_to_fft_indices (src/scf.rs:387-401): Miller triples [h,k,l]
+ * (device int32, 3*n_pw) to ix + ngx*(iy + ngy*iz) with negative indices
+ * wrapped by +n. Bit-exact. */
+int chs_pw_coords_to_fft_indices(chs_context* ctx, const int32_t* d_miller, int n_pw, int ngx,
+                                 int ngy, int ngz, int32_t* d_idx);
+/* Plane-wave sphere 0.5|k+G|^2 <= ecut of a cubic cell of edge L on an
+ * n^3 grid (the reference takes it from GVectorGrid/KptWaveBlock of its
+ * chemrust-hamiltonian-core dependency; this library defines the order:
+ * lexicographic (h, k, l), h slowest). With d_miller == NULL only *n_pw is
+ * returned; otherwise capacity max_pw must hold. Outputs: Miller triples,
+ * FFT indices, 0.5|k+G|^2. */
+int chs_gsphere(chs_context* ctx, double L, double ecut, const double kfrac[3], int n_grid,
+                int max_pw, int* n_pw, int32_t* d_miller, int32_t* d_idx, double* d_kin);
+
+/* ---- 3D FFT (BatchedFftPlan3d::{c2c_forward, c2c_inverse},
+ * src/device/fft.rs:116-228): batched, in place, unnormalised, n^3 grids
+ * stored back to back. sign = +1 inverse, -1 forward. ------------------- */
+int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int n, int batch, int sign);
+
+/* ---- Hamiltonian ------------------------------------------------------ */
+/* Bind the plane-wave set, grid and local potential used by the operator
+ * calls below (the arguments of apply_full_hamiltonian,
+ * src/eigensolver/chebyshev.rs:371-403): per-PW kinetic energies
+ * (device f64, n_pw), Miller triples (device int32, 3*n_pw), V_eff on the
+ * n^3 grid (device f64). Builds the column ("stick") tables of the fused
+ * FFT kernels. The device arrays must stay valid while bound. */
+int chs_bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_miller,
+                         const double* d_kin, const double* d_veff);
+/* Replace V_eff (device f64, n^3) in the bound Hamiltonian. */
+int chs_set_veff(chs_context* ctx, const double* d_veff);
+/* Nonlocal projectors (VnlBatchData::precompute, src/eigensolver/
+ * vnl_data.rs:25-89): beta (device c128, projector-major P x n_pw, the
+ * per-ion (n_exp x n_pw) blocks concatenated), diagonal D and Q (device f64,
+ * P). P = 0 removes them. With q != 0 the overlap S = 1 + beta Q beta^H is
+ * active and S^{-1} is prepared by the Woodbury identity. */
+int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, const double* d_d,
+                        const double* d_q);
+int chs_set_dion(chs_context* ctx, const double* d_d);
+/* H|psi> (and S|psi> when d_spsi != NULL) for n_b bands
+ * (apply_full_hamiltonian, src/eigensolver/chebyshev.rs:371-403; the
+ * kinetic term per plane wave). psi/hpsi/spsi: device, band-major. */
+int chs_apply_h(chs_context* ctx, int precision, const void* d_psi, int n_b, void* d_hpsi,
+                void* d_spsi);
+
+/* ---- eigensolver ------------------------------------------------------ */
+/* compute_spectral_bounds (src/eigensolver/chebyshev.rs:214-254) with an
+ * explicit lambda_max; out4 = {lambda_max, eps_cut, center, half_width}. */
+int chs_spectral_bounds(const double* eig, int n_eig, double lambda_max, double* out4);
+/* lambda_max of the reference's estimate: 0.5 gmax^2 (grid corner of a
+ * cubic cell of edge L) + (max V_eff - min V_eff) (chebyshev.rs:225-237,
+ * scf.rs:189-194). */
+int chs_grid_lambda_max(chs_context* ctx, double L, double* out);
+/* k-step Lanczos upper bound of H (or S^{-1}H), FP64:
+ * max eig(T_k) + ||f_k|| (Zhou & Li, eq. 2.5). The reference has only the
+ * spec (notes/pr-reviews/phase-2/deferred.md:131-160); it replaces the
+ * lambda_max of compute_spectral_bounds. Start vector: complex N(0,1) per
+ * plane wave from the LCG (src/eigensolver/subspace_diag.rs:434-447) with
+ * `seed`. ritz_min (may be NULL) gets min eig(T_k). */
+int chs_lanczos_bound(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min);
+/* chebyshev_filter (src/eigensolver/chebyshev.rs:626-816): degree-`degree`
+ * Chebyshev filter of the operator (H, or S^{-1}H with projectors) on the
+ * window bounds4 = {lambda_max, eps_cut, center, half_width}; the
+ * reference's 10x norm-growth guard returns CHS_ERR_CHEBYSHEV_DIVERGED
+ * with the fields in chs_diag. d_psi (in) and d_out (out) band-major, in
+ * the given precision; d_out may equal d_psi. */
+int chs_chebyshev_filter(chs_context* ctx, int precision, const void* d_psi, int n_b, int degree,
+                         const double* bounds4, void* d_out);
+/* rayleigh_ritz (src/eigensolver/rayleigh_ritz.rs:48-245) with the
+ * semantics of diagonalize_and_rotate (src/eigensolver/subspace_diag.rs:
+ * 229-391): H_sub = Y^H H Y, S_sub = Y^H S Y (Hermitised), Cholesky-QR
+ * S_sub = L L^H, B = L^{-1} H_sub L^{-H}, Hermitian eigensolve, Y <- Y
+ * L^{-H} U in place (FP64 throughout). d_hy (may be NULL) receives H Y of
+ * the rotated block. Eigenvalues to host array eps (n_b, ascending). */
+int chs_rayleigh_ritz(chs_context* ctx, double* d_y, int n_b, double* eps, double* d_hy);
+/* Hermitian eigensolver of a device n x n column-major c128 matrix
+ * (SolverHandle::zhegvd, src/device/solver.rs:47-95, itype 1, lower, with
+ * vectors): generalized when d_b != NULL (B is overwritten by its Cholesky
+ * factor), standard otherwise. d_a is overwritten by the eigenvectors;
+ * eigenvalues to host array w. */
+int chs_zhegvd(chs_context* ctx, int n, double* d_a, double* d_b, double* w);
+/* Cholesky-QR orthonormalisation (notes/pr-reviews/phase-2/deferred.md:
+ * 27-48): S = Y^H S Y = L L^H, Y <- Y L^{-H} in place (FP64). */
+int chs_cholqr(chs_context* ctx, double* d_y, int n_b);
+
+/* ---- primitives (BlasHandle, src/device/blas.rs:24-241) --------------
+ * Column-major, op: 'N', 'T' or 'C'. */
+int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const double alpha[2],
+              const double* d_a, int lda, const double* d_b, int ldb, const double beta[2],
+              double* d_c, int ldc);
+int chs_zdotc(chs_context* ctx, int n, const double* d_x, const double* d_y, double out[2]);
+int chs_zaxpy(chs_context* ctx, int n, const double alpha[2], const double* d_x, double* d_y);
+int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
+
+/* ---- SCF -------------------------------------------------------------- */
+/* Density from band-major c128 wavefunctions with occupations (host f64,
+ * n_b): rho(r) = sum_i f_i |psi_i(r)|^2 / Omega (+ augmentation with
+ * projectors). construct_density is todo!() in the reference
+ * (src/scf.rs:272-280); spec PROJECT_ROOT_PLAN.md:568-611. */
+int chs_density(chs_context* ctx, const double* d_psi, int n_b, const double* occ, double omega,
+                double* d_rho);
+
+typedef struct chs_scf_params {
+  int32_t n_rep;           /* diamond-cubic supercell repetitions per axis */
+  double a0;               /* conventional lattice constant (bohr) */
+  double zval, rc_loc;     /* V_loc(r) = -zval erf(r/rc_loc)/r */
+  double ecut;             /* plane-wave cutoff (Ha) */
+  int32_t n_grid, n_bands;
+  double kfrac[3];
+  int32_t n_proj_per_atom; /* 0 or 4 (l = 0, 1) */
+  double beta_rc, q_rc, q0, d_min, d_max;
+  uint64_t seed_d, seed_psi, seed_lanczos;
+  int32_t cheb_degree, first_passes, lanczos_k, pulay_depth;
+  double mix_alpha, kerker_q, etol, smear_kt;
+  int32_t max_iter;
+  int32_t precision;       /* CHS_C128, or CHS_C64 filter with FP64 RR */
+  int32_t fp64_final;      /* with CHS_C64: SCF steps forced back to FP64 at the end */
+} chs_scf_params;
+
+typedef struct chs_scf_result {
+  int32_t n_iter, converged, n_pw, n_proj;
+  double energy, mu, seconds;
+  double t_filter, t_rr, t_density, t_veff, t_lanczos; /* seconds per phase */
+} chs_scf_result;
+
+/* Full SCF (run_scf, src/scf.rs:358-376, with the todo!() transitions
+ * build_v_eff / construct_density / check / mix filled in): atom positions
+ * (host f64, 3 * n_atoms), energies (host, max_iter), eps (host, n_bands),
+ * rho (host, n^3, may be NULL). */
+int chs_scf_run(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
+                double* energies, double* eps, double* rho);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,131 @@
+"""GPU parity of the plane-wave sphere, index map, 3D FFT, H|psi>, the
+Chebyshev filter and the Lanczos bound against the FP64 oracle."""
+import numpy as np
+import pytest
+
+import chs
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def model(n_rep=1, n_grid=36, n_bands=20, **kw):
+    p = O.default_params(n_rep=n_rep, n_grid=n_grid, n_bands=n_bands, **kw)
+    return O.Model(p, O.positions(n_rep))
+
+
+@pytest.mark.parametrize("n_rep,n_grid,kf", [(1, 36, (0, 0, 0)), (2, 72, (0, 0, 0)),
+                                             (1, 36, (0.25, 0.25, 0.25)), (1, 36, (0.5, 0, 0.5))])
+def test_gsphere_bit_exact(ctx, n_rep, n_grid, kf):
+    L = 10.26 * n_rep
+    mil, idx, kin = chs.gsphere(ctx, L, 15.0, n_grid, kf)
+    n = O.lib().orc_gsphere(L, 15.0, np.array(kf, float), n_grid, 0, None, None, None)
+    assert mil.shape[0] == n
+    om = np.zeros(3 * n, np.int32); oi = np.zeros(n, np.int32); ok = np.zeros(n)
+    O.lib().orc_gsphere(L, 15.0, np.array(kf, float), n_grid, n, om.ctypes.data, oi.ctypes.data,
+                        ok.ctypes.data)
+    assert np.array_equal(mil.sync_to_host().ravel(), om)
+    assert np.array_equal(idx.sync_to_host(), oi)
+    assert np.array_equal(kin.sync_to_host().view(np.uint64), ok.view(np.uint64))
+
+
+def test_index_map_bit_exact_random(ctx):
+    rng = np.random.default_rng(7)
+    mil = rng.integers(-17, 18, size=(5000, 3)).astype(np.int32)
+    got = chs.pw_coords_to_fft_indices(ctx, mil, (72, 64, 36))
+    ref = np.zeros(5000, np.int32)
+    O.lib().orc_pw_coords_to_fft_indices(mil.ravel().copy(), 5000, 72, 64, 36, ref)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("n", [12, 36, 72, 108])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 2e-6)])
+def test_fft3d_vs_numpy(ctx, n, dtype, tol):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal((2, n, n, n)) + 1j * rng.standard_normal((2, n, n, n))).astype(dtype)
+    g = chs.Gpu.from_host(ctx, x)
+    chs.fft3d_c2c(ctx, g, -1)
+    fwd = g.sync_to_host()
+    ref = np.fft.fftn(x.astype(np.complex128), axes=(1, 2, 3))
+    assert np.abs(fwd - ref).max() / np.abs(ref).max() < tol
+    chs.fft3d_c2c(ctx, g, +1)
+    back = g.sync_to_host()
+    # unnormalised round trip = n^3 x (src/device/fft.rs:240-281)
+    assert np.abs(back / n ** 3 - x).max() / np.abs(x).max() < tol
+
+
+def bind(ctx, m, veff=None):
+    mil, idx, kin = m.pw()
+    v = m.vion() if veff is None else veff
+    return chs.Hamiltonian(ctx, m.n_grid, mil, kin, v)
+
+
+@pytest.mark.parametrize("n_rep,n_grid,nb", [(1, 36, 20), (2, 72, 8)])
+def test_apply_h_fp64(ctx, n_rep, n_grid, nb):
+    m = model(n_rep, n_grid, nb)
+    H = bind(ctx, m)
+    psi = m.initial_psi(nb)
+    ref = m.apply_h(psi)
+    got = chs.apply_full_hamiltonian(ctx, chs.Gpu.from_host(ctx, psi)).sync_to_host()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+
+
+def test_apply_h_fp32(ctx):
+    m = model(1, 36, 20)
+    H = bind(ctx, m)
+    psi = m.initial_psi(20)
+    ref = m.apply_h(psi)
+    got = chs.apply_full_hamiltonian(ctx, chs.Gpu.from_host(ctx, psi.astype(np.complex64))).sync_to_host()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-5, err
+
+
+def test_apply_h_closed_forms(ctx):
+    """V = 0 => H psi = 0.5|G|^2 psi; V = const => shift."""
+    m = model(1, 36, 4)
+    mil, idx, kin = m.pw()
+    psi = m.initial_psi(4)
+    for c in (0.0, -0.7):
+        H = chs.Hamiltonian(ctx, 36, mil, kin, np.full(36 ** 3, c))
+        got = chs.apply_full_hamiltonian(ctx, chs.Gpu.from_host(ctx, psi)).sync_to_host()
+        assert np.abs(got - (kin + c) * psi).max() < 1e-13
+
+
+def test_chebyshev_filter_fp64(ctx):
+    m = model(1, 36, 20)
+    H = bind(ctx, m)
+    psi = m.initial_psi(20)
+    lmax = m.grid_lambda_max()
+    b4 = O.spectral_bounds(None, lmax)
+    assert np.allclose(chs.compute_spectral_bounds(None, lmax), b4, rtol=0, atol=0)
+    rc, ref, _ = m.chebyshev_filter(psi, 10, b4)
+    assert rc == 0
+    got = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 10, b4).sync_to_host()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-11, err
+    assert abs(chs.grid_lambda_max(ctx, 10.26) - lmax) < 1e-12 * abs(lmax)
+
+
+def test_chebyshev_diverged(ctx):
+    """A window far above the spectrum makes every step grow > 10x."""
+    m = model(1, 36, 4)
+    H = bind(ctx, m)
+    psi = m.initial_psi(4)
+    b4 = np.array([1000.0, 999.0, 999.5, 0.5])
+    rc, _, diag = m.chebyshev_filter(psi, 4, b4)
+    assert rc == -2
+    with pytest.raises(chs.ChebyshevDiverged) as e:
+        chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 4, b4)
+    d = e.value.diag
+    assert d.iteration == int(diag[0]) == 2
+    assert abs(d.growth_factor - diag[3]) < 1e-9 * diag[3]
+
+
+def test_lanczos_bound(ctx):
+    m = model(1, 36, 4)
+    H = bind(ctx, m)
+    up, lo = chs.lanczos_bound(ctx, 10, 4)
+    oup, olo = m.lanczos_bound(10, 4)
+    assert abs(up - oup) < 1e-9 * abs(oup)
+    assert abs(lo - olo) < 1e-9 * max(1.0, abs(olo))

@@ -1,0 +1,131 @@
+"""GPU parity of the FP64 dense linear algebra and Rayleigh-Ritz: the
+reference's device KATs (src/device/blas.rs:259-310, src/device/solver.rs:
+113-163), the subspace_diag.rs KATs (:455-918) and the oracle's RR."""
+import numpy as np
+import pytest
+
+import chs
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def fmat(a):
+    """numpy (n, m) -> column-major device layout: store a.T row-major."""
+    return np.ascontiguousarray(np.asarray(a, np.complex128).T)
+
+
+def test_zgemm_kat(ctx):
+    # A = [[1+i, 0], [0, 1-i]], B = [[1, 2], [3, 4]] -> C = [[1+i, 2+2i], [3-3i, 4-4i]]
+    A = np.array([[1 + 1j, 0], [0, 1 - 1j]])
+    B = np.array([[1, 2], [3, 4]], complex)
+    a, b = chs.Gpu.from_host(ctx, fmat(A)), chs.Gpu.from_host(ctx, fmat(B))
+    c = chs.Gpu(ctx, (2, 2), np.complex128).zero()
+    chs.zgemm(ctx, "N", "N", 2, 2, 2, 1.0, a, 2, b, 2, 0.0, c, 2)
+    got = c.sync_to_host().T
+    assert np.abs(got - np.array([[1 + 1j, 2 + 2j], [3 - 3j, 4 - 4j]])).max() < 1e-10
+
+
+@pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("T", "T"), ("C", "T")])
+@pytest.mark.parametrize("m,n,k", [(7, 5, 3), (70, 90, 300), (160, 160, 23847), (24000, 160, 160)])
+def test_zgemm_random(ctx, opa, opb, m, n, k):
+    if m * k > 5e6 and (opa, opb) != ("N", "N") and (opa, opb) != ("C", "N"):
+        pytest.skip("large shape covered by the RR layouts")
+    rng = np.random.default_rng(m + n + k)
+    def op(x, o):
+        return x if o == "N" else (x.T if o == "T" else x.conj().T)
+    sa = (m, k) if opa == "N" else (k, m)
+    sb = (k, n) if opb == "N" else (n, k)
+    A = rng.standard_normal(sa) + 1j * rng.standard_normal(sa)
+    B = rng.standard_normal(sb) + 1j * rng.standard_normal(sb)
+    C0 = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    al, be = 0.7 - 0.2j, 0.3 + 0.1j
+    a, b = chs.Gpu.from_host(ctx, fmat(A)), chs.Gpu.from_host(ctx, fmat(B))
+    c = chs.Gpu.from_host(ctx, fmat(C0))
+    chs.zgemm(ctx, opa, opb, m, n, k, al, a, sa[0], b, sb[0], be, c, m)
+    ref = al * op(A, opa) @ op(B, opb) + be * C0
+    got = c.sync_to_host().T
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
+
+
+def test_zhegvd_kat_identity(ctx):
+    # src/device/solver.rs:113-163: A = diag(1,2,3,4), B = I
+    a = chs.Gpu.from_host(ctx, fmat(np.diag([1.0, 2, 3, 4]).astype(complex)))
+    b = chs.Gpu.from_host(ctx, fmat(np.eye(4, dtype=complex)))
+    w = chs.zhegvd(ctx, a, b)
+    assert np.abs(w - [1, 2, 3, 4]).max() < 1e-8
+
+
+def test_zhegvd_kat_generalized(ctx):
+    # src/eigensolver/subspace_diag.rs:534-587: H = diag(1..4), S = diag(2,1.5,1,0.8)
+    a = chs.Gpu.from_host(ctx, fmat(np.diag([1.0, 2, 3, 4]).astype(complex)))
+    b = chs.Gpu.from_host(ctx, fmat(np.diag([2.0, 1.5, 1.0, 0.8]).astype(complex)))
+    w = chs.zhegvd(ctx, a, b)
+    assert np.abs(w - [0.5, 4 / 3, 3.0, 5.0]).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [6, 33, 160, 512])
+def test_zhegvd_random_vs_scipy(ctx, n):
+    import scipy.linalg as sl
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    Hm = X + X.conj().T
+    Y = rng.standard_normal((n + 8, n)) + 1j * rng.standard_normal((n + 8, n))
+    Sm = Y.conj().T @ Y
+    a, b = chs.Gpu.from_host(ctx, fmat(Hm)), chs.Gpu.from_host(ctx, fmat(Sm))
+    w = chs.zhegvd(ctx, a, b)
+    ref = sl.eigh(Hm, Sm, eigvals_only=True)
+    assert np.abs(w - ref).max() < 1e-9 * np.abs(ref).max()
+    V = a.sync_to_host().T
+    # invariants V^H S V = I, V^H H V = diag(w) (subspace_diag.rs:680-858)
+    assert np.abs(V.conj().T @ Sm @ V - np.eye(n)).max() < 1e-9
+    assert np.abs(V.conj().T @ Hm @ V - np.diag(w)).max() < 1e-8 * np.abs(w).max()
+
+
+def test_zhegvd_not_positive_definite(ctx):
+    a = chs.Gpu.from_host(ctx, fmat(np.eye(3, dtype=complex)))
+    b = chs.Gpu.from_host(ctx, fmat(np.diag([1.0, -1.0, 1.0]).astype(complex)))
+    with pytest.raises(chs.RayleighRitzFailed) as e:
+        chs.zhegvd(ctx, a, b)
+    assert e.value.diag.solver_info == 2
+
+
+def model(n_rep=1, n_grid=36, n_bands=20):
+    p = O.default_params(n_rep=n_rep, n_grid=n_grid, n_bands=n_bands)
+    return O.Model(p, O.positions(n_rep))
+
+
+@pytest.mark.parametrize("n_rep,n_grid,nb", [(1, 36, 20), (2, 72, 160)])
+def test_rayleigh_ritz_vs_oracle(ctx, n_rep, n_grid, nb):
+    m = model(n_rep, n_grid, nb)
+    mil, idx, kin = m.pw()
+    chs.Hamiltonian(ctx, n_grid, mil, kin, m.vion())
+    psi = m.initial_psi(nb)
+    lmax = m.grid_lambda_max()
+    _, y, _ = m.chebyshev_filter(psi, 6, O.spectral_bounds(None, lmax))
+    ry, reps = m.rayleigh_ritz(y)
+    gy = chs.Gpu.from_host(ctx, y)
+    eps, hy = chs.rayleigh_ritz(ctx, gy, want_hy=True)
+    assert np.abs(eps - reps).max() < 1e-10 * max(1, np.abs(reps).max())
+    Y = gy.sync_to_host()
+    # orthonormal Ritz vectors, and H Y consistent with the rotated block
+    assert np.abs(Y.conj() @ Y.T - np.eye(nb)).max() < 1e-10
+    HY = hy.sync_to_host()
+    assert np.abs(np.einsum("ig,ig->i", Y.conj(), HY).real - eps).max() < 1e-9
+    # the spanned subspace equals the oracle's: |<y_i|y_i^ref>| ~ 1 for the
+    # lowest, well separated Ritz vector
+    assert abs(abs(np.vdot(ry[0], Y[0])) - 1) < 1e-8
+
+
+def test_cholqr(ctx):
+    m = model(1, 36, 20)
+    mil, idx, kin = m.pw()
+    chs.Hamiltonian(ctx, 36, mil, kin, m.vion())
+    psi = m.initial_psi(20)
+    g = chs.Gpu.from_host(ctx, psi)
+    chs.cholqr(ctx, g)
+    Y = g.sync_to_host()
+    assert np.abs(Y.conj() @ Y.T - np.eye(20)).max() < 1e-12
+    # same span: projector onto span(psi) leaves Y unchanged
+    Q, _ = np.linalg.qr(psi.T)
+    assert np.abs(Q @ (Q.conj().T @ Y.T) - Y.T).max() < 1e-10
