@@ -124,6 +124,7 @@ def lib():
             "chs_zscal": [vp, i, dp, vp],
             "chs_density": [vp, vp, i, dp, d, vp],
             "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
+            "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -273,6 +274,8 @@ class Hamiltonian:
         ctx.check(lib().chs_bind_hamiltonian(ctx.h, n_grid, self.n_pw, self.miller.ptr,
                                              self.kin.ptr, self.veff.ptr))
         self.n_proj = 0
+        # the context reads these device arrays until another binding replaces them
+        ctx._bound_hamiltonian = self
 
     def set_veff(self, veff):
         self.veff = veff if isinstance(veff, Gpu) else Gpu.from_host(self.ctx, np.asarray(veff, np.float64))
@@ -351,6 +354,95 @@ def zhegvd(ctx, a, b=None):
 def cholqr(ctx, y):
     """Cholesky-QR orthonormalisation of a c128 device block (in place)."""
     ctx.check(lib().chs_cholqr(ctx.h, y.ptr, y.shape[0]))
+
+
+def density(ctx, psi, occ, omega, n_grid):
+    """rho(r) = sum_b occ_b |psi_b(r)|^2 / Omega for a c128 device block on the
+    bound n_grid^3 grid (construct_density, src/scf.rs:272-280)."""
+    occ = np.ascontiguousarray(occ, np.float64)
+    rho = Gpu(ctx, (n_grid ** 3,), np.float64)
+    ctx.check(lib().chs_density(ctx.h, psi.ptr, psi.shape[0], _dptr(occ), omega, rho.ptr))
+    return rho
+
+
+def profile_hpsi(ctx, n_b, precision=C128, reps=20):
+    """Mean duration (ms) and algorithmic bytes of the three H|psi> kernels of
+    one Chebyshev step on the bound Hamiltonian (HIP events, context stream)."""
+    ms = np.zeros(3)
+    by = np.zeros(3)
+    dims = np.zeros(4, np.int32)
+    ctx.check(lib().chs_profile_hpsi(ctx.h, precision, n_b, reps, _dptr(ms), _dptr(by),
+                                     dims.ctypes.data_as(vp)))
+    return dict(ms=ms, bytes=by, n_grid=int(dims[0]), n_pw=int(dims[1]), sticks=int(dims[2]),
+                active_x=int(dims[3]))
+
+
+def synth_positions(n_rep, a0=10.26, sigma=0.05, seed=1):
+    """Diamond-cubic supercell (n_rep^3 x 8 atoms, replica-major then basis
+    order) with every Cartesian component displaced by sigma * N(0,1) from
+    the 48-bit LCG (src/eigensolver/subspace_diag.rs:434-447) and
+    Box-Muller sqrt(-2 ln(1-u1)) cos(2 pi u2)."""
+    basis = [(0, 0, 0), (0, .5, .5), (.5, 0, .5), (.5, .5, 0),
+             (.25, .25, .25), (.25, .75, .75), (.75, .25, .75), (.75, .75, .25)]
+    state = seed
+
+    def uni():
+        nonlocal state
+        state = (state * 25214903917 + 11) & 0xFFFFFFFFFFFF
+        return state / 281474976710656.0
+
+    def normal():
+        u1 = 1.0 - uni()
+        u2 = uni()
+        return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+
+    pos = []
+    for ix in range(n_rep):
+        for iy in range(n_rep):
+            for iz in range(n_rep):
+                for b in basis:
+                    for c, cell in enumerate((ix, iy, iz)):
+                        pos.append(a0 * (cell + b[c]) + sigma * normal())
+    return np.array(pos)
+
+
+def scf_params(**kw):
+    """chs_scf_params with the defaults of the reference-sized config A."""
+    p = ScfParams()
+    d = dict(n_rep=1, a0=10.26, zval=4.0, rc_loc=1.0, ecut=15.0, n_grid=36, n_bands=20,
+             n_proj_per_atom=0, beta_rc=1.3, q_rc=1.2, q0=0.0, d_min=0.5, d_max=2.0, seed_d=3,
+             seed_psi=2, seed_lanczos=4, cheb_degree=10, first_passes=4, lanczos_k=10,
+             pulay_depth=6, mix_alpha=0.5, kerker_q=0.8, etol=1e-6, smear_kt=0.01,
+             max_iter=60, precision=C128, fp64_final=3)
+    d.update(kw)
+    for k, v in d.items():
+        if k == "kfrac":
+            for i in range(3):
+                p.kfrac[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def run_scf(ctx, params, pos, want_rho=False):
+    """run_scf (src/scf.rs:358-376): full SCF on the device. Returns a dict
+    with n_iter, converged, energies, eps, timings (and rho)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    ctx._bound_hamiltonian = None  # the SCF binds (and releases) its own arrays
+    res = ScfResult()
+    energies = np.zeros(params.max_iter)
+    eps = np.zeros(params.n_bands)
+    rho = np.zeros(params.n_grid ** 3) if want_rho else None
+    rc = lib().chs_scf_run(ctx.h, C.byref(params), _dptr(pos), C.byref(res), _dptr(energies),
+                           _dptr(eps), rho.ctypes.data_as(vp) if rho is not None else None)
+    if rc not in (OK, -11):
+        ctx.check(rc)
+    out = {f: getattr(res, f) for f, _ in ScfResult._fields_}
+    out["energies"] = energies[: res.n_iter]
+    out["eps"] = eps
+    out["rho"] = rho
+    out["status"] = rc
+    return out
 
 
 def rayleigh_ritz(ctx, y, want_hy=False):

@@ -23,6 +23,7 @@
 
 #include "fft.hpp"
 #include "impl.hpp"
+#include "scf_dev.hpp"
 
 namespace chs {
 
@@ -368,6 +369,69 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   return CHS_OK;
 }
 
+// The three kernels of one Chebyshev step (steady-state epilogue) `reps`
+// times with HIP events around each launch on the context stream; ms[k] is
+// the mean duration of kernel k (A, B, C).
+template <typename T>
+static int vloc_timed_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
+                        int reps, double* ms) {
+  Hamiltonian& H = ctx->impl->ham;
+  const StickTable& st = H.st;
+  const int n = H.n_grid;
+  using C = cplx<T>;
+  CHS_TRY(ensure(ctx, ctx->impl->stick_buf, (size_t)n_b * n * st.n_st * sizeof(C)));
+  C* sbuf = (C*)ctx->impl->stick_buf.p;
+  const void* twv = nullptr;
+  CHS_TRY(twiddles(ctx, n, precision, &twv));
+  const C* tw = (const C*)twv;
+  const T* veff = precision == CHS_C128 ? (const T*)H.veff : (const T*)H.veff_f.p;
+  StickArgs sa = stick_args(st);
+  const C* psi = (const C*)psi_v;
+  C* out = (C*)out_v;
+  const double inv_n3 = 1.0 / ((double)n * n * n);
+  const int gz = ceil_div(st.n_st, kSB);
+  const size_t cmp_bytes = (size_t)n * (st.n_act + 1) * sizeof(C);
+  const size_t row_bytes = (size_t)(n + 1) * sizeof(C);
+  const int lim = max_lds_per_block(ctx);
+  if (cmp_bytes + row_bytes > (size_t)lim) return set_error(ctx, CHS_ERR_FFT, "plane too large");
+  const int ch = std::min(std::min(32, n), (int)((lim - cmp_bytes) / row_bytes));
+  const size_t plane_lds = cmp_bytes + (size_t)ch * row_bytes;
+  hipEvent_t ev[4];
+  for (auto& e : ev) CHS_HIP(ctx, hipEventCreate(&e));
+  double tot[3] = {0, 0, 0};
+  for (int r = 0; r < reps; ++r) {
+    CHS_DISPATCH_N(n, {
+      CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+      hipLaunchKernelGGL((k_stick_inv<NN, T>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
+                         sbuf, tw);
+      CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+      hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), plane_lds, ctx->stream,
+                         sbuf, sa, veff, ch, tw);
+      CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+      hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEBK>), dim3(gz, n_b), dim3(256), 0,
+                         ctx->stream, sbuf, sa, psi, H.kin, out, out, (T)0.5, (T)0.1, (T)inv_n3,
+                         nullptr, tw);
+      CHS_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+    });
+    CHS_LAUNCH_CHECK(ctx);
+    CHS_HIP(ctx, hipEventSynchronize(ev[3]));
+    for (int k = 0; k < 3; ++k) {
+      float t = 0;
+      CHS_HIP(ctx, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      tot[k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int k = 0; k < 3; ++k) ms[k] = tot[k] / reps;
+  return CHS_OK;
+}
+
+int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
+               double* ms) {
+  if (precision == CHS_C128) return vloc_timed_t<double>(ctx, precision, psi, n_b, out, reps, ms);
+  return vloc_timed_t<float>(ctx, precision, psi, n_b, out, reps, ms);
+}
+
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
                   const void* aux0, const void* /*aux1*/, double c, double inv_e,
                   double* d_norm_part, int* n_part) {
@@ -376,6 +440,126 @@ int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, voi
                                    d_norm_part, n_part);
   return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_norm_part,
                                 n_part);
+}
+
+// --- D: density planes ------------------------------------------------------
+// Per (z-plane, chunk of bands): for each band with occ > 0, sticks -> LDS
+// plane, inverse y-FFT, then row chunks: expand, inverse x-FFT and
+// accumulate occ * |psi(r)|^2 in registers (thread t owns plane elements
+// t + 256 i). The chunk's partial plane goes to part[chunk][z][N^2].
+template <int N>
+__global__ void __launch_bounds__(256) k_density_plane(const z64* __restrict__ sbuf, StickArgs st,
+                                                       const double* __restrict__ occ, int n_b,
+                                                       int chunk, int ch, double* __restrict__ part,
+                                                       const z64* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  z64* cmp = reinterpret_cast<z64*>(smem_raw);
+  const int RS = st.n_act + 1;
+  z64* stage = cmp + N * RS;
+  constexpr int LS = N + 1;
+  constexpr int PER = (N * N + 255) / 256;
+  const int z = blockIdx.x, c = blockIdx.y;
+  const int b0 = c * chunk, b1 = min(n_b, b0 + chunk);
+  double acc[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const double f = occ[b];
+    if (f == 0.0) continue;
+    const z64* plane = sbuf + ((size_t)b * N + z) * st.n_st;
+    __syncthreads();
+    for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = mk(0.0, 0.0);
+    __syncthreads();
+    for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
+    lds_fft_lines<N, 1, double>(cmp, st.n_act, [](int a) { return a; }, RS, tw);
+    for (int r0 = 0; r0 < N; r0 += ch) {
+      const int nr = min(ch, N - r0);
+      for (int i = threadIdx.x; i < ch * LS; i += blockDim.x) stage[i] = mk(0.0, 0.0);
+      __syncthreads();
+      for (int t = threadIdx.x; t < nr * st.n_act; t += blockDim.x) {
+        const int r = t / st.n_act, a = t - r * st.n_act;
+        stage[r * LS + st.act_ix[a]] = cmp[(r0 + r) * RS + a];
+      }
+      lds_fft_lines<N, 1, double>(stage, nr, [](int l) { return l * LS; }, 1, tw);
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        const int row = e / N;
+        if (e < N * N && row >= r0 && row < r0 + nr) {
+          const z64 v = stage[(row - r0) * LS + (e - row * N)];
+          acc[i] += f * (v.x * v.x + v.y * v.y);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double* out = part + ((size_t)c * N + z) * N * N;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < N * N) out[e] = acc[i];
+  }
+}
+
+__global__ void k_density_reduce(const double* __restrict__ part, int n_chunks, size_t n3,
+                                 double inv_omega, double* __restrict__ rho) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  double s = 0;
+  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
+  rho[i] = s * inv_omega;
+}
+
+int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
+                 double* d_rho) {
+  Hamiltonian& H = ctx->impl->ham;
+  if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "no Hamiltonian bound");
+  auto* im = ctx->impl;
+  const StickTable& st = H.st;
+  const int n = H.n_grid;
+  const size_t n3 = (size_t)n * n * n;
+  // only bands up to the last occupied one
+  int nb = 0;
+  for (int b = 0; b < n_b; ++b)
+    if (occ_host[b] != 0.0) nb = b + 1;
+  const int chunk = 16;
+  const int n_chunks = std::max(1, (nb + chunk - 1) / chunk);
+  CHS_TRY(ensure(ctx, im->ws6, (size_t)n_b * sizeof(double) + (size_t)n_chunks * n3 * sizeof(double)));
+  double* docc = (double*)im->ws6.p;
+  double* part = docc + n_b;
+  CHS_HIP(ctx, hipMemcpyAsync(docc, occ_host, n_b * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  ctx->h2d += n_b * sizeof(double);
+  if (nb == 0) {
+    CHS_HIP(ctx, hipMemsetAsync(d_rho, 0, n3 * sizeof(double), ctx->stream));
+    return CHS_OK;
+  }
+  CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * st.n_st * sizeof(z64)));
+  z64* sbuf = (z64*)im->stick_buf.p;
+  const void* twv = nullptr;
+  CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));
+  const z64* tw = (const z64*)twv;
+  StickArgs sa = stick_args(st);
+  const size_t cmp_bytes = (size_t)n * (st.n_act + 1) * sizeof(z64);
+  const size_t row_bytes = (size_t)(n + 1) * sizeof(z64);
+  const int lim = max_lds_per_block(ctx);
+  if (cmp_bytes + row_bytes > (size_t)lim)
+    return set_error(ctx, CHS_ERR_FFT, "density plane of %d^3 grid does not fit in LDS", n);
+  int ch = std::min(32, std::min(n, (int)((lim - cmp_bytes) / row_bytes)));
+  const size_t lds = cmp_bytes + (size_t)ch * row_bytes;
+  const int gz = ceil_div(st.n_st, kSB);
+  CHS_DISPATCH_N(n, {
+    hipLaunchKernelGGL((k_stick_inv<NN, double>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi, sa,
+                       sbuf, tw);
+    CHS_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL((k_density_plane<NN>), dim3(NN, n_chunks), dim3(256), lds, ctx->stream,
+                       sbuf, sa, docc, nb, chunk, ch, part, tw);
+    CHS_LAUNCH_CHECK(ctx);
+  });
+  hipLaunchKernelGGL(k_density_reduce, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, part,
+                     n_chunks, n3, 1.0 / omega, d_rho);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
 }
 
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {

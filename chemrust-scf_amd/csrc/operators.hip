@@ -398,11 +398,56 @@ __global__ void k_minmax(const double* __restrict__ v, size_t n, double* __restr
   }
 }
 
+int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
+               double* ms);
+
+__global__ void k_pattern(double* __restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    a[i] = ((double)(h & 0xFFFF) / 65536.0 - 0.5) * 1e-2;
+  }
+}
+
 }  // namespace chs
 
 using namespace chs;
 
 extern "C" {
+
+int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,
+                     double* bytes3, int32_t* dims4) {
+  if (!ctx || n_b <= 0 || reps <= 0 || !ms3 || (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  CHS_TRY(ensure_precision_ready(ctx, precision));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const size_t s = csize(precision), ne = (size_t)n_b * H.n_pw;
+  CHS_TRY(ensure(ctx, im->ws1, ne * s));
+  CHS_TRY(ensure(ctx, im->ws2, ne * s));
+  const size_t nd = ne * s / sizeof(double);
+  hipLaunchKernelGGL(k_pattern, dim3(ceil_div(nd, 256)), dim3(256), 0, ctx->stream,
+                     (double*)im->ws1.p, nd);
+  hipLaunchKernelGGL(k_pattern, dim3(ceil_div(nd, 256)), dim3(256), 0, ctx->stream,
+                     (double*)im->ws2.p, nd);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(vloc_timed(ctx, precision, im->ws1.p, n_b, im->ws2.p, reps, ms3));
+  const double n = H.n_grid, n3 = n * n * n, nst = H.st.n_st, npw = H.n_pw, v = s / 2.0;
+  const double stick = s * n * nst * n_b;
+  if (bytes3) {
+    bytes3[0] = s * npw * n_b + stick;                            // psi in, sticks out
+    bytes3[1] = 2.0 * stick + v * n3;                              // sticks in/out, V once
+    bytes3[2] = stick + 2.0 * s * npw * n_b + 8.0 * npw + s * npw * n_b;  // sticks, 2 psi, T, out
+  }
+  if (dims4) {
+    dims4[0] = H.n_grid;
+    dims4[1] = H.n_pw;
+    dims4[2] = H.st.n_st;
+    dims4[3] = H.st.n_act;
+  }
+  return CHS_OK;
+}
 
 int chs_apply_h(chs_context* ctx, int precision, const void* d_psi, int n_b, void* d_hpsi,
                 void* d_spsi) {

@@ -1,0 +1,241 @@
+// chs_scf_run: the SCF loop around the eigensolver hot path (run_scf,
+// reference src/scf.rs:358-376; diagonalize, :169-265).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "blas1.hpp"
+#include "impl.hpp"
+#include "linalg.hpp"
+#include "ops.hpp"
+#include "scf_dev.hpp"
+
+namespace chs {
+
+__global__ void k_z2c(const z64* __restrict__ a, c32* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = mk((float)a[i].x, (float)a[i].y);
+}
+__global__ void k_c2z(const c32* __restrict__ a, z64* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = mk((double)a[i].x, (double)a[i].y);
+}
+__global__ void k_fill_d(double* __restrict__ a, double v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Seeded start psi_{b,g} = (x + i y)/(1 + |k+G|^2), x, y ~ N(0,1) from the
+// reference's LCG (src/eigensolver/subspace_diag.rs:434-447) with
+// Box-Muller, drawn band-major then plane-wave order.
+static void initial_psi_host(const std::vector<double>& kin, int n_b, uint64_t seed,
+                             std::vector<double>& psi) {
+  const size_t n_pw = kin.size();
+  psi.resize(2 * (size_t)n_b * n_pw);
+  uint64_t s = seed;
+  auto uni = [&s]() {
+    s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
+    return (double)s / 281474976710656.0;
+  };
+  auto nrm = [&]() {
+    const double u1 = 1.0 - uni();
+    const double u2 = uni();
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+  };
+  for (int b = 0; b < n_b; ++b)
+    for (size_t g = 0; g < n_pw; ++g) {
+      const double x = nrm();
+      const double y = nrm();
+      const double d = 1.0 + 2.0 * kin[g];
+      psi[2 * ((size_t)b * n_pw + g)] = x / d;
+      psi[2 * ((size_t)b * n_pw + g) + 1] = y / d;
+    }
+}
+
+struct ScfState {
+  DevBuf miller, idx, kin, vion, veff, vhxc, rho_in, rho_out, rho_next, psi, psi32;
+  ~ScfState() {
+    for (DevBuf* b : {&miller, &idx, &kin, &vion, &veff, &vhxc, &rho_in, &rho_out, &rho_next, &psi,
+                      &psi32})
+      release(*b);
+  }
+};
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
+                 double* energies, double* eps_out, double* rho_host) {
+  if (p->n_proj_per_atom != 0)
+    return set_error(ctx, CHS_ERR_NOT_IMPLEMENTED, "projector SCF is not available yet");
+  ScfState S;
+  const int n = p->n_grid, n_b = p->n_bands;
+  const double L = p->n_rep * p->a0, omega = L * L * L;
+  const int n_atoms = 8 * p->n_rep * p->n_rep * p->n_rep;
+  const double n_e = p->zval * n_atoms;
+  const size_t n3 = (size_t)n * n * n;
+  // plane waves
+  int n_pw = 0;
+  CHS_TRY(chs_gsphere(ctx, L, p->ecut, p->kfrac, n, 0, &n_pw, nullptr, nullptr, nullptr));
+  CHS_TRY(ensure(ctx, S.miller, 3 * (size_t)n_pw * sizeof(int32_t)));
+  CHS_TRY(ensure(ctx, S.idx, (size_t)n_pw * sizeof(int32_t)));
+  CHS_TRY(ensure(ctx, S.kin, (size_t)n_pw * sizeof(double)));
+  CHS_TRY(chs_gsphere(ctx, L, p->ecut, p->kfrac, n, n_pw, &n_pw, (int32_t*)S.miller.p,
+                      (int32_t*)S.idx.p, (double*)S.kin.p));
+  if (n_b > n_pw) return set_error(ctx, CHS_ERR_INVALID_ARG, "more bands than plane waves");
+  // grids
+  for (DevBuf* b : {&S.vion, &S.veff, &S.vhxc, &S.rho_in, &S.rho_out, &S.rho_next})
+    CHS_TRY(ensure(ctx, *b, n3 * sizeof(double)));
+  GridOps G;
+  CHS_TRY(G.init(ctx, n, L, omega));
+  CHS_TRY(G.vion(pos, n_atoms, p->zval, p->rc_loc, (double*)S.vion.p));
+  // start wavefunctions (host generator, one upload)
+  std::vector<double> kin(n_pw), h_psi;
+  CHS_HIP(ctx, hipMemcpy(kin.data(), S.kin.p, n_pw * sizeof(double), hipMemcpyDeviceToHost));
+  ctx->d2h += n_pw * sizeof(double);
+  initial_psi_host(kin, n_b, p->seed_psi, h_psi);
+  const size_t ne = (size_t)n_b * n_pw;
+  CHS_TRY(ensure(ctx, S.psi, ne * sizeof(z64)));
+  CHS_HIP(ctx, hipMemcpy(S.psi.p, h_psi.data(), ne * sizeof(z64), hipMemcpyHostToDevice));
+  ctx->h2d += ne * sizeof(z64);
+  const bool mixed = p->precision == CHS_C64;
+  if (mixed) CHS_TRY(ensure(ctx, S.psi32, ne * sizeof(c32)));
+  CHS_TRY(chs_bind_hamiltonian(ctx, n, n_pw, (const int32_t*)S.miller.p, (const double*)S.kin.p,
+                               (const double*)S.veff.p));
+  PulayDev mixer;
+  CHS_TRY(mixer.init(ctx, &G, std::max(1, p->pulay_depth), p->mix_alpha, p->kerker_q));
+  hipLaunchKernelGGL(k_fill_d, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream,
+                     (double*)S.rho_in.p, n_e / omega, n3);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+  std::vector<double> eps, occ(n_b, 0.0);
+  double e_prev = 0, mu = 0, etot = 0;
+  int small = 0, it = 0, converged = 0;
+  bool fp64_phase = !mixed;
+  int fp64_iters = 0;
+  const double t0 = now();
+  double t_filter = 0, t_rr = 0, t_den = 0, t_veff = 0, t_lan = 0;
+  for (it = 1; it <= p->max_iter; ++it) {
+    double ta = now();
+    double eh_in, exc_in;
+    CHS_TRY(G.vhxc((const double*)S.rho_in.p, (const double*)S.vion.p, (double*)S.vhxc.p,
+                   (double*)S.veff.p, nullptr, &eh_in, &exc_in, nullptr));
+    CHS_TRY(chs_set_veff(ctx, (const double*)S.veff.p));
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double tb = now();
+    t_veff += tb - ta;
+    double lmax;
+    if (p->lanczos_k > 0) {
+      CHS_TRY(lanczos_impl(ctx, p->lanczos_k, p->seed_lanczos, &lmax, nullptr));
+    } else {
+      CHS_TRY(chs_grid_lambda_max(ctx, L, &lmax));
+    }
+    ta = now();
+    t_lan += ta - tb;
+    const int passes = it == 1 ? std::max(1, p->first_passes) : 1;
+    const bool use32 = mixed && !fp64_phase;
+    for (int ps = 0; ps < passes; ++ps) {
+      double b4[4];
+      chs_spectral_bounds(eps.data(), (int)eps.size(), lmax, b4);
+      double tf0 = now();
+      if (use32) {
+        hipLaunchKernelGGL(k_z2c, dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                           (const z64*)S.psi.p, (c32*)S.psi32.p, ne);
+        CHS_LAUNCH_CHECK(ctx);
+        CHS_TRY(chebyshev_filter_impl(ctx, CHS_C64, S.psi32.p, n_b, p->cheb_degree, b4, S.psi32.p));
+        hipLaunchKernelGGL(k_c2z, dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                           (const c32*)S.psi32.p, (z64*)S.psi.p, ne);
+        CHS_LAUNCH_CHECK(ctx);
+      } else {
+        CHS_TRY(chebyshev_filter_impl(ctx, CHS_C128, S.psi.p, n_b, p->cheb_degree, b4, S.psi.p));
+      }
+      double tf1 = now();
+      t_filter += tf1 - tf0;
+      eps.resize(n_b);
+      CHS_TRY(rayleigh_ritz_impl(ctx, (z64*)S.psi.p, n_b, eps.data(), nullptr, nullptr));
+      t_rr += now() - tf1;
+    }
+    tb = now();
+    double ts = 0;
+    mu = fermi_occupations(eps.data(), n_b, n_e, p->smear_kt, occ.data(), &ts);
+    CHS_TRY(density_impl(ctx, (const z64*)S.psi.p, n_b, occ.data(), omega, (double*)S.rho_out.p));
+    double eh, exc, dc;
+    CHS_TRY(G.vhxc((const double*)S.rho_out.p, (const double*)S.vion.p, nullptr, nullptr,
+                   (const double*)S.vhxc.p, &eh, &exc, &dc));
+    double eband = 0;
+    for (int i = 0; i < n_b; ++i) eband += occ[i] * eps[i];
+    etot = eband - dc + eh + exc + ts;
+    if (energies && it <= p->max_iter) energies[it - 1] = etot;
+    if (fp64_phase) ++fp64_iters;
+    if (it > 1) {
+      const double de = std::abs(etot - e_prev) / n_atoms;
+      if (!fp64_phase && de < 1e-5) {
+        // FP32 filter has reached its useful accuracy: finish in FP64
+        fp64_phase = true;
+        small = 0;
+      } else {
+        small = de < p->etol ? small + 1 : 0;
+      }
+    }
+    e_prev = etot;
+    t_den += now() - tb;
+    if (small >= 2 && fp64_phase && fp64_iters >= (mixed ? std::max(1, p->fp64_final) : 0)) {
+      converged = 1;
+      break;
+    }
+    ta = now();
+    CHS_TRY(mixer.mix((const double*)S.rho_in.p, (const double*)S.rho_out.p,
+                      (double*)S.rho_next.p));
+    std::swap(S.rho_in, S.rho_next);
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    t_den += now() - ta;
+  }
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double t1 = now();
+  res->n_iter = std::min(it, p->max_iter);
+  res->converged = converged;
+  res->n_pw = n_pw;
+  res->n_proj = 0;
+  res->energy = etot;
+  res->mu = mu;
+  res->seconds = t1 - t0;
+  res->t_filter = t_filter;
+  res->t_rr = t_rr;
+  res->t_density = t_den;
+  res->t_veff = t_veff;
+  res->t_lanczos = t_lan;
+  if (eps_out) std::copy(eps.begin(), eps.end(), eps_out);
+  if (rho_host) {
+    CHS_HIP(ctx, hipMemcpy(rho_host, S.rho_out.p, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    ctx->d2h += n3 * sizeof(double);
+  }
+  ctx->impl->ham.bound = false;  // the bound arrays die with S
+  return converged ? CHS_OK : CHS_ERR_NOT_CONVERGED;
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" {
+
+int chs_density(chs_context* ctx, const double* d_psi, int n_b, const double* occ, double omega,
+                double* d_rho) {
+  if (!ctx || !d_psi || n_b <= 0 || !occ || !(omega > 0) || !d_rho) return CHS_ERR_INVALID_ARG;
+  return density_impl(ctx, (const z64*)d_psi, n_b, occ, omega, d_rho);
+}
+
+int chs_scf_run(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
+                double* energies, double* eps, double* rho) {
+  if (!ctx || !p || !pos || !res) return CHS_ERR_INVALID_ARG;
+  if (p->n_rep <= 0 || p->n_bands <= 0 || p->max_iter <= 0 || p->cheb_degree < 0 ||
+      (p->precision != CHS_C128 && p->precision != CHS_C64))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "bad SCF parameters");
+  *res = chs_scf_result{};
+  return scf_run_impl(ctx, p, pos, res, energies, eps, rho);
+}
+
+}  // extern "C"

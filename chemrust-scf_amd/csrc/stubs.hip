@@ -22,11 +22,4 @@ int chs_bind_projectors(chs_context* ctx, int, const double*, const double*, con
 int chs_set_dion(chs_context* ctx, const double*) {
   return set_error(ctx, CHS_ERR_NOT_IMPLEMENTED, "not implemented");
 }
-int chs_density(chs_context* ctx, const double*, int, const double*, double, double*) {
-  return set_error(ctx, CHS_ERR_NOT_IMPLEMENTED, "not implemented");
-}
-int chs_scf_run(chs_context* ctx, const chs_scf_params*, const double*, chs_scf_result*, double*,
-                double*, double*) {
-  return set_error(ctx, CHS_ERR_NOT_IMPLEMENTED, "not implemented");
-}
 }

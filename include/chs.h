@@ -180,6 +180,17 @@ int chs_zdotc(chs_context* ctx, int n, const double* d_x, const double* d_y, dou
 int chs_zaxpy(chs_context* ctx, int n, const double alpha[2], const double* d_x, double* d_y);
 int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
 
+/* ---- measurement (no reference counterpart: the reference was never
+ * timed) ----------------------------------------------------------------
+ * Runs the three kernels of one steady-state Chebyshev step of the bound
+ * Hamiltonian on n_b synthetic bands `reps` times, with HIP events on the
+ * context stream around each launch. ms3: mean milliseconds of kernel
+ * A (scatter + inverse z-FFT), B (plane FFTs x V_eff), C (forward z-FFT +
+ * gather + update); bytes3: algorithmic HBM bytes per launch of each;
+ * dims4: {n_grid, n_pw, sticks, active x columns}. */
+int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,
+                     double* bytes3, int32_t* dims4);
+
 /* ---- SCF -------------------------------------------------------------- */
 /* Density from band-major c128 wavefunctions with occupations (host f64,
  * n_b): rho(r) = sum_i f_i |psi_i(r)|^2 / Omega (+ augmentation with
