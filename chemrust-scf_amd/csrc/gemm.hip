@@ -1,0 +1,181 @@
+// Complex GEMM C = alpha op(A) op(B) + beta C (column-major, op in {N, T, C})
+// in FP64 (ZGEMM, BlasHandle::gemm_c64, src/device/blas.rs:24-61) and FP32
+// with FP32 accumulation (the projector products of the c64 filter).
+//
+// Block tile 64 x 64 with 256 threads (4 x 4 complex outputs per thread),
+// K tiles of 16 staged in LDS. When the output tiles alone cannot fill the
+// device (the tall-skinny Gram products Y^H X with K = n_pw), K is split
+// across blocks; each slice writes its partial tile and k_gemm_reduce sums
+// the slices in a fixed order (deterministic).
+#include <algorithm>
+
+#include "impl.hpp"
+#include "linalg.hpp"
+
+namespace chs {
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+template <typename T, int OP>
+__device__ __forceinline__ cplx<T> load_op(const cplx<T>* __restrict__ A, int lda, int r, int c) {
+  if (OP == 0) return A[(size_t)r + (size_t)c * lda];
+  cplx<T> v = A[(size_t)c + (size_t)r * lda];
+  return OP == 2 ? conj(v) : v;
+}
+
+template <typename T, int OPA, int OPB>
+__global__ void __launch_bounds__(256) k_gemm(int m, int n, int k, cplx<T> alpha,
+                                              const cplx<T>* __restrict__ A, int lda,
+                                              const cplx<T>* __restrict__ B, int ldb, cplx<T> beta,
+                                              cplx<T>* __restrict__ C, int ldc, int klen,
+                                              cplx<T>* __restrict__ part) {
+  __shared__ cplx<T> As[GBK][GBM + 1];
+  __shared__ cplx<T> Bs[GBK][GBN + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+  const int kb = blockIdx.z * klen, ke = min(k, kb + klen);
+  T ar[4][4], ai[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) ar[i][j] = ai[i][j] = (T)0;
+  for (int k0 = kb; k0 < ke; k0 += GBK) {
+    for (int t = threadIdx.x; t < GBM * GBK; t += 256) {
+      int r, kk;
+      if (OPA == 0) { r = t % GBM; kk = t / GBM; } else { kk = t % GBK; r = t / GBK; }
+      const int gr = m0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
+    }
+    for (int t = threadIdx.x; t < GBK * GBN; t += 256) {
+      int c, kk;
+      if (OPB == 0) { kk = t % GBK; c = t / GBK; } else { c = t % GBN; kk = t / GBN; }
+      const int gc = n0 + c, gk = k0 + kk;
+      Bs[kk][c] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; ++kk) {
+      cplx<T> a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ar[i][j] = fma(a[i].x, b[j].x, ar[i][j]);
+          ar[i][j] = fma(-a[i].y, b[j].y, ar[i][j]);
+          ai[i][j] = fma(a[i].x, b[j].y, ai[i][j]);
+          ai[i][j] = fma(a[i].y, b[j].x, ai[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = m0 + tx + 16 * i, c = n0 + ty + 16 * j;
+      if (r < m && c < n) {
+        const cplx<T> acc = mk(ar[i][j], ai[i][j]);
+        if (part) {
+          part[(size_t)blockIdx.z * m * n + (size_t)r + (size_t)c * m] = acc;
+        } else {
+          cplx<T> v = alpha * acc;
+          if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * C[(size_t)r + (size_t)c * ldc];
+          C[(size_t)r + (size_t)c * ldc] = v;
+        }
+      }
+    }
+}
+
+template <typename T>
+__global__ void k_gemm_reduce(int m, int n, int nz, const cplx<T>* __restrict__ part,
+                              cplx<T> alpha, cplx<T> beta, cplx<T>* __restrict__ C, int ldc) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)m * n) return;
+  cplx<T> s = mk((T)0, (T)0);
+  for (int z = 0; z < nz; ++z) s = s + part[(size_t)z * m * n + idx];
+  const int r = (int)(idx % m), c = (int)(idx / m);
+  cplx<T> v = alpha * s;
+  if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * C[(size_t)r + (size_t)c * ldc];
+  C[(size_t)r + (size_t)c * ldc] = v;
+}
+
+int gemm_opcode(char op) {
+  switch (op) {
+    case 'N': case 'n': return 0;
+    case 'T': case 't': return 1;
+    case 'C': case 'c': return 2;
+    default: return -1;
+  }
+}
+
+template <typename T>
+static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cplx<T> alpha,
+                  const cplx<T>* A, int lda, const cplx<T>* B, int ldb, cplx<T> beta, cplx<T>* C,
+                  int ldc) {
+  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (m == 0 || n == 0) return CHS_OK;
+  const int tiles = (int)(ceil_div(m, GBM) * ceil_div(n, GBN));
+  int ksplit = 1;
+  if (tiles < 512 && k > 4 * GBK) {
+    ksplit = std::min(512 / tiles + 1, (k + 255) / 256);
+    ksplit = std::max(ksplit, 1);
+  }
+  int klen = (k + ksplit - 1) / ksplit;
+  klen = ((klen + GBK - 1) / GBK) * GBK;
+  if (klen == 0) klen = GBK;
+  ksplit = std::max(1, (k + klen - 1) / klen);
+  cplx<T>* part = nullptr;
+  if (ksplit > 1) {
+    CHS_TRY(ensure(ctx, ctx->impl->gemm_ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
+    part = (cplx<T>*)ctx->impl->gemm_ws.p;
+  }
+  dim3 grid(ceil_div(m, GBM), ceil_div(n, GBN), ksplit);
+#define CHS_GEMM_CASE(OA, OB)                                                                   \
+  if (oa == OA && ob == OB)                                                                     \
+    hipLaunchKernelGGL((k_gemm<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k, alpha, A, \
+                       lda, B, ldb, beta, C, ldc, klen, part);
+  CHS_GEMM_CASE(0, 0) CHS_GEMM_CASE(0, 1) CHS_GEMM_CASE(0, 2)
+  CHS_GEMM_CASE(1, 0) CHS_GEMM_CASE(1, 1) CHS_GEMM_CASE(1, 2)
+  CHS_GEMM_CASE(2, 0) CHS_GEMM_CASE(2, 1) CHS_GEMM_CASE(2, 2)
+#undef CHS_GEMM_CASE
+  CHS_LAUNCH_CHECK(ctx);
+  if (ksplit > 1) {
+    hipLaunchKernelGGL((k_gemm_reduce<T>), dim3(ceil_div((size_t)m * n, 256)), dim3(256), 0,
+                       ctx->stream, m, n, ksplit, part, alpha, beta, C, ldc);
+    CHS_LAUNCH_CHECK(ctx);
+  }
+  return CHS_OK;
+}
+
+int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
+          int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc) {
+  return gemm_t<double>(ctx, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,
+          int lda, const c32* B, int ldb, c32 beta, c32* C, int ldc) {
+  return gemm_t<float>(ctx, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k,
+                         const double alpha[2], const double* d_a, int lda, const double* d_b,
+                         int ldb, const double beta[2], double* d_c, int ldc) {
+  if (!ctx || !alpha || !beta || (m > 0 && n > 0 && (!d_c || (k > 0 && (!d_a || !d_b)))))
+    return CHS_ERR_INVALID_ARG;
+  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (ldc < std::max(1, m) || lda < std::max(1, oa == 0 ? m : k) ||
+      ldb < std::max(1, ob == 0 ? k : n))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "zgemm: leading dimension too small");
+  return zgemm(ctx, opa, opb, m, n, k, mk(alpha[0], alpha[1]), (const z64*)d_a, lda,
+               (const z64*)d_b, ldb, mk(beta[0], beta[1]), (z64*)d_c, ldc);
+}

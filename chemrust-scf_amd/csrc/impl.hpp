@@ -54,7 +54,7 @@ struct chs_context::Impl {
   chs::Hamiltonian ham;
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
-  chs::DevBuf gemm_ws, lin_ws, scf_ws;
+  chs::DevBuf gemm_ws, lin_ws, scf_ws, proj_ws;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 

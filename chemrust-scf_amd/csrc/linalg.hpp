@@ -6,6 +6,9 @@ namespace chs {
 
 int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
           int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc);
+int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,
+          int lda, const c32* B, int ldb, c32 beta, c32* C, int ldc);
+int gemm_opcode(char op);
 int hermitise(chs_context* ctx, z64* M, int n);
 int cholesky(chs_context* ctx, z64* A, int n, int* info_host);
 int tri_inv(chs_context* ctx, const z64* L, z64* X, int n);

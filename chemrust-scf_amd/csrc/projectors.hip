@@ -1,0 +1,390 @@
+// Nonlocal (USPP) projector terms as complex GEMMs.
+//
+// Reference: apply_v_nl_hamiltonian (src/eigensolver/chebyshev.rs:416-500)
+// runs, per ion, three skinny ZGEMMs (beta^H psi, D C, beta C) with two
+// allocations each; the gate tests' operator form adds S = 1 + beta Q
+// beta^H (tests/phase0_gate1_consistency.rs:320-385). Here all ions'
+// projectors form one (n_pw x P) block and each term is one GEMM pair over
+// all P projectors and all bands, with the diagonal D (or Q) applied as a
+// row scale between them:
+//   W = beta^H psi        (P x n_b)
+//   H psi += beta (D W),  S psi = psi + beta (Q W)
+// and S^{-1} x = x - beta M beta^H x with M = (Q^{-1} + beta^H beta)^{-1}
+// (Woodbury; the same identity as uspp_preconditioner.rs:80-131).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "blas1.hpp"
+#include "impl.hpp"
+#include "linalg.hpp"
+#include "ops.hpp"
+#include "projectors.hpp"
+
+namespace chs {
+
+template <typename T>
+__global__ void k_row_scale2(const cplx<T>* __restrict__ w, int P, int n_b,
+                             const double* __restrict__ d, const double* __restrict__ q,
+                             cplx<T>* __restrict__ dw, cplx<T>* __restrict__ qw) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)P * n_b) return;
+  const int p = (int)(i % P);
+  const cplx<T> v = w[i];
+  const T dp = (T)d[p];
+  dw[i] = mk(dp * v.x, dp * v.y);
+  if (qw) {
+    const T qp = (T)q[p];
+    qw[i] = mk(qp * v.x, qp * v.y);
+  }
+}
+
+__global__ void k_diag_add_inv(z64* __restrict__ a, const double* __restrict__ q, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < P) a[(size_t)i + (size_t)i * P].x += 1.0 / q[i];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = mk((decltype(b[i].x))a[i].x, (decltype(b[i].x))a[i].y);
+}
+
+static int ensure_c64_projectors(chs_context* ctx) {
+  Hamiltonian& H = ctx->impl->ham;
+  const size_t nb = (size_t)H.n_proj * H.n_pw;
+  if (!H.beta_f.p || H.beta_f.bytes < nb * sizeof(c32)) {
+    CHS_TRY(ensure(ctx, H.beta_f, nb * sizeof(c32)));
+    hipLaunchKernelGGL((k_convert<z64, c32>), dim3(ceil_div(nb, 256)), dim3(256), 0, ctx->stream,
+                       (const z64*)H.beta, (c32*)H.beta_f.p, nb);
+    CHS_LAUNCH_CHECK(ctx);
+    if (H.has_s) {
+      const size_t pp = (size_t)H.n_proj * H.n_proj;
+      CHS_TRY(ensure(ctx, H.sinv_m_f, pp * sizeof(c32)));
+      hipLaunchKernelGGL((k_convert<z64, c32>), dim3(ceil_div(pp, 256)), dim3(256), 0,
+                         ctx->stream, (const z64*)H.sinv_m.p, (c32*)H.sinv_m_f.p, pp);
+      CHS_LAUNCH_CHECK(ctx);
+    }
+  }
+  return CHS_OK;
+}
+
+template <typename T>
+static int gemm_any(chs_context* ctx, char oa, char ob, int m, int n, int k, double al,
+                    const cplx<T>* A, int lda, const cplx<T>* B, int ldb, double be, cplx<T>* C,
+                    int ldc);
+template <>
+int gemm_any<double>(chs_context* ctx, char oa, char ob, int m, int n, int k, double al,
+                     const z64* A, int lda, const z64* B, int ldb, double be, z64* C, int ldc) {
+  return zgemm(ctx, oa, ob, m, n, k, mk(al, 0.0), A, lda, B, ldb, mk(be, 0.0), C, ldc);
+}
+template <>
+int gemm_any<float>(chs_context* ctx, char oa, char ob, int m, int n, int k, double al,
+                    const c32* A, int lda, const c32* B, int ldb, double be, c32* C, int ldc) {
+  return cgemm(ctx, oa, ob, m, n, k, mk((float)al, 0.f), A, lda, B, ldb, mk((float)be, 0.f), C,
+               ldc);
+}
+
+template <typename T>
+static int nonlocal_t(chs_context* ctx, const cplx<T>* psi, int n_b, cplx<T>* hpsi,
+                      cplx<T>* spsi) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  const cplx<T>* beta =
+      sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
+  const size_t pb = (size_t)P * n_b;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 3 * pb * sizeof(cplx<T>)));
+  cplx<T>* W = (cplx<T>*)ctx->impl->proj_ws.p;
+  cplx<T>* DW = W + pb;
+  cplx<T>* QW = DW + pb;
+  CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, psi, n_pw, 0.0, W, P));
+  hipLaunchKernelGGL((k_row_scale2<T>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream, W, P,
+                     n_b, H.dion, H.qion, DW, spsi ? QW : nullptr);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, DW, P, 1.0, hpsi, n_pw));
+  if (spsi) {
+    if (spsi != psi)
+      CHS_HIP(ctx, hipMemcpyAsync(spsi, psi, (size_t)n_b * n_pw * sizeof(cplx<T>),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+    if (H.has_s)
+      CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, QW, P, 1.0, spsi, n_pw));
+  }
+  return CHS_OK;
+}
+
+int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
+                   void* spsi) {
+  Hamiltonian& H = ctx->impl->ham;
+  if (H.n_proj == 0) return CHS_OK;
+  if (precision == CHS_C128)
+    return nonlocal_t<double>(ctx, (const z64*)psi, n_b, (z64*)hpsi, (z64*)spsi);
+  CHS_TRY(ensure_c64_projectors(ctx));
+  return nonlocal_t<float>(ctx, (const c32*)psi, n_b, (c32*)hpsi, (c32*)spsi);
+}
+
+template <typename T>
+static int sinv_t(chs_context* ctx, cplx<T>* x, int n_b) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  const cplx<T>* beta =
+      sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
+  const cplx<T>* M =
+      sizeof(T) == 8 ? (const cplx<T>*)H.sinv_m.p : (const cplx<T>*)H.sinv_m_f.p;
+  const size_t pb = (size_t)P * n_b;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 3 * pb * sizeof(cplx<T>)));
+  cplx<T>* W = (cplx<T>*)ctx->impl->proj_ws.p;
+  cplx<T>* MW = W + pb;
+  CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, x, n_pw, 0.0, W, P));
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, M, P, W, P, 0.0, MW, P));
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, -1.0, beta, n_pw, MW, P, 1.0, x, n_pw));
+  return CHS_OK;
+}
+
+int sinv_apply(chs_context* ctx, int precision, void* x, int n_b) {
+  Hamiltonian& H = ctx->impl->ham;
+  if (H.n_proj == 0 || !H.has_s) return CHS_OK;
+  if (precision == CHS_C128) return sinv_t<double>(ctx, (z64*)x, n_b);
+  CHS_TRY(ensure_c64_projectors(ctx));
+  return sinv_t<float>(ctx, (c32*)x, n_b);
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic projectors, augmentation and screening (SCF setup / glue)
+// ---------------------------------------------------------------------------
+
+// beta_{a,j}(k+G) = Omega^{-1/2} f_j(k+G) e^{-i(k+G).R_a}, j = (l=0), (l=1; x, y, z):
+// f_0 = Y00 pi^{3/2} s^3 exp(-G^2 s^2/4); f_x = sqrt(3/4pi) (-i s^2 G_x/2) pi^{3/2} s^3 exp(..)
+__global__ void k_build_beta(const int32_t* __restrict__ miller, int n_pw, double b, double kf0,
+                             double kf1, double kf2, const double* __restrict__ pos, int n_atoms,
+                             int np, double s, double omega, z64* __restrict__ beta) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n_pw * n_atoms) return;
+  const int g = (int)(i % n_pw), a = (int)(i / n_pw);
+  const double gx = b * (miller[3 * g] + kf0), gy = b * (miller[3 * g + 1] + kf1),
+               gz = b * (miller[3 * g + 2] + kf2);
+  const double g2 = gx * gx + gy * gy + gz * gz;
+  const double pref = pow(M_PI, 1.5) * s * s * s / sqrt(omega);
+  const double rad = pref * exp(-g2 * s * s / 4.0);
+  const double t = -(gx * pos[3 * a] + gy * pos[3 * a + 1] + gz * pos[3 * a + 2]);
+  double sn, cs;
+  sincos(t, &sn, &cs);
+  const double y00 = 1.0 / sqrt(4.0 * M_PI), y1 = sqrt(3.0 / (4.0 * M_PI));
+  const size_t base = (size_t)a * np * n_pw + g;
+  beta[base] = mk(y00 * rad * cs, y00 * rad * sn);
+  if (np > 1) {
+    // y1 rad g_a (-i s^2/2) (cs + i sn) = y1 rad g_a (s^2/2) (sn - i cs)
+    const double h = 0.5 * s * s * y1 * rad;
+    const double ga[3] = {gx, gy, gz};
+    for (int j = 0; j < 3; ++j)
+      beta[base + (size_t)(1 + j) * n_pw] = mk(h * ga[j] * sn, -h * ga[j] * cs);
+  }
+}
+
+// rho_p = sum_b occ_b |W[p, b]|^2 (W = beta^H psi, P x n_b)
+__global__ void k_proj_occ(const z64* __restrict__ w, int P, int n_b, const double* __restrict__ occ,
+                           double* __restrict__ rho_p) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  double s = 0;
+  for (int b = 0; b < n_b; ++b) s += occ[b] * norm2(w[(size_t)p + (size_t)b * P]);
+  rho_p[p] = s;
+}
+
+// aug(G) = Omega^-1 exp(-G^2 rq^2/4) sum_a ra[a] e^{-iG.R_a} (Nyquist zero)
+__global__ void k_aug_g(z64* __restrict__ g, int n, int n_atoms, const z64* __restrict__ ph,
+                        const double* __restrict__ ra, double b, double rq, double omega) {
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n3 = (size_t)n * n * n;
+  if (id >= n3) return;
+  const int ix = (int)(id % n), iy = (int)((id / n) % n), iz = (int)(id / ((size_t)n * n));
+  if ((n % 2 == 0) && (ix == n / 2 || iy == n / 2 || iz == n / 2)) {
+    g[id] = mk(0.0, 0.0);
+    return;
+  }
+  const int h = ix <= n / 2 ? ix : ix - n, k = iy <= n / 2 ? iy : iy - n, l = iz <= n / 2 ? iz : iz - n;
+  const double g2 = b * b * (double)(h * h + k * k + l * l);
+  z64 s = mk(0.0, 0.0);
+  for (int a = 0; a < n_atoms; ++a) {
+    const z64* pa = ph + (size_t)a * 3 * n;
+    const z64 e = pa[ix] * pa[n + iy] * pa[2 * n + iz];
+    s = s + ra[a] * e;
+  }
+  const double f = exp(-g2 * rq * rq / 4.0) / omega;
+  g[id] = mk(f * s.x, f * s.y);
+}
+
+__global__ void k_real_to_complex_proj(const double* __restrict__ r, z64* __restrict__ g, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = mk(r[i], 0.0);
+}
+
+__global__ void k_add_real(const z64* __restrict__ g, double* __restrict__ r, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] += g[i].x;
+}
+
+// part[a * nchunk + c] = sum over the chunk of Re(V(G) e^{+iG.R_a}) exp(-G^2 rq^2/4)
+__global__ void __launch_bounds__(256) k_screen(const z64* __restrict__ vg, int n, int n_atoms,
+                                                const z64* __restrict__ ph, double b, double rq,
+                                                double* __restrict__ part) {
+  __shared__ double sh[8];
+  const int a = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
+  const size_t n3 = (size_t)n * n * n;
+  const z64* pa = ph + (size_t)a * 3 * n;
+  double s = 0;
+  for (size_t id = (size_t)c * blockDim.x + threadIdx.x; id < n3; id += (size_t)nchunk * blockDim.x) {
+    const int ix = (int)(id % n), iy = (int)((id / n) % n), iz = (int)(id / ((size_t)n * n));
+    if ((n % 2 == 0) && (ix == n / 2 || iy == n / 2 || iz == n / 2)) continue;
+    const int h = ix <= n / 2 ? ix : ix - n, k = iy <= n / 2 ? iy : iy - n,
+              l = iz <= n / 2 ? iz : iz - n;
+    const double g2 = b * b * (double)(h * h + k * k + l * l);
+    const z64 e = conj(pa[ix] * pa[n + iy] * pa[2 * n + iz]);
+    const z64 v = vg[id];
+    s += (v.x * e.x - v.y * e.y) * exp(-g2 * rq * rq / 4.0);
+  }
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) part[(size_t)a * nchunk + c] = r;
+}
+
+__global__ void k_screen_finish(const double* __restrict__ part, int nchunk, int n_atoms, int np,
+                                const double* __restrict__ d0, double q0, double inv_n3,
+                                double* __restrict__ d) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_atoms) return;
+  double s = 0;
+  for (int c = 0; c < nchunk; ++c) s += part[(size_t)a * nchunk + c];
+  for (int j = 0; j < np; ++j) d[a * np + j] = d0[a * np + j] + q0 * s * inv_n3;
+}
+
+int build_beta(chs_context* ctx, const int32_t* miller, int n_pw, double L, const double* kf,
+               const double* d_pos, int n_atoms, int np, double s, z64* beta) {
+  const double b = 2.0 * M_PI / L;
+  hipLaunchKernelGGL(k_build_beta, dim3(ceil_div((size_t)n_pw * n_atoms, 256)), dim3(256), 0,
+                     ctx->stream, miller, n_pw, b, kf[0], kf[1], kf[2], d_pos, n_atoms, np, s,
+                     L * L * L, beta);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int augment_density(chs_context* ctx, const z64* psi, int n_b, const double* d_occ, int n_atoms,
+                    int np, double q0, double rq, const z64* d_ph, int n, double L, z64* cgrid,
+                    double* rho) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  const size_t pb = (size_t)P * n_b;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 3 * pb * sizeof(z64) + (P + n_atoms) * sizeof(double)));
+  z64* W = (z64*)ctx->impl->proj_ws.p;
+  double* rho_p = (double*)(W + 3 * pb);
+  double* ra = rho_p + P;
+  CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0, 0.0), (const z64*)H.beta, n_pw, psi, n_pw,
+                mk(0.0, 0.0), W, P));
+  hipLaunchKernelGGL(k_proj_occ, dim3(ceil_div(P, 256)), dim3(256), 0, ctx->stream, W, P, n_b,
+                     d_occ, rho_p);
+  CHS_LAUNCH_CHECK(ctx);
+  // ra[a] = q0 sum_j rho_{a,j}
+  std::vector<double> hp(P);
+  CHS_HIP(ctx, hipMemcpyAsync(hp.data(), rho_p, P * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<double> hra(n_atoms, 0.0);
+  for (int a = 0; a < n_atoms; ++a)
+    for (int j = 0; j < np; ++j) hra[a] += q0 * hp[a * np + j];
+  CHS_HIP(ctx, hipMemcpyAsync(ra, hra.data(), n_atoms * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  const size_t n3 = (size_t)n * n * n;
+  hipLaunchKernelGGL(k_aug_g, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, cgrid, n,
+                     n_atoms, d_ph, ra, 2.0 * M_PI / L, rq, L * L * L);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(fft3d(ctx, CHS_C128, cgrid, n, 1, +1));
+  hipLaunchKernelGGL(k_add_real, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, cgrid, rho,
+                     n3);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int screen_dion(chs_context* ctx, const double* veff, int n, double L, int n_atoms, int np,
+                double q0, double rq, const z64* d_ph, const double* d_d0, z64* cgrid,
+                double* d_out) {
+  const size_t n3 = (size_t)n * n * n;
+  const int nchunk = 64;
+  CHS_TRY(ensure(ctx, ctx->impl->part, sizeof(double) * ((size_t)n_atoms * nchunk + 2 * kRedBlocks)));
+  hipLaunchKernelGGL(k_real_to_complex_proj, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream,
+                     veff, cgrid, n3);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(fft3d(ctx, CHS_C128, cgrid, n, 1, -1));
+  hipLaunchKernelGGL(k_screen, dim3(nchunk, n_atoms), dim3(256), 0, ctx->stream, cgrid, n,
+                     n_atoms, d_ph, 2.0 * M_PI / L, rq, (double*)ctx->impl->part.p);
+  CHS_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_screen_finish, dim3(ceil_div(n_atoms, 256)), dim3(256), 0, ctx->stream,
+                     (const double*)ctx->impl->part.p, nchunk, n_atoms, np, d_d0, q0,
+                     1.0 / (double)n3, d_out);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+}  // namespace chs
+
+using namespace chs;
+
+extern "C" {
+
+int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, const double* d_d,
+                        const double* d_q) {
+  if (!ctx || n_proj < 0 || (n_proj > 0 && (!d_beta || !d_d || !d_q))) return CHS_ERR_INVALID_ARG;
+  Hamiltonian& H = ctx->impl->ham;
+  if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "bind the Hamiltonian first");
+  H.n_proj = n_proj;
+  H.beta = d_beta;
+  H.dion = d_d;
+  H.qion = d_q;
+  H.has_s = false;
+  release(H.beta_f);
+  release(H.sinv_m_f);
+  if (n_proj == 0) return CHS_OK;
+  std::vector<double> q(n_proj);
+  CHS_HIP(ctx, hipMemcpyAsync(q.data(), d_q, n_proj * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += n_proj * sizeof(double);
+  bool any = false, all = true;
+  for (double v : q) {
+    any = any || v != 0.0;
+    all = all && v != 0.0;
+  }
+  if (any && !all)
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "Q must be zero for all or no projectors");
+  if (!any) return CHS_OK;
+  // M = (Q^{-1} + beta^H beta)^{-1} through Cholesky
+  const int P = n_proj;
+  const size_t pp = (size_t)P * P;
+  CHS_TRY(ensure(ctx, H.sinv_m, pp * sizeof(z64)));
+  CHS_TRY(ensure(ctx, ctx->impl->lin_ws, 2 * pp * sizeof(z64)));
+  z64* A = (z64*)ctx->impl->lin_ws.p;
+  z64* Li = A + pp;
+  CHS_TRY(zgemm(ctx, 'C', 'N', P, P, H.n_pw, mk(1.0, 0.0), (const z64*)d_beta, H.n_pw,
+                (const z64*)d_beta, H.n_pw, mk(0.0, 0.0), A, P));
+  CHS_TRY(hermitise(ctx, A, P));
+  hipLaunchKernelGGL(k_diag_add_inv, dim3(ceil_div(P, 256)), dim3(256), 0, ctx->stream, A, d_q, P);
+  CHS_LAUNCH_CHECK(ctx);
+  int info = 0;
+  CHS_TRY(cholesky(ctx, A, P, &info));
+  if (info)
+    return set_error(ctx, CHS_ERR_SOLVER, "Q^-1 + beta^H beta is not positive definite (column %d)",
+                     info);
+  CHS_TRY(tri_inv(ctx, A, Li, P));
+  CHS_TRY(zgemm(ctx, 'C', 'N', P, P, P, mk(1.0, 0.0), Li, P, Li, P, mk(0.0, 0.0),
+                (z64*)H.sinv_m.p, P));
+  H.has_s = true;
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CHS_OK;
+}
+
+int chs_set_dion(chs_context* ctx, const double* d_d) {
+  if (!ctx || !d_d) return CHS_ERR_INVALID_ARG;
+  Hamiltonian& H = ctx->impl->ham;
+  if (H.n_proj == 0) return set_error(ctx, CHS_ERR_INVALID_ARG, "no projectors bound");
+  H.dion = d_d;
+  return CHS_OK;
+}
+
+}  // extern "C"
