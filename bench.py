@@ -125,6 +125,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--hpsi-only", action="store_true",
+                    help="only time the H|psi> kernels (for counter runs) and print the roofline")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="with --hpsi-only: also run 4 device-to-device copies of this many "
+                         "bytes (the counter scale reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -133,6 +138,16 @@ def main():
     import chs
     ctx = chs.Context(0)
     cfg = CONFIGS[args.config]
+    if args.hpsi_only:
+        if args.copy_bytes:
+            import numpy as np
+            a = chs.Gpu(ctx, (args.copy_bytes,), np.uint8).zero()
+            b = chs.Gpu(ctx, (args.copy_bytes,), np.uint8)
+            for _ in range(4):
+                ctx.check(chs.lib().chs_d2d(ctx.h, b.ptr, a.ptr, args.copy_bytes))
+            ctx.synchronize()
+        print(json.dumps(roofline(chs, ctx, cfg)))
+        return
     pos = chs.synth_positions(cfg["n_rep"])
     params = chs.scf_params(**cfg, max_iter=100)
     for _ in range(args.warmup):

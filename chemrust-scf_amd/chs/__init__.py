@@ -125,6 +125,7 @@ def lib():
             "chs_density": [vp, vp, i, dp, d, vp],
             "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
             "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
+            "chs_band_cg": [vp, vp, i, i, d, i, vp, dp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -443,6 +444,21 @@ def run_scf(ctx, params, pos, want_rho=False):
     out["rho"] = rho
     out["status"] = rc
     return out
+
+
+def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None):
+    """band_cg_minimize (src/eigensolver/band_cg.rs:157-437) for every band of the
+    c128 device block psi in place; band i against bands j < i of the block as
+    handed in. precond: 0 TPA, 1 TPA + USPP term, 2 MLP scale table (device f32,
+    n_b x n_pw). Returns (eps, n_steps, converged)."""
+    n_b = psi.shape[0]
+    eps = np.zeros(n_b)
+    steps = np.zeros(n_b, np.int32)
+    conv = np.zeros(n_b, np.int32)
+    ctx.check(lib().chs_band_cg(ctx.h, psi.ptr, n_b, max_steps, tol, precond,
+                                mlp_scale.ptr if mlp_scale is not None else None, _dptr(eps),
+                                steps.ctypes.data_as(vp), conv.ctypes.data_as(vp)))
+    return eps, steps, conv
 
 
 def rayleigh_ritz(ctx, y, want_hy=False):

@@ -189,6 +189,81 @@ __device__ __forceinline__ void lds_fft_lines(cplx<T>* lds, int nlines, Off off,
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Lane-group line FFT: one line of length N = N1 * N2 in LDS is transformed
+// by the N2 lanes of a group that lives inside one wavefront, so the two
+// exchange points need only wavefront-scope ordering, not a workgroup
+// barrier. Lane n2 does the DFT_N1 of x[n1*N2 + n2] and its twiddles in
+// registers; lane k1 (and k1 + N2, ...) then does the DFT_N2 of row k1 and
+// hands X[k1 + N1*k2] to `post(k, v)`, whose result is stored in place.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+struct PostId {
+  template <typename T> __device__ __forceinline__ cplx<T> operator()(int, cplx<T> v) const {
+    return v;
+  }
+};
+
+template <int N, int S, typename T, typename Post>
+__device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
+                                               const cplx<T>* __restrict__ tw, Post post) {
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int R = (N1 + N2 - 1) / N2;
+  {
+    cplx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = base[(n1 * N2 + lane) * es];
+    Dft<N1, S, T>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * twiddle<S>(tw, lane * k1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) base[(k1 * N2 + lane) * es] = a[k1];
+  }
+  wave_sync();
+  cplx<T> b[R][N2];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[r][n2] = base[(k1 * N2 + n2) * es];
+      Dft<N2, S, T>::run(b[r]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int k = k1 + N1 * k2;
+        base[k * es] = post(k, b[r][k2]);
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Group geometry of a block: groups of N2 lanes packed into each wavefront.
+template <int N> struct Groups {
+  static constexpr int N2 = Split<N>::N2;
+  static constexpr int PER_WAVE = 64 / N2;
+  __device__ __forceinline__ static int count() { return (int)(blockDim.x >> 6) * PER_WAVE; }
+  // group id (or -1 for lanes outside any group) and lane within the group
+  __device__ __forceinline__ static int id(int* lane) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int g = l / N2;
+    *lane = l - g * N2;
+    return g < PER_WAVE ? w * PER_WAVE + g : -1;
+  }
+};
+
 // Dispatch a functor templated on the supported line length N.
 #define CHS_DISPATCH_N(n, ...)                 \
   switch (n) {                                  \

@@ -9,15 +9,18 @@
 //                    into LDS, inverse z-FFT, write the stick buffer
 //                    sbuf[b][z][stick] (coalesced over sticks);
 //   B  k_plane     : per (band, z-plane): sticks -> LDS plane in compact
-//                    form [iy][active ix], inverse y-FFT on the active
-//                    columns, then row chunks: expand to full x rows,
-//                    inverse x-FFT, x V_eff(r), forward x-FFT, compact;
-//                    forward y-FFT, sticks back to sbuf (in place);
-//   C  k_stick_fwd : per (band, 32 sticks): forward z-FFT in LDS, gather the
-//                    plane waves, x 1/N^3, + 0.5|k+G|^2 psi, and the
-//                    Chebyshev three-term update of chebyshev.rs:700-762
-//                    fused in the same epilogue, with per-block norm
-//                    partial sums for the divergence guard (:546-563).
+//                    form [iy][active ix]; inverse y-FFT on the active
+//                    columns; per row: expand to a full x row, inverse
+//                    x-FFT with V_eff(r) applied as the transform's last
+//                    store, forward x-FFT, compact; forward y-FFT; sticks
+//                    back to sbuf (in place);
+//   C  k_stick_fwd : per (band, 32 sticks): forward z-FFT, gather the plane
+//                    waves, x 1/N^3, + 0.5|k+G|^2 psi, and the Chebyshev
+//                    three-term update of chebyshev.rs:700-762 in the same
+//                    epilogue, with per-block norm partial sums for the
+//                    divergence guard (:546-563).
+// Every 1D line is transformed by a lane group inside one wavefront
+// (group_fft_line), so workgroup barriers are needed only between phases.
 #include <algorithm>
 #include <cmath>
 
@@ -30,7 +33,7 @@ namespace chs {
 constexpr int kSB = 32;  // sticks per block in the z passes
 
 struct StickArgs {
-  int n_st, n_act, n_pw;
+  int n_st, n_act, n_pw, rs;  // rs: row stride of the compact plane (= 1 mod 8)
   const int32_t* iy;
   const int32_t* act;
   const int32_t* pw0;
@@ -63,7 +66,12 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
       lds[sl * LS + iz] = pb[st.pw0[s] + j];
     }
   }
-  lds_fft_lines<N, 1, T>(lds, ns, [](int l) { return l * LS; }, 1, tw);
+  __syncthreads();
+  int lane;
+  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
+  if (g >= 0)
+    for (int sl = g; sl < ns; sl += ng) group_fft_line<N, 1>(lds + sl * LS, 1, lane, tw, PostId());
+  __syncthreads();
   cplx<T>* out = sbuf + (size_t)b * N * st.n_st + s0;
   for (int t = threadIdx.x; t < N * kSB; t += blockDim.x) {
     const int z = t / kSB, sl = t - z * kSB;
@@ -71,47 +79,73 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
   }
 }
 
+// V_eff(r) applied to the inverse x-FFT output of one row
+template <typename T> struct MulV {
+  const T* v;
+  __device__ __forceinline__ cplx<T> operator()(int k, cplx<T> c) const {
+    const T s = v[k];
+    return mk(c.x * s, c.y * s);
+  }
+};
+
+// occ * |psi(r)|^2 accumulated into the block's own density plane row
+struct AccRho {
+  double* row;
+  double f;
+  __device__ __forceinline__ z64 operator()(int k, z64 c) const {
+    row[k] += f * (c.x * c.x + c.y * c.y);
+    return c;
+  }
+};
+
+// compact plane <-> one full x row, by the N2 lanes of a group
+template <int N, typename T>
+__device__ __forceinline__ void expand_row(const cplx<T>* cmp_row, cplx<T>* rb, const StickArgs& st,
+                                           int lane) {
+  constexpr int N2 = Split<N>::N2;
+  for (int x = lane; x < N; x += N2) rb[x] = zero_c<T>();
+  wave_sync();
+  for (int a = lane; a < st.n_act; a += N2) rb[st.act_ix[a]] = cmp_row[a];
+  wave_sync();
+}
+
 // --- B: plane pass with V_eff ----------------------------------------------
 template <int N, typename T>
 __global__ void __launch_bounds__(256) k_plane(cplx<T>* __restrict__ sbuf, StickArgs st,
-                                               const T* __restrict__ veff, int ch,
+                                               const T* __restrict__ veff, int nrb,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
+  constexpr int LS = N + 1, N2 = Split<N>::N2;
+  const int RS = st.rs;
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int RS = st.n_act + 1;
-  cplx<T>* stage = cmp + N * RS;
-  constexpr int LS = N + 1;
+  cplx<T>* rowbuf = cmp + N * RS;
   const int z = blockIdx.x, b = blockIdx.y;
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.n_st;
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
-  // inverse y-FFT on the active columns
-  lds_fft_lines<N, 1, T>(cmp, st.n_act, [](int a) { return a; }, RS, tw);
-  const T* vz = veff + (size_t)z * N * N;
-  for (int r0 = 0; r0 < N; r0 += ch) {
-    const int nr = min(ch, N - r0);
-    for (int i = threadIdx.x; i < ch * LS; i += blockDim.x) stage[i] = zero_c<T>();
-    __syncthreads();
-    for (int t = threadIdx.x; t < nr * st.n_act; t += blockDim.x) {
-      const int r = t / st.n_act, a = t - r * st.n_act;
-      stage[r * LS + st.act_ix[a]] = cmp[(r0 + r) * RS + a];
+  __syncthreads();
+  int lane;
+  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
+  if (g >= 0)
+    for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
+  __syncthreads();
+  if (g >= 0 && g < nrb) {
+    cplx<T>* rb = rowbuf + g * LS;
+    const T* vz = veff + (size_t)z * N * N;
+    for (int iy = g; iy < N; iy += nrb) {
+      cplx<T>* crow = cmp + iy * RS;
+      expand_row<N, T>(crow, rb, st, lane);
+      group_fft_line<N, 1>(rb, 1, lane, tw, MulV<T>{vz + iy * N});
+      group_fft_line<N, -1>(rb, 1, lane, tw, PostId());
+      for (int a = lane; a < st.n_act; a += N2) crow[a] = rb[st.act_ix[a]];
+      wave_sync();
     }
-    lds_fft_lines<N, 1, T>(stage, nr, [](int l) { return l * LS; }, 1, tw);
-    for (int t = threadIdx.x; t < nr * N; t += blockDim.x) {
-      const int r = t / N, x = t - r * N;
-      const T v = vz[(r0 + r) * N + x];
-      cplx<T> c = stage[r * LS + x];
-      stage[r * LS + x] = mk(c.x * v, c.y * v);
-    }
-    lds_fft_lines<N, -1, T>(stage, nr, [](int l) { return l * LS; }, 1, tw);
-    for (int t = threadIdx.x; t < nr * st.n_act; t += blockDim.x) {
-      const int r = t / st.n_act, a = t - r * st.n_act;
-      cmp[(r0 + r) * RS + a] = stage[r * LS + st.act_ix[a]];
-    }
-    __syncthreads();
   }
-  lds_fft_lines<N, -1, T>(cmp, st.n_act, [](int a) { return a; }, RS, tw);
+  __syncthreads();
+  if (g >= 0)
+    for (int a = g; a < st.n_act; a += ng) group_fft_line<N, -1>(cmp + a, RS, lane, tw, PostId());
+  __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) plane[s] = cmp[st.iy[s] * RS + st.act[s]];
 }
 
@@ -134,7 +168,12 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
     const int z = t / kSB, sl = t - z * kSB;
     if (sl < ns) lds[sl * LS + z] = in[(size_t)z * st.n_st + sl];
   }
-  lds_fft_lines<N, -1, T>(lds, ns, [](int l) { return l * LS; }, 1, tw);
+  __syncthreads();
+  int lane;
+  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
+  if (g >= 0)
+    for (int sl = g; sl < ns; sl += ng) group_fft_line<N, -1>(lds + sl * LS, 1, lane, tw, PostId());
+  __syncthreads();
   const size_t boff = (size_t)b * st.n_pw;
   double so = 0.0, sp = 0.0;
   for (int t = threadIdx.x; t < ns * N; t += blockDim.x) {
@@ -143,10 +182,10 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
     if (j < st.cnt[s]) {
       int iz = st.iz0[s] + j;
       if (iz >= N) iz -= N;
-      const int g = st.pw0[s] + j;
+      const int gi = st.pw0[s] + j;
       const cplx<T> v = lds[sl * LS + iz];
-      const cplx<T> p = psi[boff + g];
-      const T k = (T)kin[g];
+      const cplx<T> p = psi[boff + gi];
+      const T k = (T)kin[gi];
       cplx<T> h = mk(k * p.x + inv_n3 * v.x, k * p.y + inv_n3 * v.y);
       cplx<T> o;
       if (MODE == EPI_HPSI) {
@@ -154,16 +193,15 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
       } else if (MODE == EPI_CHEB1) {
         o = mk((h.x - c * p.x) * inv_e, (h.y - c * p.y) * inv_e);
       } else {
-        const cplx<T> q = aux0[boff + g];
+        const cplx<T> q = aux0[boff + gi];
         o = mk((T)2 * (h.x - c * p.x) * inv_e - q.x, (T)2 * (h.y - c * p.y) * inv_e - q.y);
       }
-      out[boff + g] = o;
+      out[boff + gi] = o;
       so += (double)o.x * o.x + (double)o.y * o.y;
       sp += (double)p.x * p.x + (double)p.y * p.y;
     }
   }
   if (part) {
-    // wave reduction, then across the 4 waves
     for (int off = 32; off > 0; off >>= 1) {
       so += __shfl_down(so, off);
       sp += __shfl_down(sp, off);
@@ -186,38 +224,61 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
   }
 }
 
-// Sum the per-block norm partials (fixed order) and apply the reference's
-// 10x growth guard for step k; the first failing step is kept in flag[0..4].
-__global__ void k_norm_guard(const double* __restrict__ part, int n_part, int k,
-                             double* __restrict__ flag) {
-  __shared__ double r0[256], r1[256];
-  double a = 0, b = 0;
-  for (int i = threadIdx.x; i < n_part; i += blockDim.x) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
-  }
-  r0[threadIdx.x] = a;
-  r1[threadIdx.x] = b;
-  __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      r0[threadIdx.x] += r0[threadIdx.x + s];
-      r1[threadIdx.x] += r1[threadIdx.x + s];
-    }
+// --- D: density planes ------------------------------------------------------
+// Per (z-plane, chunk of bands): for each band with occ > 0, sticks -> LDS
+// plane, inverse y-FFT, then per row: expand and inverse x-FFT whose final
+// store accumulates occ * |psi(r)|^2 into the block's own partial plane
+// part[chunk][z][N^2] (each element owned by one lane: no atomics).
+template <int N>
+__global__ void __launch_bounds__(256) k_density_plane(const z64* __restrict__ sbuf, StickArgs st,
+                                                       const double* __restrict__ occ, int n_b,
+                                                       int chunk, int nrb, double* __restrict__ part,
+                                                       const z64* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int LS = N + 1;
+  const int RS = st.rs;
+  z64* cmp = reinterpret_cast<z64*>(smem_raw);
+  z64* rowbuf = cmp + N * RS;
+  const int z = blockIdx.x, c = blockIdx.y;
+  const int b0 = c * chunk, b1 = min(n_b, b0 + chunk);
+  double* out = part + ((size_t)c * N + z) * N * N;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) out[i] = 0.0;
+  int lane;
+  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
+  for (int b = b0; b < b1; ++b) {
+    const double f = occ[b];
+    if (f == 0.0) continue;
+    const z64* plane = sbuf + ((size_t)b * N + z) * st.n_st;
     __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double nc = sqrt(r0[0]), np = sqrt(r1[0]);
-    const double growth = np > 0.0 ? nc / np : 1.0;
-    if (flag[0] == 0.0 && growth > 10.0) {
-      flag[0] = k;
-      flag[1] = np;
-      flag[2] = nc;
-      flag[3] = growth;
+    for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = mk(0.0, 0.0);
+    __syncthreads();
+    for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
+    __syncthreads();
+    if (g >= 0)
+      for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
+    __syncthreads();
+    if (g >= 0 && g < nrb) {
+      z64* rb = rowbuf + g * LS;
+      for (int iy = g; iy < N; iy += nrb) {
+        expand_row<N, double>(cmp + iy * RS, rb, st, lane);
+        group_fft_line<N, 1>(rb, 1, lane, tw, AccRho{out + iy * N, f});
+      }
     }
   }
 }
 
+__global__ void k_density_reduce(const double* __restrict__ part, int n_chunks, size_t n3,
+                                 double inv_omega, double* __restrict__ rho) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  double s = 0;
+  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
+  rho[i] = s * inv_omega;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
 int build_sticks(chs_context* ctx) {
   Hamiltonian& H = ctx->impl->ham;
   const int n = H.n_grid, n_pw = H.n_pw;
@@ -287,6 +348,8 @@ static StickArgs stick_args(const StickTable& st) {
   a.n_st = st.n_st;
   a.n_act = st.n_act;
   a.n_pw = st.n_pw;
+  a.rs = st.n_act + 1;
+  while (a.rs % 8 != 1) ++a.rs;  // spreads column accesses over LDS banks
   a.iy = st.iy;
   a.act = st.act;
   a.pw0 = st.pw0;
@@ -309,72 +372,31 @@ static int max_lds_per_block(chs_context* ctx) {
   return cached;
 }
 
-template <typename T>
-static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
-                           int mode, const void* aux0_v, double c, double inv_e, double* part,
-                           int* n_part) {
-  Hamiltonian& H = ctx->impl->ham;
-  const StickTable& st = H.st;
-  const int n = H.n_grid;
-  using C = cplx<T>;
-  const size_t sbytes = (size_t)n_b * n * st.n_st * sizeof(C);
-  CHS_TRY(ensure(ctx, ctx->impl->stick_buf, sbytes));
-  C* sbuf = (C*)ctx->impl->stick_buf.p;
-  const void* twv = nullptr;
-  CHS_TRY(twiddles(ctx, n, precision, &twv));
-  const C* tw = (const C*)twv;
-  const T* veff;
-  if (precision == CHS_C128) {
-    veff = (const T*)H.veff;
-  } else {
-    veff = (const T*)H.veff_f.p;
-  }
-  StickArgs sa = stick_args(st);
-  const C* psi = (const C*)psi_v;
-  C* out = (C*)out_v;
-  const C* aux0 = (const C*)aux0_v;
-  const double inv_n3 = 1.0 / ((double)n * n * n);
-  const int gz = ceil_div(st.n_st, kSB);
-  if (n_part) *n_part = gz * n_b;
-  const size_t cmp_bytes = (size_t)n * (st.n_act + 1) * sizeof(C);
-  const size_t row_bytes = (size_t)(n + 1) * sizeof(C);
+// Dynamic LDS of the plane kernels: compact plane + nrb row buffers.
+struct PlaneLaunch {
+  size_t lds = 0;
+  int nrb = 0;
+};
+static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz, int n2,
+                        PlaneLaunch* out) {
+  const size_t cmp_bytes = (size_t)n * sa.rs * csz;
+  const size_t row_bytes = (size_t)(n + 1) * csz;
   const int lim = max_lds_per_block(ctx);
   if (cmp_bytes + row_bytes > (size_t)lim)
     return set_error(ctx, CHS_ERR_FFT, "plane of %d^3 grid does not fit in LDS (%zu > %d bytes)",
                      n, cmp_bytes + row_bytes, lim);
-  int ch = (int)((lim - cmp_bytes) / row_bytes);
-  ch = std::min(ch, 32);
-  ch = std::min(ch, n);
-  const size_t plane_lds = cmp_bytes + (size_t)ch * row_bytes;
-  CHS_DISPATCH_N(n, {
-    hipLaunchKernelGGL((k_stick_inv<NN, T>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
-                       sbuf, tw);
-    CHS_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), plane_lds, ctx->stream, sbuf,
-                       sa, veff, ch, tw);
-    CHS_LAUNCH_CHECK(ctx);
-    if (mode == EPI_HPSI)
-      hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_HPSI>), dim3(gz, n_b), dim3(256), 0, ctx->stream,
-                         sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e, (T)inv_n3, part, tw);
-    else if (mode == EPI_CHEB1)
-      hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEB1>), dim3(gz, n_b), dim3(256), 0,
-                         ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e, (T)inv_n3,
-                         part, tw);
-    else
-      hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEBK>), dim3(gz, n_b), dim3(256), 0,
-                         ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e, (T)inv_n3,
-                         part, tw);
-    CHS_LAUNCH_CHECK(ctx);
-  });
+  const int groups = 4 * (64 / n2);  // 256 threads
+  int nrb = (int)((lim - cmp_bytes) / row_bytes);
+  nrb = std::min(nrb, groups);
+  out->nrb = nrb;
+  out->lds = cmp_bytes + (size_t)nrb * row_bytes;
   return CHS_OK;
 }
 
-// The three kernels of one Chebyshev step (steady-state epilogue) `reps`
-// times with HIP events around each launch on the context stream; ms[k] is
-// the mean duration of kernel k (A, B, C).
 template <typename T>
-static int vloc_timed_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
-                        int reps, double* ms) {
+static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
+                           int mode, const void* aux0_v, double c, double inv_e, double* part,
+                           int* n_part, int reps, double* ms) {
   Hamiltonian& H = ctx->impl->ham;
   const StickTable& st = H.st;
   const int n = H.n_grid;
@@ -388,48 +410,54 @@ static int vloc_timed_t(chs_context* ctx, int precision, const void* psi_v, int 
   StickArgs sa = stick_args(st);
   const C* psi = (const C*)psi_v;
   C* out = (C*)out_v;
+  const C* aux0 = (const C*)aux0_v;
   const double inv_n3 = 1.0 / ((double)n * n * n);
   const int gz = ceil_div(st.n_st, kSB);
-  const size_t cmp_bytes = (size_t)n * (st.n_act + 1) * sizeof(C);
-  const size_t row_bytes = (size_t)(n + 1) * sizeof(C);
-  const int lim = max_lds_per_block(ctx);
-  if (cmp_bytes + row_bytes > (size_t)lim) return set_error(ctx, CHS_ERR_FFT, "plane too large");
-  const int ch = std::min(std::min(32, n), (int)((lim - cmp_bytes) / row_bytes));
-  const size_t plane_lds = cmp_bytes + (size_t)ch * row_bytes;
+  if (n_part) *n_part = gz * n_b;
   hipEvent_t ev[4];
-  for (auto& e : ev) CHS_HIP(ctx, hipEventCreate(&e));
+  if (ms)
+    for (auto& e : ev) CHS_HIP(ctx, hipEventCreate(&e));
   double tot[3] = {0, 0, 0};
-  for (int r = 0; r < reps; ++r) {
+  for (int r = 0; r < std::max(1, reps); ++r) {
     CHS_DISPATCH_N(n, {
-      CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+      PlaneLaunch pl;
+      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, &pl));
+      if (ms) CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
       hipLaunchKernelGGL((k_stick_inv<NN, T>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
                          sbuf, tw);
-      CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-      hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), plane_lds, ctx->stream,
-                         sbuf, sa, veff, ch, tw);
-      CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-      hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEBK>), dim3(gz, n_b), dim3(256), 0,
-                         ctx->stream, sbuf, sa, psi, H.kin, out, out, (T)0.5, (T)0.1, (T)inv_n3,
-                         nullptr, tw);
-      CHS_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+      if (ms) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+      hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf,
+                         sa, veff, pl.nrb, tw);
+      if (ms) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+      if (mode == EPI_HPSI)
+        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_HPSI>), dim3(gz, n_b), dim3(256), 0,
+                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
+                           (T)inv_n3, part, tw);
+      else if (mode == EPI_CHEB1)
+        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEB1>), dim3(gz, n_b), dim3(256), 0,
+                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
+                           (T)inv_n3, part, tw);
+      else
+        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEBK>), dim3(gz, n_b), dim3(256), 0,
+                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
+                           (T)inv_n3, part, tw);
+      if (ms) CHS_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+      CHS_LAUNCH_CHECK(ctx);
     });
-    CHS_LAUNCH_CHECK(ctx);
-    CHS_HIP(ctx, hipEventSynchronize(ev[3]));
-    for (int k = 0; k < 3; ++k) {
-      float t = 0;
-      CHS_HIP(ctx, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-      tot[k] += t;
+    if (ms) {
+      CHS_HIP(ctx, hipEventSynchronize(ev[3]));
+      for (int k = 0; k < 3; ++k) {
+        float t = 0;
+        CHS_HIP(ctx, hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        tot[k] += t;
+      }
     }
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  for (int k = 0; k < 3; ++k) ms[k] = tot[k] / reps;
+  if (ms) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int k = 0; k < 3; ++k) ms[k] = tot[k] / std::max(1, reps);
+  }
   return CHS_OK;
-}
-
-int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
-               double* ms) {
-  if (precision == CHS_C128) return vloc_timed_t<double>(ctx, precision, psi, n_b, out, reps, ms);
-  return vloc_timed_t<float>(ctx, precision, psi, n_b, out, reps, ms);
 }
 
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
@@ -437,77 +465,21 @@ int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, voi
                   double* d_norm_part, int* n_part) {
   if (precision == CHS_C128)
     return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e,
-                                   d_norm_part, n_part);
+                                   d_norm_part, n_part, 1, nullptr);
   return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_norm_part,
-                                n_part);
+                                n_part, 1, nullptr);
 }
 
-// --- D: density planes ------------------------------------------------------
-// Per (z-plane, chunk of bands): for each band with occ > 0, sticks -> LDS
-// plane, inverse y-FFT, then row chunks: expand, inverse x-FFT and
-// accumulate occ * |psi(r)|^2 in registers (thread t owns plane elements
-// t + 256 i). The chunk's partial plane goes to part[chunk][z][N^2].
-template <int N>
-__global__ void __launch_bounds__(256) k_density_plane(const z64* __restrict__ sbuf, StickArgs st,
-                                                       const double* __restrict__ occ, int n_b,
-                                                       int chunk, int ch, double* __restrict__ part,
-                                                       const z64* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
-  z64* cmp = reinterpret_cast<z64*>(smem_raw);
-  const int RS = st.n_act + 1;
-  z64* stage = cmp + N * RS;
-  constexpr int LS = N + 1;
-  constexpr int PER = (N * N + 255) / 256;
-  const int z = blockIdx.x, c = blockIdx.y;
-  const int b0 = c * chunk, b1 = min(n_b, b0 + chunk);
-  double acc[PER];
-#pragma unroll
-  for (int i = 0; i < PER; ++i) acc[i] = 0.0;
-  for (int b = b0; b < b1; ++b) {
-    const double f = occ[b];
-    if (f == 0.0) continue;
-    const z64* plane = sbuf + ((size_t)b * N + z) * st.n_st;
-    __syncthreads();
-    for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = mk(0.0, 0.0);
-    __syncthreads();
-    for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
-    lds_fft_lines<N, 1, double>(cmp, st.n_act, [](int a) { return a; }, RS, tw);
-    for (int r0 = 0; r0 < N; r0 += ch) {
-      const int nr = min(ch, N - r0);
-      for (int i = threadIdx.x; i < ch * LS; i += blockDim.x) stage[i] = mk(0.0, 0.0);
-      __syncthreads();
-      for (int t = threadIdx.x; t < nr * st.n_act; t += blockDim.x) {
-        const int r = t / st.n_act, a = t - r * st.n_act;
-        stage[r * LS + st.act_ix[a]] = cmp[(r0 + r) * RS + a];
-      }
-      lds_fft_lines<N, 1, double>(stage, nr, [](int l) { return l * LS; }, 1, tw);
-#pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + 256 * i;
-        const int row = e / N;
-        if (e < N * N && row >= r0 && row < r0 + nr) {
-          const z64 v = stage[(row - r0) * LS + (e - row * N)];
-          acc[i] += f * (v.x * v.x + v.y * v.y);
-        }
-      }
-      __syncthreads();
-    }
-  }
-  double* out = part + ((size_t)c * N + z) * N * N;
-#pragma unroll
-  for (int i = 0; i < PER; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    if (e < N * N) out[e] = acc[i];
-  }
-}
-
-__global__ void k_density_reduce(const double* __restrict__ part, int n_chunks, size_t n3,
-                                 double inv_omega, double* __restrict__ rho) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n3) return;
-  double s = 0;
-  for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
-  rho[i] = s * inv_omega;
+// The three kernels of one steady-state Chebyshev step `reps` times with HIP
+// events around each launch on the context stream; ms[k] = mean duration of
+// kernel k (A, B, C). out doubles as Y_{k-2}.
+int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
+               double* ms) {
+  if (precision == CHS_C128)
+    return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5,
+                                   0.1, nullptr, nullptr, reps, ms);
+  return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5, 0.1,
+                                nullptr, nullptr, reps, ms);
 }
 
 int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
@@ -518,8 +490,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   const StickTable& st = H.st;
   const int n = H.n_grid;
   const size_t n3 = (size_t)n * n * n;
-  // only bands up to the last occupied one
-  int nb = 0;
+  int nb = 0;  // bands up to the last occupied one
   for (int b = 0; b < n_b; ++b)
     if (occ_host[b] != 0.0) nb = b + 1;
   const int chunk = 16;
@@ -540,20 +511,15 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));
   const z64* tw = (const z64*)twv;
   StickArgs sa = stick_args(st);
-  const size_t cmp_bytes = (size_t)n * (st.n_act + 1) * sizeof(z64);
-  const size_t row_bytes = (size_t)(n + 1) * sizeof(z64);
-  const int lim = max_lds_per_block(ctx);
-  if (cmp_bytes + row_bytes > (size_t)lim)
-    return set_error(ctx, CHS_ERR_FFT, "density plane of %d^3 grid does not fit in LDS", n);
-  int ch = std::min(32, std::min(n, (int)((lim - cmp_bytes) / row_bytes)));
-  const size_t lds = cmp_bytes + (size_t)ch * row_bytes;
   const int gz = ceil_div(st.n_st, kSB);
   CHS_DISPATCH_N(n, {
+    PlaneLaunch pl;
+    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, &pl));
     hipLaunchKernelGGL((k_stick_inv<NN, double>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi, sa,
                        sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL((k_density_plane<NN>), dim3(NN, n_chunks), dim3(256), lds, ctx->stream,
-                       sbuf, sa, docc, nb, chunk, ch, part, tw);
+    hipLaunchKernelGGL((k_density_plane<NN>), dim3(NN, n_chunks), dim3(256), pl.lds, ctx->stream,
+                       sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
     CHS_LAUNCH_CHECK(ctx);
   });
   hipLaunchKernelGGL(k_density_reduce, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, part,

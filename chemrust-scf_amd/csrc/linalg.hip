@@ -227,11 +227,17 @@ struct JacobiPlan {
   int rounds = 0, pairs = 0;
   DevBuf sched;  // int2 [rounds][pairs]
   DevBuf rot;    // double4 [pairs]
+  hipGraphExec_t exec = nullptr;  // one captured sweep for (gA, gV)
+  const z64* gA = nullptr;
+  const z64* gV = nullptr;
 };
 static JacobiPlan g_plan;  // guarded per context device by rebuild on size change
 
 static int jacobi_plan(chs_context* ctx, int n, JacobiPlan& P) {
   if (P.n == n && P.sched.p) return CHS_OK;
+  if (P.exec) (void)hipGraphExecDestroy(P.exec);  // the captured sweep bakes in n and buffers
+  P.exec = nullptr;
+  P.gA = P.gV = nullptr;
   const int m = (n % 2) ? n + 1 : n;  // add a dummy index for odd n
   P.n = n;
   P.rounds = m - 1;
@@ -277,14 +283,27 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
     CHS_HIP(ctx, hipMemcpyAsync(h, dn, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (h[0] <= 1e-30 * h[1] || h[0] == 0.0) break;
-    for (int r = 0; r < P.rounds; ++r) {
-      const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
-      hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, ctx->stream, A, V, n, pr,
-                         (double4*)P.rot.p);
-      hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, ctx->stream, A, n, pr,
-                         (const double4*)P.rot.p);
+    // one sweep = 2 (n-1) small launches: captured once into a graph and
+    // replayed (no memset nodes inside)
+    if (!P.exec || P.gA != A || P.gV != V) {
+      if (P.exec) (void)hipGraphExecDestroy(P.exec);
+      P.exec = nullptr;
+      hipGraph_t graph;
+      CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      for (int r = 0; r < P.rounds; ++r) {
+        const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
+        hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, ctx->stream, A, V, n, pr,
+                           (double4*)P.rot.p);
+        hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, ctx->stream, A, n, pr,
+                           (const double4*)P.rot.p);
+      }
+      CHS_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
+      CHS_HIP(ctx, hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      P.gA = A;
+      P.gV = V;
     }
-    CHS_LAUNCH_CHECK(ctx);
+    CHS_HIP(ctx, hipGraphLaunch(P.exec, ctx->stream));
   }
   if (sweeps_out) *sweeps_out = sweeps;
   if (n > 1024) return set_error(ctx, CHS_ERR_SOLVER, "eigensolver limited to n <= 1024");

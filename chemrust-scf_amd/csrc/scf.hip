@@ -281,31 +281,35 @@ int GridOps::init(chs_context* c, int n_, double L, double omega_) {
   return prep_reduce(ctx);
 }
 
-// V_ion on the grid from atom positions.
-int GridOps::vion(const double* pos, int n_atoms, double zval, double rc, double* d_vion) {
-  std::vector<double> ph((size_t)n_atoms * 3 * n * 2);
+// Per-atom 1D structure-phase tables (host cos/sin, one upload per geometry).
+int GridOps::set_atoms(const double* pos, int n_at) {
+  n_atoms = n_at;
+  std::vector<double> h((size_t)n_atoms * 3 * n * 2);
   for (int a = 0; a < n_atoms; ++a)
     for (int c = 0; c < 3; ++c)
       for (int i = 0; i < n; ++i) {
         const int f = i <= n / 2 ? i : i - n;
         const double t = -b * f * pos[3 * a + c];
-        ph[(((size_t)a * 3 + c) * n + i) * 2] = std::cos(t);
-        ph[(((size_t)a * 3 + c) * n + i) * 2 + 1] = std::sin(t);
+        h[(((size_t)a * 3 + c) * n + i) * 2] = std::cos(t);
+        h[(((size_t)a * 3 + c) * n + i) * 2 + 1] = std::sin(t);
       }
-  DevBuf dph;
-  CHS_TRY(ensure(ctx, dph, ph.size() * sizeof(double)));
-  CHS_HIP(ctx, hipMemcpyAsync(dph.p, ph.data(), ph.size() * sizeof(double), hipMemcpyHostToDevice,
+  CHS_TRY(ensure(ctx, ph, h.size() * sizeof(double)));
+  CHS_HIP(ctx, hipMemcpyAsync(ph.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
-  ctx->h2d += ph.size() * sizeof(double);
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->h2d += h.size() * sizeof(double);
+  return CHS_OK;
+}
+
+// V_ion on the grid from the atom phase tables.
+int GridOps::vion(double zval, double rc, double* d_vion) {
   hipLaunchKernelGGL(k_vion_g, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, (z64*)cgrid.p,
-                     n, n_atoms, (const z64*)dph.p, b, zval, rc, omega);
+                     n, n_atoms, (const z64*)ph.p, b, zval, rc, omega);
   CHS_LAUNCH_CHECK(ctx);
   CHS_TRY(fft3d(ctx, CHS_C128, cgrid.p, n, 1, +1));
   hipLaunchKernelGGL(k_real_part, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream,
                      (const z64*)cgrid.p, d_vion, n3);
   CHS_LAUNCH_CHECK(ctx);
-  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  release(dph);
   return CHS_OK;
 }
 

@@ -12,12 +12,18 @@ struct GridOps {
   size_t n3 = 0;
   double b = 0, omega = 0;
   DevBuf cgrid;  // complex scratch grid
+  DevBuf ph;     // per-atom 1D phase tables [atom][axis][i] = e^{-i b f(i) x_a}
+  int n_atoms = 0;
   int init(chs_context* c, int n, double L, double omega);
-  int vion(const double* pos, int n_atoms, double zval, double rc, double* d_vion);
+  int set_atoms(const double* pos, int n_atoms);
+  int vion(double zval, double rc, double* d_vion);
   int vhxc(const double* rho, const double* vion, double* vhxc_out, double* veff,
            const double* vin, double* eh, double* exc, double* dc);
   int kerker(double* f, double q);
-  ~GridOps() { release(cgrid); }
+  ~GridOps() {
+    release(cgrid);
+    release(ph);
+  }
 };
 
 struct PulayDev {

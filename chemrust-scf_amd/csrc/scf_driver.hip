@@ -9,6 +9,7 @@
 #include "impl.hpp"
 #include "linalg.hpp"
 #include "ops.hpp"
+#include "projectors.hpp"
 #include "scf_dev.hpp"
 
 namespace chs {
@@ -55,9 +56,10 @@ static void initial_psi_host(const std::vector<double>& kin, int n_b, uint64_t s
 
 struct ScfState {
   DevBuf miller, idx, kin, vion, veff, vhxc, rho_in, rho_out, rho_next, psi, psi32;
+  DevBuf pos, beta, d0, dscr, q, occ;  // projector part
   ~ScfState() {
     for (DevBuf* b : {&miller, &idx, &kin, &vion, &veff, &vhxc, &rho_in, &rho_out, &rho_next, &psi,
-                      &psi32})
+                      &psi32, &pos, &beta, &d0, &dscr, &q, &occ})
       release(*b);
   }
 };
@@ -68,8 +70,9 @@ static double now() {
 
 int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
                  double* energies, double* eps_out, double* rho_host) {
-  if (p->n_proj_per_atom != 0)
-    return set_error(ctx, CHS_ERR_NOT_IMPLEMENTED, "projector SCF is not available yet");
+  const int np = p->n_proj_per_atom;
+  if (np != 0 && np != 1 && np != 4)
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "projectors per atom must be 0, 1 or 4");
   ScfState S;
   const int n = p->n_grid, n_b = p->n_bands;
   const double L = p->n_rep * p->a0, omega = L * L * L;
@@ -90,7 +93,8 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     CHS_TRY(ensure(ctx, *b, n3 * sizeof(double)));
   GridOps G;
   CHS_TRY(G.init(ctx, n, L, omega));
-  CHS_TRY(G.vion(pos, n_atoms, p->zval, p->rc_loc, (double*)S.vion.p));
+  CHS_TRY(G.set_atoms(pos, n_atoms));
+  CHS_TRY(G.vion(p->zval, p->rc_loc, (double*)S.vion.p));
   // start wavefunctions (host generator, one upload)
   std::vector<double> kin(n_pw), h_psi;
   CHS_HIP(ctx, hipMemcpy(kin.data(), S.kin.p, n_pw * sizeof(double), hipMemcpyDeviceToHost));
@@ -104,6 +108,31 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   if (mixed) CHS_TRY(ensure(ctx, S.psi32, ne * sizeof(c32)));
   CHS_TRY(chs_bind_hamiltonian(ctx, n, n_pw, (const int32_t*)S.miller.p, (const double*)S.kin.p,
                                (const double*)S.veff.p));
+  // USPP projectors: beta on the device, D0 ~ U[d_min, d_max] from the LCG
+  // (seed_d, atom-major), Q = q0 on every projector
+  const int P = np * n_atoms;
+  if (P > 0) {
+    CHS_TRY(ensure(ctx, S.pos, 3 * (size_t)n_atoms * sizeof(double)));
+    CHS_HIP(ctx, hipMemcpy(S.pos.p, pos, 3 * (size_t)n_atoms * sizeof(double),
+                           hipMemcpyHostToDevice));
+    CHS_TRY(ensure(ctx, S.beta, (size_t)P * n_pw * sizeof(z64)));
+    CHS_TRY(build_beta(ctx, (const int32_t*)S.miller.p, n_pw, L, p->kfrac,
+                       (const double*)S.pos.p, n_atoms, np, p->beta_rc, (z64*)S.beta.p));
+    std::vector<double> d0(P), q(P, p->q0);
+    uint64_t s = p->seed_d;
+    for (int i = 0; i < P; ++i) {
+      s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
+      d0[i] = p->d_min + (p->d_max - p->d_min) * ((double)s / 281474976710656.0);
+    }
+    for (DevBuf* b : {&S.d0, &S.dscr, &S.q}) CHS_TRY(ensure(ctx, *b, P * sizeof(double)));
+    CHS_HIP(ctx, hipMemcpy(S.d0.p, d0.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    CHS_HIP(ctx, hipMemcpy(S.dscr.p, d0.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    CHS_HIP(ctx, hipMemcpy(S.q.p, q.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    ctx->h2d += 3 * (size_t)n_atoms * sizeof(double) + 3 * (size_t)P * sizeof(double);
+    CHS_TRY(chs_bind_projectors(ctx, P, (const double*)S.beta.p, (const double*)S.dscr.p,
+                                (const double*)S.q.p));
+  }
+  CHS_TRY(ensure(ctx, S.occ, (size_t)n_b * sizeof(double)));
   PulayDev mixer;
   CHS_TRY(mixer.init(ctx, &G, std::max(1, p->pulay_depth), p->mix_alpha, p->kerker_q));
   hipLaunchKernelGGL(k_fill_d, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream,
@@ -124,6 +153,10 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     CHS_TRY(G.vhxc((const double*)S.rho_in.p, (const double*)S.vion.p, (double*)S.vhxc.p,
                    (double*)S.veff.p, nullptr, &eh_in, &exc_in, nullptr));
     CHS_TRY(chs_set_veff(ctx, (const double*)S.veff.p));
+    if (P > 0 && p->q0 != 0.0)
+      CHS_TRY(screen_dion(ctx, (const double*)S.veff.p, n, L, n_atoms, np, p->q0, p->q_rc,
+                          (const z64*)G.ph.p, (const double*)S.d0.p, (z64*)G.cgrid.p,
+                          (double*)S.dscr.p));
     CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double tb = now();
     t_veff += tb - ta;
@@ -162,6 +195,13 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     double ts = 0;
     mu = fermi_occupations(eps.data(), n_b, n_e, p->smear_kt, occ.data(), &ts);
     CHS_TRY(density_impl(ctx, (const z64*)S.psi.p, n_b, occ.data(), omega, (double*)S.rho_out.p));
+    if (P > 0 && p->q0 != 0.0) {
+      CHS_HIP(ctx, hipMemcpyAsync(S.occ.p, occ.data(), n_b * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream));
+      CHS_TRY(augment_density(ctx, (const z64*)S.psi.p, n_b, (const double*)S.occ.p, n_atoms, np,
+                              p->q0, p->q_rc, (const z64*)G.ph.p, n, L, (z64*)G.cgrid.p,
+                              (double*)S.rho_out.p));
+    }
     double eh, exc, dc;
     CHS_TRY(G.vhxc((const double*)S.rho_out.p, (const double*)S.vion.p, nullptr, nullptr,
                    (const double*)S.vhxc.p, &eh, &exc, &dc));
@@ -198,7 +238,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   res->n_iter = std::min(it, p->max_iter);
   res->converged = converged;
   res->n_pw = n_pw;
-  res->n_proj = 0;
+  res->n_proj = P;
   res->energy = etot;
   res->mu = mu;
   res->seconds = t1 - t0;

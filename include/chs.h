@@ -171,6 +171,19 @@ int chs_zhegvd(chs_context* ctx, int n, double* d_a, double* d_b, double* w);
  * 27-48): S = Y^H S Y = L L^H, Y <- Y L^{-H} in place (FP64). */
 int chs_cholqr(chs_context* ctx, double* d_y, int n_b);
 
+/* band_cg_minimize (src/eigensolver/band_cg.rs:157-437) for every band of
+ * the c128 block d_psi (n_b x n_pw, band-major, in place) at once: band i
+ * is minimised against the converged set {(psi_j, S psi_j), j < i} of the
+ * block as handed in (the Rayleigh-Ritz vectors), with the reference's
+ * closed-form line search (line_search.rs:57-198), CG helpers
+ * (cg_helpers.rs:49-216) and stopping rules. precond: 0 = TPA
+ * (uspp_preconditioner.rs:148-180), 1 = TPA + USPP Woodbury term
+ * (:56-131), 2 = a per-(band, plane wave) scale table d_mlp_scale (f32,
+ * n_b x n_pw) in place of the TPA diagonal. Outputs per band (host):
+ * eps, n_steps, converged. */
+int chs_band_cg(chs_context* ctx, double* d_psi, int n_b, int max_steps, double tol, int precond,
+                const float* d_mlp_scale, double* eps, int* n_steps, int* converged);
+
 /* ---- primitives (BlasHandle, src/device/blas.rs:24-241) --------------
  * Column-major, op: 'N', 'T' or 'C'. */
 int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const double alpha[2],

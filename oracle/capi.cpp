@@ -337,6 +337,50 @@ int orc_uspp_precond_apply(const double* beta, int n_pw, int n_proj, const doubl
   return 0;
 }
 
+// band_cg_minimize for every band of a block, band i against the converged
+// set {(psi_j, S psi_j), j < i} of the block as handed in, with the model's
+// H and S. precond 0: TPA only (beta = 0, Q = I); 1: the full USPP
+// preconditioner with the model's projectors and Q.
+int orc_band_cg_model(void* mm, double* psi_io, int n_b, int max_steps, double tol, int precond,
+                      double* eps, int* n_steps, int* converged) {
+  auto& m = static_cast<OrcModel*>(mm)->m;
+  const int n = m.pw.n_pw;
+  const Projectors* pj = m.pj.n_proj ? &m.pj : nullptr;
+  std::vector<cd> in(C(psi_io), C(psi_io) + (size_t)n_b * n), sin((size_t)n_b * n), tmp(n);
+  for (int b = 0; b < n_b; ++b)
+    apply_h(m.pw, m.sys.n_grid, m.veff.data(), pj, in.data() + (size_t)b * n, 1, tmp.data(),
+            sin.data() + (size_t)b * n);
+  UsppPrecond pc;
+  int rc;
+  if (precond == 1 && pj && m.sys.q0 != 0.0) {
+    const int P = pj->n_proj;
+    std::vector<cd> beta_rm((size_t)n * P), q((size_t)P * P, cd(0));
+    for (int p = 0; p < P; ++p) {
+      for (int g = 0; g < n; ++g) beta_rm[(size_t)g * P + p] = pj->beta[(size_t)p * n + g];
+      q[(size_t)p * P + p] = pj->q[p];
+    }
+    rc = pc.init(beta_rm.data(), n, P, q.data(), m.pw.kin.data());
+  } else {
+    std::vector<cd> beta0(n, cd(0)), q1(1, cd(1));
+    rc = pc.init(beta0.data(), n, 1, q1.data(), m.pw.kin.data());
+  }
+  if (rc) return rc;
+  HsFn hs = [&](const cd* x, cd* hx, cd* sx) {
+    apply_h(m.pw, m.sys.n_grid, m.veff.data(), pj, x, 1, hx, sx);
+  };
+  PrecondFn pre = [&](const cd* r, cd* o) { pc.apply(r, o); };
+  for (int b = 0; b < n_b; ++b) {
+    CgResult res = band_cg_minimize(in.data() + (size_t)b * n, n, pre, b, in.data(), sin.data(),
+                                    max_steps, tol, hs);
+    if (res.status) return res.status;
+    std::copy(res.psi.begin(), res.psi.end(), C(psi_io) + (size_t)b * n);
+    eps[b] = res.eigenvalue;
+    n_steps[b] = res.n_steps;
+    converged[b] = res.converged;
+  }
+  return 0;
+}
+
 int orc_band_cg_dense(const double* psi_init, int n, const double* hmat, const double* smat,
                       const double* beta, int n_proj, const double* q, const double* kin,
                       int n_conv, const double* conv_psi, const double* conv_spsi, int max_steps,

@@ -88,6 +88,10 @@ void orc_tpa_diag(const double* kin, int n, double* t);
 int orc_uspp_precond_apply(const double* beta, int n_pw, int n_proj, const double* q,
                            const double* kin, const double* res, int n_vec, double* out,
                            double* r_matrix, double* c_matrix);
+/* band CG of every band of a block against the lower bands of the block as
+ * handed in, with the model's H/S; precond 0 = TPA, 1 = TPA + USPP term */
+int orc_band_cg_model(void* m, double* psi_io, int n_b, int max_steps, double tol, int precond,
+                      double* eps, int* n_steps, int* converged);
 /* band CG with dense H and S (n x n column-major complex) and the
  * TPA/USPP preconditioner built from beta (n x n_proj row-major), q, kin. */
 int orc_band_cg_dense(const double* psi_init, int n, const double* hmat, const double* smat,

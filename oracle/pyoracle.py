@@ -89,6 +89,8 @@ def lib():
             "orc_tpa_diag": (None, [f64p, C.c_int, f64p]),
             "orc_uspp_precond_apply": (C.c_int, [c128p, C.c_int, C.c_int, c128p, f64p, c128p, C.c_int,
                                                  c128p, vp, vp]),
+            "orc_band_cg_model": (C.c_int, [vp, c128p, C.c_int, C.c_int, C.c_double, C.c_int, f64p,
+                                            i32p, i32p]),
             "orc_band_cg_dense": (C.c_int, [c128p, C.c_int, c128p, c128p, c128p, C.c_int, c128p, f64p,
                                             C.c_int, vp, vp, C.c_int, C.c_double, c128p, f64p,
                                             C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -229,6 +231,17 @@ class Model:
         eh, exc = C.c_double(), C.c_double()
         lib().orc_vhxc(self.h, np.ascontiguousarray(rho, np.float64), v, C.byref(eh), C.byref(exc))
         return v, eh.value, exc.value
+
+    def band_cg(self, psi, max_steps, tol, precond=0):
+        psi = np.ascontiguousarray(psi, np.complex128).copy()
+        n_b = psi.shape[0]
+        eps = np.zeros(n_b)
+        steps = np.zeros(n_b, np.int32)
+        conv = np.zeros(n_b, np.int32)
+        rc = lib().orc_band_cg_model(self.h, psi, n_b, max_steps, tol, precond, eps, steps, conv)
+        if rc:
+            raise RuntimeError(f"oracle band_cg failed: {rc}")
+        return psi, eps, steps, conv
 
     def screen_d(self):
         d = np.zeros(self.n_proj)
