@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--precision", default="c128", choices=["c128", "c64"],
+                    help="filter precision: c128 throughout, or complex64 filter with FP64 "
+                         "Rayleigh-Ritz and the last SCF iterations in c128")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only time the H|psi> kernels (for counter runs) and print the roofline")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -149,7 +152,8 @@ def main():
         print(json.dumps(roofline(chs, ctx, cfg)))
         return
     pos = chs.synth_positions(cfg["n_rep"])
-    params = chs.scf_params(**cfg, max_iter=100)
+    prec = chs.C128 if args.precision == "c128" else chs.C64
+    params = chs.scf_params(**cfg, max_iter=100, precision=prec)
     for _ in range(args.warmup):
         r = chs.run_scf(ctx, params, pos)
         print(f"warmup: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
@@ -175,7 +179,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round((t1 - t0) / args.steps * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128", "data": "synthetic",
+        "dtype": args.precision, "data": "synthetic",
         "config": {"workload": WORKLOAD[args.config], "atoms": 8 * cfg["n_rep"] ** 3,
                    "n_pw": last["n_pw"], "grid": cfg["n_grid"], "bands": cfg["n_bands"],
                    "scf_iterations": last["n_iter"], "energy_Ha": last["energy"],

@@ -1,4 +1,5 @@
 // Context, memory, transfer accounting and status reporting.
+#include <cstdlib>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -85,6 +86,7 @@ int chs_context_create(int device, chs_context** out) {
   auto* ctx = new chs_context;
   ctx->device = device;
   ctx->impl = new chs_context::Impl;
+  if (const char* ng = std::getenv("CHS_NO_GRAPHS")) ctx->impl->use_graphs = !(ng[0] && ng[0] != '0');
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev0) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev1) != hipSuccess) {
@@ -104,7 +106,7 @@ int chs_context_destroy(chs_context* ctx) {
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
                     &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
-                    &im->proj_ws,
+                    &im->proj_ws, &im->chol_ws, &im->eig_ws,
                     &im->ham.st_mem, &im->ham.veff_f, &im->ham.beta_f, &im->ham.sinv_m,
                     &im->ham.sinv_m_f};
   for (DevBuf* b : bufs) release(*b);

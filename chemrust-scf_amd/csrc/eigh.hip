@@ -1,0 +1,503 @@
+// Hermitian eigensolver for the subspace problems of Rayleigh-Ritz
+// (n <= 1024, FP64): Householder reduction to real tridiagonal form,
+// multisection for the eigenvalues, inverse iteration for the tridiagonal
+// eigenvectors, and the blocked (compact WY) back-transformation.
+//
+// Reference: the generalized problem is solved by cuSOLVER ZHEGVD in
+// SolverHandle::zhegvd (src/device/solver.rs:47-95) and by a dense CPU
+// eigensolver in diagonalize_and_rotate (src/eigensolver/
+// subspace_diag.rs:229-391); both return eigenvalues ascending with
+// orthonormal eigenvectors, as this solver does.
+//
+// Reduction, step k (trailing size m = n - k - 1, x = A[k+1:, k]):
+//   H = I - tau v v^H with H^H x = beta e_1, beta real, v_0 = 1;
+//   p = tau A22 v, s = v^H A22 v, w = p - (|tau|^2 s / 2) v,
+//   A22 <- H^H A22 H = A22 - v w^H - w v^H.
+// Then A = Q T Q^H with Q = H_0 H_1 ... H_{n-2}, and A's eigenvectors are
+// Q Z for T's eigenvectors Z. Each step is two kernels (partial products of
+// p and s; rank-2 update), replayed from a captured graph.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "blas1.hpp"
+#include "impl.hpp"
+#include "linalg.hpp"
+
+namespace chs {
+
+constexpr int kHC = 64;   // row/column chunk of the symmetric product
+constexpr int kHT = 32;   // tile of the rank-2 update
+constexpr int kWY = 32;   // reflectors per compact-WY block
+
+// Block sum visible to every thread (sh: at least 9 doubles).
+__device__ __forceinline__ double block_sum_all(double v, double* sh) {
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) sh[8] = s;
+  __syncthreads();
+  const double r = sh[8];
+  __syncthreads();
+  return r;
+}
+
+// Reflector of x = A[k+1:, k] (every block recomputes ||x_rest||^2: x is at
+// most 1023 elements and this saves a launch). Returns tau, beta and the
+// scale 1 / (alpha - beta) that maps x_j (j >= 1) to v_j.
+__device__ void hh_reflector(const z64* __restrict__ x, int m, double* sh, z64* tau, double* beta,
+                             z64* scale) {
+  double s = 0.0;
+  for (int t = threadIdx.x + 1; t < m; t += blockDim.x) s += norm2(x[t]);
+  const double xn2 = block_sum_all(s, sh);
+  const z64 alpha = x[0];
+  if (xn2 == 0.0 && alpha.y == 0.0) {
+    *tau = mk(0.0, 0.0);
+    *beta = alpha.x;
+    *scale = mk(0.0, 0.0);
+    return;
+  }
+  const double b = -copysign(sqrt(norm2(alpha) + xn2), alpha.x);
+  *beta = b;
+  *tau = mk((b - alpha.x) / b, -alpha.y / b);
+  const z64 den = mk(alpha.x - b, alpha.y);
+  const double dn = norm2(den);
+  *scale = mk(den.x / dn, -den.y / dn);
+}
+
+// Partial p and s of step k. Block (bi, bj) multiplies rows chunk bi of A22
+// with v over columns chunk bj: ppart[bj][i] = sum_j A22[i][j] v_j,
+// spart[bi][bj] = sum_i conj(v_i) ppart[bj][i]. Block (0, 0) also stores
+// v in column k of Vst, tau_k, e_k = beta and d_k = Re A[k][k].
+__global__ void __launch_bounds__(256) k_hh_symv(const z64* __restrict__ A, int n, int k,
+                                                 z64* __restrict__ Vst, z64* __restrict__ ppart,
+                                                 z64* __restrict__ spart, z64* __restrict__ tau,
+                                                 double* __restrict__ dd, double* __restrict__ ee) {
+  __shared__ double sh[9];
+  __shared__ z64 vj[kHC];
+  __shared__ z64 acc_sh[4][kHC];
+  const int m = n - k - 1;
+  const int nch = (m + kHC - 1) / kHC;
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  const z64* x = A + (size_t)(k + 1) + (size_t)k * n;
+  z64 t_k, scale;
+  double beta;
+  hh_reflector(x, m, sh, &t_k, &beta, &scale);
+  if (bi == 0 && bj == 0) {
+    z64* vcol = Vst + (size_t)k * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const int j = r - k - 1;
+      vcol[r] = j < 0 ? mk(0.0, 0.0) : (j == 0 ? mk(1.0, 0.0) : x[j] * scale);
+    }
+    if (threadIdx.x == 0) {
+      tau[k] = t_k;
+      ee[k] = beta;
+      dd[k] = A[(size_t)k + (size_t)k * n].x;
+    }
+  }
+  for (int t = threadIdx.x; t < kHC; t += blockDim.x) {
+    const int j = bj * kHC + t;
+    vj[t] = j < m ? (j == 0 ? mk(1.0, 0.0) : x[j] * scale) : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = bi * kHC + lane;
+  z64 acc = mk(0.0, 0.0);
+  if (i < m) {
+    const z64* arow = A + (size_t)(k + 1 + i);
+#pragma unroll 4
+    for (int jj = 0; jj < kHC / 4; ++jj) {
+      const int jl = w * (kHC / 4) + jj, j = bj * kHC + jl;
+      if (j < m) acc = acc + arow[(size_t)(k + 1 + j) * n] * vj[jl];
+    }
+  }
+  acc_sh[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) {
+    z64 p = acc_sh[0][lane] + acc_sh[1][lane] + acc_sh[2][lane] + acc_sh[3][lane];
+    z64 sv = mk(0.0, 0.0);
+    if (i < m) {
+      ppart[(size_t)bj * n + i] = p;
+      const z64 vi = i == 0 ? mk(1.0, 0.0) : x[i] * scale;
+      sv = cmulc(vi, p);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sv.x += __shfl_down(sv.x, off);
+      sv.y += __shfl_down(sv.y, off);
+    }
+    if (lane == 0) spart[(size_t)bi * nch + bj] = sv;
+  }
+}
+
+// Rank-2 update of step k on 32 x 32 tiles of A22.
+__global__ void __launch_bounds__(256) k_hh_update(z64* __restrict__ A, int n, int k,
+                                                   const z64* __restrict__ Vst,
+                                                   const z64* __restrict__ ppart,
+                                                   const z64* __restrict__ spart,
+                                                   const z64* __restrict__ tau) {
+  __shared__ double sh[9];
+  __shared__ z64 vr[kHT], wr[kHT], vc[kHT], wc[kHT];
+  const int m = n - k - 1;
+  const int nch = (m + kHC - 1) / kHC;
+  const z64 t_k = tau[k];
+  if (t_k.x == 0.0 && t_k.y == 0.0) return;  // H = I
+  double s = 0.0;
+  for (int t = threadIdx.x; t < nch * nch; t += blockDim.x) s += spart[t].x;
+  const double sv = block_sum_all(s, sh);  // v^H A22 v (real)
+  const double c = 0.5 * norm2(t_k) * sv;
+  const int r0 = blockIdx.x * kHT, c0 = blockIdx.y * kHT;
+  const z64* v = Vst + (size_t)k * n + (k + 1);
+  if (threadIdx.x < 2 * kHT) {
+    const int side = threadIdx.x / kHT, l = threadIdx.x % kHT;
+    const int i = (side ? c0 : r0) + l;
+    z64 vi = mk(0.0, 0.0), wi = mk(0.0, 0.0);
+    if (i < m) {
+      z64 p = mk(0.0, 0.0);
+      for (int b = 0; b < nch; ++b) p = p + ppart[(size_t)b * n + i];
+      p = t_k * p;
+      vi = v[i];
+      wi = mk(p.x - c * vi.x, p.y - c * vi.y);
+    }
+    if (side) {
+      vc[l] = vi;
+      wc[l] = wi;
+    } else {
+      vr[l] = vi;
+      wr[l] = wi;
+    }
+  }
+  __syncthreads();
+  const int r = threadIdx.x % kHT, cq = threadIdx.x / kHT;
+  const int i = r0 + r;
+  if (i >= m) return;
+#pragma unroll
+  for (int q = 0; q < kHT / 8; ++q) {
+    const int cl = cq * (kHT / 8) + q, j = c0 + cl;
+    if (j < m) {
+      z64* a = A + (size_t)(k + 1 + i) + (size_t)(k + 1 + j) * n;
+      const z64 u = vr[r] * conj(wc[cl]) + wr[r] * conj(vc[cl]);
+      *a = *a - u;
+    }
+  }
+}
+
+__global__ void k_hh_last(const z64* __restrict__ A, int n, double* __restrict__ dd) {
+  if (threadIdx.x == 0) dd[n - 1] = A[(size_t)(n - 1) + (size_t)(n - 1) * n].x;
+}
+
+// Number of eigenvalues of the tridiagonal (d, e) below x (Sturm count).
+__device__ __forceinline__ int sturm_count(const double* __restrict__ d,
+                                           const double* __restrict__ e2, int n, double x,
+                                           double pivmin) {
+  int cnt = 0;
+  double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
+  cnt += q < 0.0;
+  for (int j = 1; j < n; ++j) {
+    q = d[j] - x - e2[j - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+  }
+  return cnt;
+}
+
+// Eigenvalue i of the tridiagonal by 64-way multisection: one wavefront per
+// eigenvalue, each lane evaluates the Sturm count at one of 64 interior
+// points, and the bracket [lo, hi] with count(lo) <= i < count(hi) shrinks
+// 65-fold per round. Also writes e2 = e^2 (block 0) for the caller.
+__global__ void __launch_bounds__(64) k_tri_multisect(const double* __restrict__ d,
+                                                      const double* __restrict__ e, int n,
+                                                      double* __restrict__ e2s,
+                                                      double* __restrict__ w) {
+  __shared__ double e2[1024];
+  const int lane = threadIdx.x, i = blockIdx.x;
+  double gl = INFINITY, gu = -INFINITY, emax2 = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double el = j > 0 ? fabs(e[j - 1]) : 0.0, er = j < n - 1 ? fabs(e[j]) : 0.0;
+    gl = fmin(gl, d[j] - el - er);
+    gu = fmax(gu, d[j] + el + er);
+    if (j < n - 1) {
+      e2[j] = e[j] * e[j];
+      emax2 = fmax(emax2, e2[j]);
+      if (i == 0) e2s[j] = e2[j];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    gl = fmin(gl, __shfl_xor(gl, off));
+    gu = fmax(gu, __shfl_xor(gu, off));
+    emax2 = fmax(emax2, __shfl_xor(emax2, off));
+  }
+  __syncthreads();
+  const double pivmin = DBL_MIN * fmax(1.0, emax2);
+  const double tnorm = fmax(fabs(gl), fabs(gu));
+  double lo = gl - 2.0 * DBL_EPSILON * tnorm * n - 2.0 * pivmin;
+  double hi = gu + 2.0 * DBL_EPSILON * tnorm * n + 2.0 * pivmin;
+  for (int it = 0; it < 16; ++it) {
+    if (hi - lo <= 2.0 * DBL_EPSILON * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin) break;
+    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+    const bool le = sturm_count(d, e2, n, x, pivmin) <= i;
+    const unsigned long long mask = __ballot(le);
+    const int below = __popcll(mask);  // lanes 0 .. below-1 have count <= i
+    const double xlo = __shfl(x, below > 0 ? below - 1 : 0);
+    const double xhi = __shfl(x, below < 64 ? below : 63);
+    if (below > 0) lo = xlo;
+    if (below < 64) hi = xhi;
+  }
+  if (lane == 0) w[i] = 0.5 * (lo + hi);
+}
+
+// Eigenvectors of the tridiagonal by inverse iteration with partial
+// pivoting LU of (T - lambda I). Eigenvalues closer than ortol form a
+// cluster; one thread handles a whole cluster and orthogonalises each new
+// vector against the cluster's earlier ones (modified Gram-Schmidt), so no
+// thread reads another thread's vectors. Scratch and output are interleaved
+// by eigenvalue index (element j of column t at j * n + t) so that
+// neighbouring threads touch neighbouring addresses.
+__global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
+                                                  const double* __restrict__ e,
+                                                  const double* __restrict__ w, int n,
+                                                  double rel_ortol, double* __restrict__ scr,
+                                                  double* __restrict__ Z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double tnorm = 0.0;  // infinity norm of T
+  for (int j = 0; j < n; ++j)
+    tnorm = fmax(tnorm, fabs(d[j]) + (j > 0 ? fabs(e[j - 1]) : 0.0) + (j < n - 1 ? fabs(e[j]) : 0.0));
+  const double ortol = rel_ortol * tnorm;
+  if (i > 0 && w[i] - w[i - 1] <= ortol) return;  // not a cluster leader
+  int end = i + 1;
+  while (end < n && w[end] - w[end - 1] <= ortol) ++end;
+  double* mult = scr;
+  double* u0 = scr + (size_t)n * n;
+  double* u1 = scr + 2 * (size_t)n * n;
+  double* u2 = scr + 3 * (size_t)n * n;
+  double* swp = scr + 4 * (size_t)n * n;
+  auto at = [n](int j, int t) { return (size_t)j * n + t; };
+  const double eps_t = DBL_EPSILON * fmax(tnorm, DBL_MIN);
+  const double pert = 10.0 * eps_t;
+  double lam_prev = -INFINITY;
+  for (int t = i; t < end; ++t) {
+    double lam = w[t];
+    if (lam - lam_prev < pert) lam = lam_prev + pert;  // separate repeated values
+    lam_prev = lam;
+    // LU of T - lam I with row interchanges; U has up to two superdiagonals
+    double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0, cc = 0.0;
+    for (int r = 0; r < n - 1; ++r) {
+      const double er = e[r], an = d[r + 1] - lam, en = r + 1 < n - 1 ? e[r + 1] : 0.0;
+      if (fabs(ca) >= fabs(er)) {
+        const double mu = ca != 0.0 ? er / ca : 0.0;
+        mult[at(r, i)] = mu;
+        swp[at(r, i)] = 0.0;
+        u0[at(r, i)] = ca;
+        u1[at(r, i)] = cb;
+        u2[at(r, i)] = cc;
+        ca = an - mu * cb;
+        cb = en - mu * cc;
+        cc = 0.0;
+      } else {
+        const double mu = ca / er;
+        mult[at(r, i)] = mu;
+        swp[at(r, i)] = 1.0;
+        u0[at(r, i)] = er;
+        u1[at(r, i)] = an;
+        u2[at(r, i)] = en;
+        const double na = cb - mu * an, nb = cc - mu * en;
+        ca = na;
+        cb = nb;
+        cc = 0.0;
+      }
+    }
+    u0[at(n - 1, i)] = ca;
+    // start vector: deterministic pseudo-random entries in [-1, 1)
+    unsigned long long st = 0x9E3779B97F4A7C15ull ^ (unsigned long long)(t + 1) * 0xBF58476D1CE4E5B9ull;
+    for (int j = 0; j < n; ++j) {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      Z[at(j, t)] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    for (int iter = 0; iter < 3; ++iter) {
+      // forward: row operations of the factorisation
+      for (int r = 0; r < n - 1; ++r) {
+        double yr = Z[at(r, t)], yn = Z[at(r + 1, t)];
+        if (swp[at(r, i)] != 0.0) {
+          const double tmp = yr;
+          yr = yn;
+          yn = tmp;
+        }
+        Z[at(r, t)] = yr;
+        Z[at(r + 1, t)] = yn - mult[at(r, i)] * yr;
+      }
+      // back substitution with U (tiny pivots replaced by eps ||T||)
+      for (int r = n - 1; r >= 0; --r) {
+        double s = Z[at(r, t)];
+        if (r + 1 < n) s -= u1[at(r, i)] * Z[at(r + 1, t)];
+        if (r + 2 < n) s -= u2[at(r, i)] * Z[at(r + 2, t)];
+        double piv = u0[at(r, i)];
+        if (fabs(piv) < eps_t) piv = piv < 0.0 ? -eps_t : eps_t;
+        Z[at(r, t)] = s / piv;
+      }
+      // orthogonalise against the cluster's earlier vectors, normalise
+      for (int q = i; q < t; ++q) {
+        double dot = 0.0;
+        for (int j = 0; j < n; ++j) dot += Z[at(j, q)] * Z[at(j, t)];
+        for (int j = 0; j < n; ++j) Z[at(j, t)] -= dot * Z[at(j, q)];
+      }
+      double nrm = 0.0;
+      for (int j = 0; j < n; ++j) nrm += Z[at(j, t)] * Z[at(j, t)];
+      const double inv = 1.0 / sqrt(nrm);
+      for (int j = 0; j < n; ++j) Z[at(j, t)] *= inv;
+    }
+  }
+}
+
+// U (complex, column-major) = real Z stored interleaved (Z[j * n + t] is
+// row j of eigenvector t).
+__global__ void k_tri_to_complex(const double* __restrict__ Z, int n, z64* __restrict__ U) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int j = (int)(idx % n), t = (int)(idx / n);
+  U[idx] = mk(Z[(size_t)j * n + t], 0.0);
+}
+
+// Triangular factor T (b x b, column-major, upper) of the compact WY form
+// H_k0 ... H_{k0+b-1} = I - V T V^H (forward, columnwise):
+// T_cc = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] (V[:, 0:c]^H v_c).
+__global__ void __launch_bounds__(256) k_wy_tmat(const z64* __restrict__ Vst, int n, int k0, int b,
+                                                 const z64* __restrict__ tau, z64* __restrict__ T) {
+  __shared__ z64 G[kWY][kWY + 1];
+  __shared__ z64 Ts[kWY][kWY + 1];
+  for (int t = threadIdx.x; t < kWY * kWY; t += blockDim.x) {
+    const int a = t % kWY, c = t / kWY;
+    z64 g = mk(0.0, 0.0);
+    if (a < c && c < b) {
+      const z64* va = Vst + (size_t)(k0 + a) * n;
+      const z64* vc = Vst + (size_t)(k0 + c) * n;
+      for (int r = k0 + c + 1; r < n; ++r) g = g + cmulc(va[r], vc[r]);  // v_c is 0 above row k0+c+1
+    }
+    G[a][c] = g;
+    Ts[a][c] = mk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int c = 0; c < b; ++c) {
+    const z64 tc = tau[k0 + c];
+    if (threadIdx.x < c) {
+      const int a = threadIdx.x;
+      z64 s = mk(0.0, 0.0);
+      for (int q = a; q < c; ++q) s = s + Ts[a][q] * G[q][c];
+      Ts[a][c] = mk(-(tc.x * s.x - tc.y * s.y), -(tc.x * s.y + tc.y * s.x));
+    }
+    if (threadIdx.x == 0) Ts[c][c] = tc;
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < b * b; t += blockDim.x) T[t] = Ts[t % b][t / b];
+}
+
+struct EighPlan {
+  int n = 0;
+  const z64* gA = nullptr;
+  const void* gws = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static EighPlan g_eigh;
+
+// Workspace layout (bytes aligned to 256 by construction of the sizes).
+struct EighWs {
+  z64 *vst, *ppart, *spart, *tau, *tmat, *w1, *w2;
+  double *dd, *ee, *e2, *scr, *zt;
+};
+static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
+  const size_t nn = (size_t)n * n;
+  const int nch = (n + kHC - 1) / kHC;
+  const size_t zb = (nn + (size_t)nch * n + (size_t)nch * nch + n + kWY * kWY + 2 * (size_t)kWY * n) *
+                    sizeof(z64);
+  const size_t db = (3 * (size_t)n + 6 * nn) * sizeof(double);
+  CHS_TRY(ensure(ctx, ctx->impl->eig_ws, zb + db + 1024));
+  char* p = (char*)ctx->impl->eig_ws.p;
+  auto take_z = [&p](size_t cnt) {
+    z64* r = (z64*)p;
+    p += ((cnt * sizeof(z64) + 255) / 256) * 256;
+    return r;
+  };
+  auto take_d = [&p](size_t cnt) {
+    double* r = (double*)p;
+    p += ((cnt * sizeof(double) + 255) / 256) * 256;
+    return r;
+  };
+  ws->vst = take_z(nn);
+  ws->ppart = take_z((size_t)nch * n);
+  ws->spart = take_z((size_t)nch * nch);
+  ws->tau = take_z(n);
+  ws->tmat = take_z(kWY * kWY);
+  ws->w1 = take_z((size_t)kWY * n);
+  ws->w2 = take_z((size_t)kWY * n);
+  ws->dd = take_d(n);
+  ws->ee = take_d(n);
+  ws->e2 = take_d(n);
+  ws->scr = take_d(5 * nn);
+  ws->zt = take_d(nn);
+  return CHS_OK;
+}
+
+static void tridiagonalise_launches(chs_context* ctx, z64* A, int n, const EighWs& ws) {
+  for (int k = 0; k + 1 < n; ++k) {
+    const int m = n - k - 1;
+    const unsigned nc = ceil_div(m, kHC), nt = ceil_div(m, kHT);
+    hipLaunchKernelGGL(k_hh_symv, dim3(nc, nc), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
+                       ws.ppart, ws.spart, ws.tau, ws.dd, ws.ee);
+    hipLaunchKernelGGL(k_hh_update, dim3(nt, nt), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
+                       ws.ppart, ws.spart, ws.tau);
+  }
+  hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(64), 0, ctx->stream, A, n, ws.dd);
+}
+
+// Hermitian eigensolve: A (n x n, destroyed) -> eigenvalues ascending in
+// device w, orthonormal eigenvectors in U (n x n column-major).
+int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U) {
+  if (n < 1 || n > 1024) return set_error(ctx, CHS_ERR_INVALID_ARG, "eigensolver limited to 1 <= n <= 1024");
+  EighWs ws;
+  CHS_TRY(eigh_ws(ctx, n, &ws));
+  // 1. tridiagonal reduction (2 (n - 1) launches; graph-replayed)
+  EighPlan& P = g_eigh;
+  if (ctx->impl->use_graphs && n > 1) {
+    if (!P.exec || P.n != n || P.gA != A || P.gws != ctx->impl->eig_ws.p) {
+      if (P.exec) (void)hipGraphExecDestroy(P.exec);
+      P.exec = nullptr;
+      hipGraph_t graph;
+      CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      tridiagonalise_launches(ctx, A, n, ws);
+      CHS_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
+      const hipError_t ie = hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      CHS_HIP(ctx, ie);
+      P.n = n;
+      P.gA = A;
+      P.gws = ctx->impl->eig_ws.p;
+    }
+    CHS_HIP(ctx, hipGraphLaunch(P.exec, ctx->stream));
+  } else {
+    tridiagonalise_launches(ctx, A, n, ws);
+  }
+  CHS_LAUNCH_CHECK(ctx);
+  // 2. eigenvalues (multisection) and tridiagonal eigenvectors
+  hipLaunchKernelGGL(k_tri_multisect, dim3(n), dim3(64), 0, ctx->stream, ws.dd, ws.ee, n, ws.e2, w);
+  CHS_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_tri_invit, dim3(ceil_div(n, 64)), dim3(64), 0, ctx->stream, ws.dd, ws.ee, w,
+                     n, 1e-5, ws.scr, ws.zt);
+  hipLaunchKernelGGL(k_tri_to_complex, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0,
+                     ctx->stream, ws.zt, n, U);
+  CHS_LAUNCH_CHECK(ctx);
+  // 3. U <- Q U, blocks of reflectors from the last to the first
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0), mone = mk(-1.0, 0.0);
+  const int nref = n - 1;
+  for (int kb = ((nref - 1) / kWY) * kWY; nref > 0 && kb >= 0; kb -= kWY) {
+    const int b = std::min(kWY, nref - kb);
+    const int rows = n - kb - 1;
+    hipLaunchKernelGGL(k_wy_tmat, dim3(1), dim3(256), 0, ctx->stream, ws.vst, n, kb, b, ws.tau,
+                       ws.tmat);
+    CHS_LAUNCH_CHECK(ctx);
+    const z64* V = ws.vst + (size_t)kb * n + (kb + 1);
+    CHS_TRY(zgemm(ctx, 'C', 'N', b, n, rows, one, V, n, U + (kb + 1), n, zero, ws.w1, b));
+    CHS_TRY(zgemm(ctx, 'N', 'N', b, n, b, one, ws.tmat, b, ws.w1, b, zero, ws.w2, b));
+    CHS_TRY(zgemm(ctx, 'N', 'N', rows, n, b, mone, V, n, ws.w2, b, one, U + (kb + 1), n));
+  }
+  return CHS_OK;
+}
+
+}  // namespace chs

@@ -9,6 +9,7 @@
 // subspace_diag.rs:229-391): S = L L^H, B = L^{-1} H L^{-H} (Hermitised),
 // B U = U diag(eps) ascending, V = L^{-H} U, psi <- psi V.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 #include <vector>
@@ -36,58 +37,187 @@ __global__ void k_hermitise(z64* __restrict__ M, int n) {
   }
 }
 
-// Right-looking lower Cholesky in one block; info[0] = failing column + 1.
-__global__ void __launch_bounds__(1024) k_cholesky(z64* __restrict__ A, int n, int* __restrict__ info) {
-  __shared__ double piv;
+// Blocked right-looking lower Cholesky, panel width kCB. Per panel:
+// k_chol_panel factors the diagonal block in LDS (every block of the launch
+// does it redundantly, which is cheaper than a launch) and solves its own
+// rows of the panel, X L_kk^H = A; k_chol_update subtracts L_p L_p^H from
+// the trailing lower triangle by 32 x 32 tiles. info[0] = failing column + 1
+// (0 on success); every kernel returns at once when it is set.
+constexpr int kCB = 32;
+
+__global__ void __launch_bounds__(256) k_chol_panel(z64* __restrict__ A, int n, int kb,
+                                                    z64* __restrict__ dscr,
+                                                    int* __restrict__ info) {
+  __shared__ z64 D[kCB][kCB + 1];
   __shared__ int bad;
+  if (info[0]) return;
+  const int b = min(kCB, n - kb);
+  for (int t = threadIdx.x; t < kCB * kCB; t += blockDim.x) {
+    const int i = t % kCB, j = t / kCB;
+    D[i][j] = (i < b && j < b && j <= i) ? A[(size_t)(kb + i) + (size_t)(kb + j) * n] : mk(0.0, 0.0);
+  }
   if (threadIdx.x == 0) bad = 0;
   __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    if (threadIdx.x == 0) {
-      const double d = A[(size_t)j + (size_t)j * n].x;
-      if (!(d > 0.0)) {
-        bad = j + 1;
-      } else {
-        piv = sqrt(d);
-        A[(size_t)j + (size_t)j * n] = mk(piv, 0.0);
-      }
+  for (int j = 0; j < b; ++j) {
+    const double d = D[j][j].x;
+    if (!(d > 0.0)) {
+      if (threadIdx.x == 0) bad = kb + j + 1;
+      break;  // uniform: every thread reads the same d
     }
+    const double piv = sqrt(d), inv = 1.0 / piv;
     __syncthreads();
-    if (bad) break;
-    const double inv = 1.0 / piv;
-    for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) {
-      z64 v = A[(size_t)i + (size_t)j * n];
-      A[(size_t)i + (size_t)j * n] = mk(v.x * inv, v.y * inv);
-    }
+    if (threadIdx.x == 0) D[j][j] = mk(piv, 0.0);
+    for (int i = j + 1 + threadIdx.x; i < b; i += blockDim.x) D[i][j] = mk(D[i][j].x * inv, D[i][j].y * inv);
     __syncthreads();
-    // trailing lower triangle: A[i][c] -= L[i][j] conj(L[c][j]) for j < c <= i
-    const int m = n - j - 1;
-    const long tot = (long)m * (m + 1) / 2;
-    for (long t = threadIdx.x; t < tot; t += blockDim.x) {
-      // map t -> (ii, cc) with 0 <= cc <= ii < m, column-major over cc
-      int cc = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) / 2.0);
-      while ((long)cc * (cc + 1) / 2 > t) --cc;
-      while ((long)(cc + 1) * (cc + 2) / 2 <= t) ++cc;
-      const int rem = (int)(t - (long)cc * (cc + 1) / 2);
-      // (cc, rem) enumerates a lower triangle by rows: row = cc, col = rem
-      const int i = j + 1 + cc, c = j + 1 + rem;
-      const z64 lij = A[(size_t)i + (size_t)j * n], lcj = A[(size_t)c + (size_t)j * n];
-      z64 a = A[(size_t)i + (size_t)c * n];
-      a.x -= lij.x * lcj.x + lij.y * lcj.y;
-      a.y -= lij.y * lcj.x - lij.x * lcj.y;
-      A[(size_t)i + (size_t)c * n] = a;
+    for (int t = threadIdx.x; t < kCB * kCB; t += blockDim.x) {
+      const int i = t % kCB, c = t / kCB;
+      if (c > j && c <= i && i < b) D[i][c] = D[i][c] - D[i][j] * conj(D[c][j]);
     }
     __syncthreads();
   }
-  // zero the strict upper triangle
-  for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) {
-    const int i = (int)(t % n), c = (int)(t / n);
-    if (i < c) A[t] = mk(0.0, 0.0);
+  __syncthreads();
+  if (bad) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) info[0] = bad;
+    return;
   }
-  if (threadIdx.x == 0) info[0] = bad;
+  // the factored diagonal block goes to dscr, not A: other blocks of this
+  // launch may still be reading A's diagonal block
+  if (blockIdx.x == 0)
+    for (int t = threadIdx.x; t < kCB * kCB; t += blockDim.x) {
+      const int i = t % kCB, j = t / kCB;
+      dscr[(size_t)(kb / kCB) * kCB * kCB + t] = j <= i ? D[i][j] : mk(0.0, 0.0);
+    }
+  // rows of the panel below the diagonal block: x L_kk^H = a, one row per thread
+  const int i = kb + b + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  z64 x[kCB];
+#pragma unroll
+  for (int c = 0; c < kCB; ++c) x[c] = c < b ? A[(size_t)i + (size_t)(kb + c) * n] : mk(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < kCB; ++c) {
+    if (c < b) {
+      z64 s = x[c];
+#pragma unroll
+      for (int k = 0; k < c; ++k) s = s - x[k] * conj(D[c][k]);
+      const double inv = 1.0 / D[c][c].x;
+      x[c] = mk(s.x * inv, s.y * inv);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kCB; ++c)
+    if (c < b) A[(size_t)i + (size_t)(kb + c) * n] = x[c];
 }
 
-// X = L^{-1} (lower), rows in sequence, columns in parallel.
+// A[i][j] -= sum_c L[i][c] conj(L[j][c]) over the panel columns c, for the
+// lower-triangle tiles (ti >= tj) of the trailing matrix starting at k0.
+__global__ void __launch_bounds__(256) k_chol_update(z64* __restrict__ A, int n, int kb, int k0,
+                                                     const int* __restrict__ info) {
+  __shared__ z64 Li[32][kCB + 1], Lj[32][kCB + 1];
+  if (info[0]) return;
+  // blockIdx.x enumerates lower tiles (ti, tj), tj <= ti
+  int t = blockIdx.x, ti = 0;
+  while (t > ti) {
+    t -= ti + 1;
+    ++ti;
+  }
+  const int tj = t;
+  const int r0 = k0 + 32 * ti, c0 = k0 + 32 * tj;
+  const int b = min(kCB, n - kb);
+  for (int e = threadIdx.x; e < 32 * kCB; e += blockDim.x) {
+    const int r = e % 32, c = e / 32;
+    const bool okc = c < b;
+    Li[r][c] = (okc && r0 + r < n) ? A[(size_t)(r0 + r) + (size_t)(kb + c) * n] : mk(0.0, 0.0);
+    Lj[r][c] = (okc && c0 + r < n) ? A[(size_t)(c0 + r) + (size_t)(kb + c) * n] : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  // 256 threads x 4 outputs: thread (r, cq) computes rows r, columns cq*4 .. +3
+  const int r = threadIdx.x % 32, cq = threadIdx.x / 32;
+  z64 acc[4] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+  for (int c = 0; c < b; ++c) {
+    const z64 a = Li[r][c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = acc[q] + a * conj(Lj[cq * 4 + q][c]);
+  }
+  const int i = r0 + r;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = c0 + cq * 4 + q;
+    if (j <= i && j < n) {
+      z64* p = A + (size_t)i + (size_t)j * n;
+      *p = *p - acc[q];
+    }
+  }
+}
+
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+// Final Cholesky factor: diagonal blocks from dscr, strict upper triangle 0.
+__global__ void k_chol_finish(z64* __restrict__ A, int n, const z64* __restrict__ dscr,
+                              const int* __restrict__ info) {
+  if (info[0]) return;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * n) return;
+  const int i = (int)(t % n), j = (int)(t / n);
+  if (i < j)
+    A[t] = mk(0.0, 0.0);
+  else if (i / kCB == j / kCB)
+    A[t] = dscr[(size_t)(i / kCB) * kCB * kCB + (i % kCB) + (size_t)(j % kCB) * kCB];
+}
+
+// X = L^{-1} for n <= 64 R: one wavefront per CPW columns j0 .. j0+CPW-1,
+// column-oriented forward substitution (x_k /= L_kk; x_i -= L_ik x_k for
+// i > k) with lane l holding rows l + 64 r in registers, so L is read one
+// coalesced column at a time and no workgroup barrier is needed.
+template <int R, int CPW>
+__global__ void __launch_bounds__(64) k_tri_inv_wave(const z64* __restrict__ L, z64* __restrict__ X,
+                                                     int n) {
+  const int lane = threadIdx.x;
+  const int j0 = blockIdx.x * CPW;
+  z64 x[CPW][R];
+#pragma unroll
+  for (int c = 0; c < CPW; ++c)
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[c][r] = mk(lane + 64 * r == j0 + c ? 1.0 : 0.0, 0.0);
+#pragma unroll
+  for (int rb = 0; rb < R; ++rb) {
+    const int kbeg = max(j0, 64 * rb), kend = min(n, 64 * rb + 64);
+    for (int k = kbeg; k < kend; ++k) {
+      const int lk = k & 63;
+      const z64* col = L + (size_t)k * n;
+      const double inv = 1.0 / col[k].x;
+      z64 xk[CPW];
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) {
+        xk[c] = mk(__shfl(x[c][rb].x, lk) * inv, __shfl(x[c][rb].y, lk) * inv);
+        if (lane == lk) x[c][rb] = xk[c];
+      }
+#pragma unroll
+      for (int r = rb; r < R; ++r) {
+        const int row = lane + 64 * r;
+        if (row > k && row < n) {
+          const z64 l = col[row];
+#pragma unroll
+          for (int c = 0; c < CPW; ++c) x[c][r] = x[c][r] - l * xk[c];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < CPW; ++c) {
+    if (j0 + c >= n) break;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int row = lane + 64 * r;
+      if (row < n) X[(size_t)row + (size_t)(j0 + c) * n] = x[c][r];
+    }
+  }
+}
+
+// X = L^{-1} (lower) for n > 1024: one block, rows in sequence.
+
 __global__ void __launch_bounds__(1024) k_tri_inv(const z64* __restrict__ L, z64* __restrict__ X, int n) {
   for (long t = threadIdx.x; t < (long)n * n; t += blockDim.x) X[t] = mk(0.0, 0.0);
   __syncthreads();
@@ -285,6 +415,17 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
     if (h[0] <= 1e-30 * h[1] || h[0] == 0.0) break;
     // one sweep = 2 (n-1) small launches: captured once into a graph and
     // replayed (no memset nodes inside)
+    if (!ctx->impl->use_graphs) {
+      for (int r = 0; r < P.rounds; ++r) {
+        const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
+        hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, ctx->stream, A, V, n, pr,
+                           (double4*)P.rot.p);
+        hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, ctx->stream, A, n, pr,
+                           (const double4*)P.rot.p);
+      }
+      CHS_LAUNCH_CHECK(ctx);
+      continue;
+    }
     if (!P.exec || P.gA != A || P.gV != V) {
       if (P.exec) (void)hipGraphExecDestroy(P.exec);
       P.exec = nullptr;
@@ -316,10 +457,24 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
 }
 
 int cholesky(chs_context* ctx, z64* A, int n, int* info_host) {
-  if (n > 4096) return set_error(ctx, CHS_ERR_INVALID_ARG, "Cholesky limited to n <= 4096");
   CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
   int* dinfo = (int*)((double*)ctx->impl->small.p + 56);
-  hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(1024), 0, ctx->stream, A, n, dinfo);
+  CHS_TRY(ensure(ctx, ctx->impl->chol_ws, (size_t)ceil_div(n, kCB) * kCB * kCB * sizeof(z64)));
+  z64* dscr = (z64*)ctx->impl->chol_ws.p;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, dinfo, 1, 0);
+  for (int kb = 0; kb < n; kb += kCB) {
+    const int b = std::min(kCB, n - kb);
+    const int rows = n - kb - b;
+    hipLaunchKernelGGL(k_chol_panel, dim3(std::max(1u, ceil_div(rows, 256))), dim3(256), 0,
+                       ctx->stream, A, n, kb, dscr, dinfo);
+    if (rows > 0) {
+      const int t = (int)ceil_div(rows, 32);
+      hipLaunchKernelGGL(k_chol_update, dim3(t * (t + 1) / 2), dim3(256), 0, ctx->stream, A, n, kb,
+                         kb + b, dinfo);
+    }
+  }
+  hipLaunchKernelGGL(k_chol_finish, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0, ctx->stream,
+                     A, n, dscr, dinfo);
   CHS_LAUNCH_CHECK(ctx);
   CHS_HIP(ctx, hipMemcpyAsync(info_host, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -327,7 +482,18 @@ int cholesky(chs_context* ctx, z64* A, int n, int* info_host) {
 }
 
 int tri_inv(chs_context* ctx, const z64* L, z64* X, int n) {
-  hipLaunchKernelGGL(k_tri_inv, dim3(1), dim3(1024), 0, ctx->stream, L, X, n);
+  if (n <= 64)
+    hipLaunchKernelGGL((k_tri_inv_wave<1, 4>), dim3(ceil_div(n, 4)), dim3(64), 0, ctx->stream, L, X, n);
+  else if (n <= 128)
+    hipLaunchKernelGGL((k_tri_inv_wave<2, 4>), dim3(ceil_div(n, 4)), dim3(64), 0, ctx->stream, L, X, n);
+  else if (n <= 256)
+    hipLaunchKernelGGL((k_tri_inv_wave<4, 4>), dim3(ceil_div(n, 4)), dim3(64), 0, ctx->stream, L, X, n);
+  else if (n <= 512)
+    hipLaunchKernelGGL((k_tri_inv_wave<8, 2>), dim3(ceil_div(n, 2)), dim3(64), 0, ctx->stream, L, X, n);
+  else if (n <= 1024)
+    hipLaunchKernelGGL((k_tri_inv_wave<16, 1>), dim3(n), dim3(64), 0, ctx->stream, L, X, n);
+  else
+    hipLaunchKernelGGL(k_tri_inv, dim3(1), dim3(1024), 0, ctx->stream, L, X, n);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
 }
@@ -364,8 +530,12 @@ int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w) {
     CHS_TRY(zgemm(ctx, 'N', 'C', n, n, n, one, T, n, Li, n, zero, A, n));
     CHS_TRY(hermitise(ctx, A, n));
   }
-  int sweeps = 0;
-  CHS_TRY(heev_jacobi(ctx, A, n, w, U, V, 60, &sweeps));
+  if (std::getenv("CHS_EIGH_JACOBI")) {
+    int sweeps = 0;
+    CHS_TRY(heev_jacobi(ctx, A, n, w, U, V, 60, &sweeps));
+  } else {
+    CHS_TRY(heev_tridiag(ctx, A, n, w, U));
+  }
   if (B) {
     // A <- Li^H U
     CHS_TRY(zgemm(ctx, 'C', 'N', n, n, n, one, Li, n, U, n, zero, A, n));

@@ -14,6 +14,7 @@ int cholesky(chs_context* ctx, z64* A, int n, int* info_host);
 int tri_inv(chs_context* ctx, const z64* L, z64* X, int n);
 int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int max_sweeps,
                 int* sweeps_out);
+int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U);
 int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w);
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
                        double* eps_dev_out);

@@ -64,7 +64,7 @@ def test_zhegvd_kat_generalized(ctx):
     assert np.abs(w - [0.5, 4 / 3, 3.0, 5.0]).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [6, 33, 160, 512])
+@pytest.mark.parametrize("n", [6, 33, 160, 512, 1000])
 def test_zhegvd_random_vs_scipy(ctx, n):
     import scipy.linalg as sl
     rng = np.random.default_rng(n)
@@ -80,6 +80,37 @@ def test_zhegvd_random_vs_scipy(ctx, n):
     # invariants V^H S V = I, V^H H V = diag(w) (subspace_diag.rs:680-858)
     assert np.abs(V.conj().T @ Sm @ V - np.eye(n)).max() < 1e-9
     assert np.abs(V.conj().T @ Hm @ V - np.diag(w)).max() < 1e-8 * np.abs(w).max()
+
+
+@pytest.mark.parametrize("n", [1, 2, 97, 300])
+def test_heev_degenerate_spectrum(ctx, n):
+    """Standard problem with eigenvalues of multiplicity up to 4 (the
+    degenerate shells of a cubic cell): eigenvalues exact to 1e-12 and an
+    orthonormal eigenbasis even inside the degenerate subspaces."""
+    rng = np.random.default_rng(7 + n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lev = np.sort(np.repeat(np.linspace(-1.0, 2.0, (n + 3) // 4), 4)[:n])
+    Hm = (Q * lev) @ Q.conj().T
+    Hm = 0.5 * (Hm + Hm.conj().T)
+    a = chs.Gpu.from_host(ctx, fmat(Hm))
+    w = chs.zhegvd(ctx, a)
+    assert np.abs(w - lev).max() < 1e-12
+    U = a.sync_to_host().T
+    assert np.abs(U.conj().T @ U - np.eye(n)).max() < 1e-10
+    assert np.abs(Hm @ U - U * w).max() < 1e-11
+
+
+def test_cholesky_failure_past_first_panel(ctx):
+    """Blocked Cholesky reports the failing column (1-based) beyond the first
+    32-column panel."""
+    n = 100
+    Sm = np.eye(n, dtype=complex)
+    Sm[69, 69] = -1.0
+    a = chs.Gpu.from_host(ctx, fmat(np.eye(n, dtype=complex)))
+    b = chs.Gpu.from_host(ctx, fmat(Sm))
+    with pytest.raises(chs.RayleighRitzFailed) as e:
+        chs.zhegvd(ctx, a, b)
+    assert e.value.diag.solver_info == 70
 
 
 def test_zhegvd_not_positive_definite(ctx):
