@@ -126,6 +126,16 @@ def lib():
             "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
             "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
             "chs_band_cg": [vp, vp, i, i, d, i, vp, dp, vp, vp],
+            "chs_build_system": [vp, C.POINTER(ScfParams), dp, C.POINTER(i), C.POINTER(i), vp, vp,
+                                 vp, vp, vp, vp],
+            "chs_mlp_create": [vp, u64, C.POINTER(vp)],
+            "chs_mlp_destroy": [vp],
+            "chs_mlp_set_params": [vp, vp, vp],
+            "chs_mlp_get_params": [vp, vp, vp],
+            "chs_mlp_loss_grad": [vp, vp, vp, vp, vp, vp, C.c_int64, dp, vp],
+            "chs_mlp_train": [vp, vp, vp, vp, vp, vp, C.c_int64, i, C.c_int64, C.c_float, dp],
+            "chs_mlp_scale_table": [vp, vp, vp, i, vp],
+            "chs_mlp_pairs": [vp, vp, vp, dp, i, d, vp, vp, vp, vp, C.POINTER(i)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -444,6 +454,105 @@ def run_scf(ctx, params, pos, want_rho=False):
     out["rho"] = rho
     out["status"] = rc
     return out
+
+
+def build_system(ctx, params, pos):
+    """Plane waves, V_ion and (USPP) projectors of a synthetic cell on the
+    device. Returns a dict of device arrays: miller, kin, vion and, with
+    projectors, beta (P, n_pw), d0, q."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    n_pw, n_proj = C.c_int(), C.c_int()
+    ctx.check(lib().chs_build_system(ctx.h, C.byref(params), _dptr(pos), C.byref(n_pw),
+                                     C.byref(n_proj), None, None, None, None, None, None))
+    npw, P = n_pw.value, n_proj.value
+    out = {"miller": Gpu(ctx, (npw, 3), np.int32), "kin": Gpu(ctx, (npw,), np.float64),
+           "vion": Gpu(ctx, (params.n_grid ** 3,), np.float64), "n_pw": npw, "n_proj": P}
+    if P:
+        out.update(beta=Gpu(ctx, (P, npw), np.complex128), d0=Gpu(ctx, (P,), np.float64),
+                   q=Gpu(ctx, (P,), np.float64))
+    ptr = lambda k: out[k].ptr if k in out else None
+    ctx.check(lib().chs_build_system(ctx.h, C.byref(params), _dptr(pos), C.byref(n_pw),
+                                     C.byref(n_proj), out["miller"].ptr, out["kin"].ptr,
+                                     out["vion"].ptr, ptr("beta"), ptr("d0"), ptr("q")))
+    return out
+
+
+class Mlp:
+    """The learned band-CG preconditioner network on a context (chs_mlp_*)."""
+    N_PARAMS = 8705
+
+    def __init__(self, ctx, seed=11):
+        self.ctx = ctx
+        h = vp()
+        ctx.check(lib().chs_mlp_create(ctx.h, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chs_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        th = np.zeros(self.N_PARAMS, np.float32)
+        self.ctx.check(lib().chs_mlp_get_params(self.ctx.h, self.h, th.ctypes.data_as(vp)))
+        return th
+
+    @params.setter
+    def params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == self.N_PARAMS
+        self.ctx.check(lib().chs_mlp_set_params(self.ctx.h, self.h, th.ctypes.data_as(vp)))
+
+    def loss_grad(self, pairs, n=None):
+        n = pairs["n"] if n is None else n
+        loss = C.c_double()
+        g = np.zeros(self.N_PARAMS, np.float32)
+        self.ctx.check(lib().chs_mlp_loss_grad(self.ctx.h, self.h, pairs["feat"].ptr,
+                                               pairs["r"].ptr, pairs["e"].ptr, pairs["w"].ptr, n,
+                                               C.byref(loss), g.ctypes.data_as(vp)))
+        return loss.value, g
+
+    def train(self, pairs, steps, batch, lr, n=None):
+        n = pairs["n"] if n is None else n
+        trace = np.zeros(max(steps, 1))
+        self.ctx.check(lib().chs_mlp_train(self.ctx.h, self.h, pairs["feat"].ptr, pairs["r"].ptr,
+                                           pairs["e"].ptr, pairs["w"].ptr, n, steps, batch, lr,
+                                           _dptr(trace)))
+        return trace[:steps]
+
+    def scale_table(self, psi, out=None):
+        out = out if out is not None else Gpu(self.ctx, psi.shape, np.float32)
+        self.ctx.check(lib().chs_mlp_scale_table(self.ctx.h, self.h, psi.ptr, psi.shape[0],
+                                                 out.ptr))
+        return out
+
+
+def mlp_pair_buffers(ctx, n):
+    """Device arrays for n training pairs (feat n x 4, r / e complex64, w)."""
+    return {"feat": Gpu(ctx, (n, 4), np.float32), "r": Gpu(ctx, (n,), np.complex64),
+            "e": Gpu(ctx, (n,), np.complex64), "w": Gpu(ctx, (n,), np.float32), "n": 0,
+            "cap": n}
+
+
+def mlp_pairs(ctx, psi, psi_ref, eps_ref, pairs, gap_min=1e-3):
+    """Append the training pairs of block psi against the converged psi_ref to
+    the buffers of mlp_pair_buffers; returns the number of bands used."""
+    n_b, n_pw = psi.shape
+    o = pairs["n"]
+    if o + n_b * n_pw > pairs["cap"]:
+        raise ValueError("pair buffers full")
+    used = C.c_int()
+    er = np.ascontiguousarray(eps_ref, np.float64)
+    at = lambda k, sz: C.c_void_p(pairs[k].ptr.value + o * sz)
+    ctx.check(lib().chs_mlp_pairs(ctx.h, psi.ptr, psi_ref.ptr, _dptr(er), n_b, gap_min,
+                                  at("feat", 16), at("r", 8), at("e", 8), at("w", 4),
+                                  C.byref(used)))
+    pairs["n"] = o + n_b * n_pw
+    return used.value
 
 
 def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None):

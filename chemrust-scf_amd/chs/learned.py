@@ -1,0 +1,129 @@
+"""Training data, training and evaluation of the learned band-CG
+preconditioner (chs_mlp_* in libchs_gpu.so).
+
+Plain runs: for a seeded displacement sample of the synthetic cell, with
+the potential fixed at V_ion (+ USPP projectors when asked), the converged
+block psi* comes from plain Chebyshev-filtered subspace iteration with
+Rayleigh-Ritz; the unconverged blocks are the early subspace-iteration
+passes and a few band-CG (TPA) steps started from them. Each unconverged
+block gives residual / correction pairs (chs_mlp_pairs); the network is
+trained on the pairs of many samples and judged on held-out samples by
+band-CG steps to convergence against plain TPA, with psi*'s eigenvalues as
+the reference.
+"""
+import numpy as np
+
+import chs
+
+
+def start_block(n_b, kin, seed):
+    """Complex N(0,1) / (1 + 2 kin) start vectors from the 48-bit LCG with
+    Box-Muller normals, the SCF driver's generator: per plane wave, the real
+    then the imaginary part, each from two uniforms."""
+    kin = np.asarray(kin)
+    n = n_b * kin.size * 4
+    st = int(seed) & 0xFFFFFFFFFFFF
+    u = np.empty(n)
+    for i in range(n):
+        st = (st * 25214903917 + 11) & 0xFFFFFFFFFFFF
+        u[i] = st / 281474976710656.0
+    z = np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
+    x = z.reshape(n_b, kin.size, 2)
+    return (x[..., 0] + 1j * x[..., 1]) / (1.0 + 2.0 * kin)
+
+
+class Sample:
+    """One seeded system bound on a context."""
+
+    def __init__(self, ctx, seed, n_rep=1, n_grid=36, n_bands=24, uspp=False):
+        self.ctx = ctx
+        self.params = chs.scf_params(n_rep=n_rep, n_grid=n_grid, n_bands=n_bands,
+                                     n_proj_per_atom=4 if uspp else 0,
+                                     q0=0.25 if uspp else 0.0)
+        self.pos = chs.synth_positions(n_rep, seed=seed)
+        self.sys = chs.build_system(ctx, self.params, self.pos)
+        self.H = chs.Hamiltonian(ctx, n_grid, self.sys["miller"], self.sys["kin"],
+                                 self.sys["vion"])
+        if uspp:
+            self.H.bind_projectors(self.sys["beta"], self.sys["d0"], self.sys["q"])
+        self.kin = self.sys["kin"].sync_to_host()
+        self.n_b = n_bands
+        self.seed = seed
+
+    def bind(self):
+        """Re-bind this sample's arrays (another sample may have been bound)."""
+        self.H = chs.Hamiltonian(self.ctx, self.params.n_grid, self.sys["miller"],
+                                 self.sys["kin"], self.sys["vion"])
+        if self.params.n_proj_per_atom:
+            self.H.bind_projectors(self.sys["beta"], self.sys["d0"], self.sys["q"])
+
+    def chfsi(self, psi, passes, degree, eps=None):
+        """`passes` filter + Rayleigh-Ritz passes of a device c128 block (in place)."""
+        lmax, _ = chs.lanczos_bound(self.ctx, 10, 4)
+        for _ in range(passes):
+            b4 = chs.compute_spectral_bounds(eps, lmax)
+            chs.chebyshev_filter(self.ctx, psi, degree, b4, out=psi)
+            eps, _ = chs.rayleigh_ritz(self.ctx, psi)
+        return eps
+
+    def converged(self, degree=16, max_passes=60, tol=1e-12):
+        """psi* and eps* by plain subspace iteration to eigenvalue changes < tol."""
+        psi = chs.Gpu.from_host(self.ctx, start_block(self.n_b, self.kin, 1000 + self.seed))
+        eps = self.chfsi(psi, 2, degree)
+        for _ in range(max_passes):
+            new = self.chfsi(psi, 1, degree, eps)
+            done = np.abs(new - eps).max() < tol
+            eps = new
+            if done:
+                break
+        return psi, eps
+
+    def rough(self, passes=2, degree=6):
+        """An unconverged block: `passes` low-degree passes from a fresh start."""
+        psi = chs.Gpu.from_host(self.ctx, start_block(self.n_b, self.kin, 2000 + self.seed))
+        eps = self.chfsi(psi, passes, degree)
+        return psi, eps
+
+
+def copy_block(ctx, psi):
+    out = chs.Gpu(ctx, psi.shape, psi.dtype)
+    ctx.check(chs.lib().chs_d2d(ctx.h, out.ptr, psi.ptr, psi.nbytes))
+    return out
+
+
+def collect_pairs(ctx, samples, pairs, cg_steps=(1, 3), gap_min=1e-3):
+    """Training pairs of every sample: its early subspace-iteration blocks
+    (1 and 2 passes) and band-CG (TPA) from them after each of cg_steps."""
+    used = 0
+    for s in samples:
+        s.bind()
+        ref, eps_ref = s.converged()
+        for passes in (1, 2):
+            blk, _ = s.rough(passes)
+            used += chs.mlp_pairs(ctx, blk, ref, eps_ref, pairs, gap_min)
+            for k in cg_steps:
+                b2 = copy_block(ctx, blk)
+                chs.band_cg_minimize(ctx, b2, k, 1e-12, precond=0)
+                used += chs.mlp_pairs(ctx, b2, ref, eps_ref, pairs, gap_min)
+    return used
+
+
+def evaluate(ctx, sample, net, tol=1e-9, max_steps=300, uspp_term=False):
+    """Band-CG from the same rough block with TPA and with the network.
+    Returns dict with total steps, max |eps - eps*| for both."""
+    sample.bind()
+    ref, eps_ref = sample.converged()
+    blk, _ = sample.rough(2)
+    out = {}
+    for name in ("tpa", "mlp"):
+        b = copy_block(ctx, blk)
+        if name == "tpa":
+            eps, steps, conv = chs.band_cg_minimize(ctx, b, max_steps, tol,
+                                                    precond=1 if uspp_term else 0)
+        else:
+            table = net.scale_table(b)
+            eps, steps, conv = chs.band_cg_minimize(ctx, b, max_steps, tol, precond=2,
+                                                    mlp_scale=table)
+        out[name] = {"steps": int(steps.sum()), "max_steps": int(steps.max()),
+                     "converged": int(conv.sum()), "eps_err": float(np.abs(eps - eps_ref).max())}
+    return out

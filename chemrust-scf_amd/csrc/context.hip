@@ -106,7 +106,7 @@ int chs_context_destroy(chs_context* ctx) {
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
                     &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
-                    &im->proj_ws, &im->chol_ws, &im->eig_ws,
+                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws,
                     &im->ham.st_mem, &im->ham.veff_f, &im->ham.beta_f, &im->ham.sinv_m,
                     &im->ham.sinv_m_f};
   for (DevBuf* b : bufs) release(*b);

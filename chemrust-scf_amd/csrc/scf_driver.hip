@@ -64,6 +64,17 @@ struct ScfState {
   }
 };
 
+// D0 ~ U[d_min, d_max] from the 48-bit LCG (seed_d), one value per projector
+static std::vector<double> seeded_d0(const chs_scf_params* p, int P) {
+  std::vector<double> d0(P);
+  uint64_t s = p->seed_d;
+  for (int i = 0; i < P; ++i) {
+    s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
+    d0[i] = p->d_min + (p->d_max - p->d_min) * ((double)s / 281474976710656.0);
+  }
+  return d0;
+}
+
 static double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -118,12 +129,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     CHS_TRY(ensure(ctx, S.beta, (size_t)P * n_pw * sizeof(z64)));
     CHS_TRY(build_beta(ctx, (const int32_t*)S.miller.p, n_pw, L, p->kfrac,
                        (const double*)S.pos.p, n_atoms, np, p->beta_rc, (z64*)S.beta.p));
-    std::vector<double> d0(P), q(P, p->q0);
-    uint64_t s = p->seed_d;
-    for (int i = 0; i < P; ++i) {
-      s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
-      d0[i] = p->d_min + (p->d_max - p->d_min) * ((double)s / 281474976710656.0);
-    }
+    std::vector<double> d0 = seeded_d0(p, P), q(P, p->q0);
     for (DevBuf* b : {&S.d0, &S.dscr, &S.q}) CHS_TRY(ensure(ctx, *b, P * sizeof(double)));
     CHS_HIP(ctx, hipMemcpy(S.d0.p, d0.data(), P * sizeof(double), hipMemcpyHostToDevice));
     CHS_HIP(ctx, hipMemcpy(S.dscr.p, d0.data(), P * sizeof(double), hipMemcpyHostToDevice));
@@ -276,6 +282,52 @@ int chs_scf_run(chs_context* ctx, const chs_scf_params* p, const double* pos, ch
     return set_error(ctx, CHS_ERR_INVALID_ARG, "bad SCF parameters");
   *res = chs_scf_result{};
   return scf_run_impl(ctx, p, pos, res, energies, eps, rho);
+}
+
+int chs_build_system(chs_context* ctx, const chs_scf_params* p, const double* pos, int* n_pw_out,
+                     int* n_proj_out, int32_t* d_miller, double* d_kin, double* d_vion,
+                     double* d_beta, double* d_d0, double* d_q) {
+  if (!ctx || !p || !pos || !n_pw_out || !n_proj_out || p->n_rep <= 0) return CHS_ERR_INVALID_ARG;
+  const int np = p->n_proj_per_atom;
+  if (np != 0 && np != 1 && np != 4)
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "projectors per atom must be 0, 1 or 4");
+  const int n = p->n_grid;
+  const double L = p->n_rep * p->a0, omega = L * L * L;
+  const int n_atoms = 8 * p->n_rep * p->n_rep * p->n_rep;
+  int n_pw = 0;
+  CHS_TRY(chs_gsphere(ctx, L, p->ecut, p->kfrac, n, 0, &n_pw, nullptr, nullptr, nullptr));
+  const int P = np * n_atoms;
+  *n_pw_out = n_pw;
+  *n_proj_out = P;
+  if (!d_miller && !d_kin && !d_vion && !d_beta && !d_d0 && !d_q) return CHS_OK;
+  if (!d_miller || !d_kin || !d_vion || (P > 0 && (!d_beta || !d_d0 || !d_q)))
+    return CHS_ERR_INVALID_ARG;
+  DevBuf idx;
+  int got = 0;
+  CHS_TRY(ensure(ctx, idx, (size_t)n_pw * sizeof(int32_t)));
+  int rc = chs_gsphere(ctx, L, p->ecut, p->kfrac, n, n_pw, &got, d_miller, (int32_t*)idx.p, d_kin);
+  release(idx);
+  CHS_TRY(rc);
+  GridOps G;
+  CHS_TRY(G.init(ctx, n, L, omega));
+  CHS_TRY(G.set_atoms(pos, n_atoms));
+  CHS_TRY(G.vion(p->zval, p->rc_loc, d_vion));
+  if (P > 0) {
+    DevBuf dpos;
+    CHS_TRY(ensure(ctx, dpos, 3 * (size_t)n_atoms * sizeof(double)));
+    CHS_HIP(ctx, hipMemcpy(dpos.p, pos, 3 * (size_t)n_atoms * sizeof(double), hipMemcpyHostToDevice));
+    rc = build_beta(ctx, d_miller, n_pw, L, p->kfrac, (const double*)dpos.p, n_atoms, np,
+                    p->beta_rc, (z64*)d_beta);
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    release(dpos);
+    CHS_TRY(rc);
+    std::vector<double> d0 = seeded_d0(p, P), q(P, p->q0);
+    CHS_HIP(ctx, hipMemcpy(d_d0, d0.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    CHS_HIP(ctx, hipMemcpy(d_q, q.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    ctx->h2d += 3 * (size_t)n_atoms * sizeof(double) + 2 * (size_t)P * sizeof(double);
+  }
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return CHS_OK;
 }
 
 }  // extern "C"

@@ -242,6 +242,59 @@ typedef struct chs_scf_result {
 int chs_scf_run(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
                 double* energies, double* eps, double* rho);
 
+/* ---- system set-up on the device (the construction run_scf performs
+ * before its loop, src/scf.rs:358-376 with the synthetic cell of the
+ * tests, tests/phase0_gate1_consistency.rs:75-112,187-197) --------------
+ * Plane waves of p (lexicographic (h, k, l)), V_ion on the n^3 grid and,
+ * with n_proj_per_atom > 0, the projectors beta (P x n_pw c128, atom-major),
+ * unscreened D0 (seeded uniform [d_min, d_max]) and Q (= q0). Call with all
+ * device pointers NULL to get n_pw and n_proj, then again with buffers. */
+int chs_build_system(chs_context* ctx, const chs_scf_params* p, const double* pos, int* n_pw,
+                     int* n_proj, int32_t* d_miller, double* d_kin, double* d_vion,
+                     double* d_beta, double* d_d0, double* d_q);
+
+/* ---- learned band-CG preconditioner (no reference counterpart: the
+ * reference's band CG has the TPA preconditioner only,
+ * uspp_preconditioner.rs:148-180) ---------------------------------------
+ * Network 4 -> 64 -> 64 -> 64 -> 1 (SiLU), bfloat16 weights, FP32
+ * accumulation; features f = [kin_g / E_b, log E_b, V_mean, V_std] of each
+ * (band b, plane wave g) with E_b the band's kinetic energy and V the bound
+ * local potential; scale s = t(kin_g / E_b) exp(o) / E_b (t: the TPA
+ * polynomial), so a zero-output network is per-band TPA. Parameters (FP32
+ * master, CHS_MLP_PARAMS floats): W1[64][4], b1, W2[64][64], b2, W3[64][64],
+ * b3, w4[64], b4. Training pairs: features (f32, n x 4), residual r and
+ * correction e (interleaved complex f32, n), weight w (f32, n); loss
+ * L = sum_i w_i |s_i r_i + e_i|^2, minimised with Adam. */
+#define CHS_MLP_PARAMS 8705
+typedef struct chs_mlp chs_mlp;
+int chs_mlp_create(chs_context* ctx, uint64_t seed, chs_mlp** out);
+int chs_mlp_destroy(chs_mlp* m);
+int chs_mlp_set_params(chs_context* ctx, chs_mlp* m, const float* theta); /* resets Adam */
+int chs_mlp_get_params(chs_context* ctx, chs_mlp* m, float* theta);
+/* L and dL/dtheta (host, CHS_MLP_PARAMS floats, may be NULL) over all pairs */
+int chs_mlp_loss_grad(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
+                      const float* d_e, const float* d_w, int64_t n, double* loss, float* grad);
+/* `steps` Adam steps (learning rate lr) on consecutive minibatches of
+ * `batch` pairs of a fixed permutation; loss_trace (host, steps, may be
+ * NULL) receives each minibatch's loss before its update. */
+int chs_mlp_train(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
+                  const float* d_e, const float* d_w, int64_t n, int steps, int64_t batch,
+                  float lr, double* loss_trace);
+/* Scale table (f32, n_b x n_pw) of a c128 block on the bound Hamiltonian,
+ * for chs_band_cg's precond 2. */
+int chs_mlp_scale_table(chs_context* ctx, chs_mlp* m, const double* d_psi, int n_b,
+                        float* d_table);
+/* Training pairs of an unconverged c128 block d_psi against the converged
+ * block d_psi_ref (eigenvalues eps_ref, host) on the bound Hamiltonian:
+ * r = (H - eps S) psi^ with psi^ = psi / |psi|_S, e = psi_ref (phase
+ * aligned) - psi^ with its psi^ component removed. Bands closer than
+ * gap_min to a neighbour, or whose overlap with the reference band is
+ * below 0.9, get weight 0; the others 1 / (n_used |e_b|^2). Outputs n_b x
+ * n_pw pairs in band-major order. */
+int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref,
+                  const double* eps_ref, int n_b, double gap_min, float* d_feat, float* d_r,
+                  float* d_e, float* d_w, int* n_used);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,142 @@
+"""GPU tests of the learned band-CG preconditioner: the device forward /
+backward / Adam kernels against the numpy oracle (oracle/mlp.py, itself
+pinned by finite differences in test_oracle_mlp.py), the scale table, the
+training-pair builder's invariants, and what the trained network gains over
+plain TPA in band-CG on held-out samples."""
+import numpy as np
+import pytest
+
+import chs
+import mlp as M
+import pyoracle as O
+from chs import learned
+
+pytestmark = pytest.mark.gpu
+
+
+def _theta(seed=5):
+    rng = np.random.default_rng(seed)
+    th = M.init_params(11).astype(np.float64) + rng.standard_normal(M.N_PARAMS) * 0.05
+    return M.to_bf16(th).astype(np.float32)  # master == bf16 copy: same weights as the oracle
+
+
+def _pairs(ctx, n, seed=3):
+    rng = np.random.default_rng(seed)
+    f = np.column_stack([rng.uniform(0, 8, n), rng.uniform(-1, 1, n), rng.uniform(-0.5, 0, n),
+                         rng.uniform(0.1, 0.6, n)]).astype(np.float32)
+    r = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    e = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    w = (rng.uniform(0.5, 1.5, n) / n).astype(np.float32)
+    p = {"feat": chs.Gpu.from_host(ctx, f), "r": chs.Gpu.from_host(ctx, r),
+         "e": chs.Gpu.from_host(ctx, e), "w": chs.Gpu.from_host(ctx, w), "n": n}
+    return p, (f.astype(np.float64), r.astype(np.complex128), e.astype(np.complex128),
+               w.astype(np.float64))
+
+
+@pytest.mark.parametrize("n", [1, 37, 5000])
+def test_loss_grad_vs_oracle(ctx, n):
+    net = chs.Mlp(ctx)
+    th = _theta()
+    net.params = th
+    p, (f, r, e, w) = _pairs(ctx, n)
+    L, g = net.loss_grad(p)
+    Lo, go = M.loss_grad(th.astype(np.float64), f, r, e, w)
+    assert abs(L - Lo) <= 2e-5 * abs(Lo)
+    scale = np.abs(go).max()
+    assert np.abs(g - go).max() <= 2e-4 * scale, np.abs(g - go).max() / scale
+
+
+def test_adam_step_vs_oracle(ctx):
+    net = chs.Mlp(ctx)
+    th = _theta(7)
+    net.params = th
+    p, _ = _pairs(ctx, 4096, 9)
+    _, g = net.loss_grad(p)
+    net.train(p, 1, 4096, 1e-3)  # full batch: the gradient above
+    got = net.params
+    ref, _, _ = M.adam_step(th.astype(np.float64), np.zeros(M.N_PARAMS), np.zeros(M.N_PARAMS),
+                            g.astype(np.float64), 1, 1e-3)
+    assert np.abs(got - ref).max() < 1e-6
+
+
+def test_untrained_table_is_per_band_tpa(ctx):
+    m = O.Model(O.default_params(), O.positions(1))
+    mil, idx, kin = m.pw()
+    H = chs.Hamiltonian(ctx, 36, mil, kin, m.vion())
+    psi = m.initial_psi(8)
+    g = chs.Gpu.from_host(ctx, psi)
+    net = chs.Mlp(ctx)  # zero output layer
+    t = net.scale_table(g).sync_to_host()
+    eb = (np.abs(psi) ** 2 @ kin) / (np.abs(psi) ** 2).sum(1)
+    ref = M.tpa(kin[None, :] / eb[:, None]) / eb[:, None]
+    assert np.abs(t - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+def test_scale_table_vs_oracle(ctx):
+    m = O.Model(O.default_params(), O.positions(1))
+    mil, idx, kin = m.pw()
+    v = m.vion()
+    H = chs.Hamiltonian(ctx, 36, mil, kin, v)
+    psi = m.initial_psi(6)
+    net = chs.Mlp(ctx)
+    th = _theta(13)
+    net.params = th
+    t = net.scale_table(chs.Gpu.from_host(ctx, psi)).sync_to_host()
+    eb = (np.abs(psi) ** 2 @ kin) / (np.abs(psi) ** 2).sum(1)
+    f = M.features(kin, eb, v.mean(), v.std()).reshape(-1, 4)
+    ref = M.scale(th.astype(np.float64), f).reshape(t.shape)
+    assert np.abs(t / ref - 1).max() < 2e-4
+
+
+def test_pairs_invariants(ctx):
+    s = learned.Sample(ctx, seed=21, n_bands=16)
+    ref, eps_ref = s.converged()
+    blk, _ = s.rough(2)
+    pairs = chs.mlp_pair_buffers(ctx, 16 * s.H.n_pw)
+    used = chs.mlp_pairs(ctx, blk, ref, eps_ref, pairs)
+    assert 0 < used <= 16
+    w = pairs["w"].sync_to_host().reshape(16, -1)
+    r = pairs["r"].sync_to_host().reshape(16, -1).astype(np.complex128)
+    e = pairs["e"].sync_to_host().reshape(16, -1).astype(np.complex128)
+    f = pairs["feat"].sync_to_host().reshape(16, -1, 4)
+    psi = blk.sync_to_host()
+    psi = psi / np.linalg.norm(psi, axis=1, keepdims=True)  # S = I here
+    # r is orthogonal to psi (eps is the Rayleigh quotient); e has no psi component
+    assert np.abs(np.einsum("bg,bg->b", psi.conj(), r)).max() < 1e-5 * np.abs(r).max()
+    assert np.abs(np.einsum("bg,bg->b", psi.conj(), e)).max() < 1e-5 * np.abs(e).max()
+    # used bands carry weights 1 / (used |e_b|^2): the zero-scale loss is 1
+    wb = w[:, 0]
+    assert np.allclose(w, wb[:, None])
+    assert abs((wb * (np.abs(e) ** 2).sum(1)).sum() - 1.0) < 1e-4
+    # features: x = kin / E_b
+    kin = s.kin
+    assert np.allclose(f[:, :, 0] * np.exp(f[:, :, 1]), kin[None, :], rtol=1e-5, atol=1e-6)
+    # a converged block gives (almost) no residual
+    pz = chs.mlp_pair_buffers(ctx, 16 * s.H.n_pw)
+    chs.mlp_pairs(ctx, ref, ref, eps_ref, pz)
+    assert np.abs(pz["r"].sync_to_host()).max() < 1e-3 * np.abs(r).max()
+
+
+def test_training_gain_over_tpa(ctx):
+    """Train on plain-run pairs of 12 seeded samples; on 3 held-out samples
+    band-CG with the network needs fewer steps than with TPA and reaches the
+    same eigenvalues."""
+    train = [learned.Sample(ctx, seed=100 + i, n_bands=24) for i in range(12)]
+    cap = 12 * 2 * 3 * 24 * train[0].H.n_pw + 1024
+    pairs = chs.mlp_pair_buffers(ctx, cap)
+    used = learned.collect_pairs(ctx, train, pairs)
+    assert used > 0
+    net = chs.Mlp(ctx, seed=11)
+    L0, _ = net.loss_grad(pairs)
+    trace = net.train(pairs, 600, 8192, 3e-3)
+    L1, _ = net.loss_grad(pairs)
+    print(f"pairs {pairs['n']} bands used {used}: loss {L0:.4f} -> {L1:.4f}")
+    assert L1 < 0.9 * L0
+    tot = {"tpa": 0, "mlp": 0}
+    for seed in (900, 901, 902):
+        res = learned.evaluate(ctx, learned.Sample(ctx, seed=seed, n_bands=24), net)
+        print(seed, res)
+        for k in tot:
+            tot[k] += res[k]["steps"]
+        assert res["mlp"]["eps_err"] <= max(1e-8, 2 * res["tpa"]["eps_err"])
+    assert tot["mlp"] < tot["tpa"], tot
