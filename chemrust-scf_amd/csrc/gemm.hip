@@ -90,6 +90,109 @@ __global__ void __launch_bounds__(256) k_gemm(int m, int n, int k, cplx<T> alpha
     }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core variant: the same 64 x 64 x 16 LDS tiles; each of the 4
+// wavefronts owns a 32 x 32 complex sub-tile as 2 x 2 MFMA 16 x 16 tiles and
+// forms the complex product from four real 16x16x4 MFMAs per k-step
+// (re += ar br - ai bi, im += ar bi + ai br). Operand layout of both
+// v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32: lane l supplies
+// A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]. Result layout: lane l
+// holds column l % 16, rows 4 (l / 16) + v (f32) or l / 16 + 4 v (f64).
+// ---------------------------------------------------------------------------
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  using acc_t = __attribute__((ext_vector_type(4))) double;
+  __device__ __forceinline__ static acc_t mac(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+  using acc_t = __attribute__((ext_vector_type(4))) float;
+  __device__ __forceinline__ static acc_t mac(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <typename T, int OPA, int OPB>
+__global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
+                                                   const cplx<T>* __restrict__ A, int lda,
+                                                   const cplx<T>* __restrict__ B, int ldb,
+                                                   cplx<T> beta, cplx<T>* __restrict__ C, int ldc,
+                                                   int klen, cplx<T>* __restrict__ part) {
+  using M = Mfma<T>;
+  using acc_t = typename M::acc_t;
+  __shared__ cplx<T> As[GBK][GBM + 1];
+  __shared__ cplx<T> Bs[GBK][GBN + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wr = w >> 1, wc = w & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+  const int kb = blockIdx.z * klen, ke = min(k, kb + klen);
+  acc_t cre[2][2], cim[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cre[i][j] = acc_t{0, 0, 0, 0};
+      cim[i][j] = acc_t{0, 0, 0, 0};
+    }
+  for (int k0 = kb; k0 < ke; k0 += GBK) {
+    for (int t = threadIdx.x; t < GBM * GBK; t += 256) {
+      int r, kk;
+      if (OPA == 0) { r = t % GBM; kk = t / GBM; } else { kk = t % GBK; r = t / GBK; }
+      const int gr = m0 + r, gk = k0 + kk;
+      As[kk][r] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
+    }
+    for (int t = threadIdx.x; t < GBK * GBN; t += 256) {
+      int c, kk;
+      if (OPB == 0) { kk = t % GBK; c = t / GBK; } else { c = t % GBN; kk = t / GBN; }
+      const int gc = n0 + c, gk = k0 + kk;
+      Bs[kk][c] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < GBK / 4; ++s) {
+      const int kk = 4 * s + lk;
+      cplx<T> a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[kk][32 * wr + 16 * i + li];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][32 * wc + 16 * j + li];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cre[i][j] = M::mac(a[i].x, b[j].x, cre[i][j]);
+          cre[i][j] = M::mac(-a[i].y, b[j].y, cre[i][j]);
+          cim[i][j] = M::mac(a[i].x, b[j].y, cim[i][j]);
+          cim[i][j] = M::mac(a[i].y, b[j].x, cim[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = m0 + 32 * wr + 16 * i + M::row(lane, v);
+        const int c = n0 + 32 * wc + 16 * j + li;
+        if (r < m && c < n) {
+          const cplx<T> acc = mk((T)cre[i][j][v], (T)cim[i][j][v]);
+          if (part) {
+            part[(size_t)blockIdx.z * m * n + (size_t)r + (size_t)c * m] = acc;
+          } else {
+            cplx<T> val = alpha * acc;
+            if (beta.x != (T)0 || beta.y != (T)0) val = val + beta * C[(size_t)r + (size_t)c * ldc];
+            C[(size_t)r + (size_t)c * ldc] = val;
+          }
+        }
+      }
+}
+
 template <typename T>
 __global__ void k_gemm_reduce(int m, int n, int nz, const cplx<T>* __restrict__ part,
                               cplx<T> alpha, cplx<T> beta, cplx<T>* __restrict__ C, int ldc) {
@@ -135,10 +238,15 @@ static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cpl
     part = (cplx<T>*)ctx->impl->gemm_ws.p;
   }
   dim3 grid(ceil_div(m, GBM), ceil_div(n, GBN), ksplit);
-#define CHS_GEMM_CASE(OA, OB)                                                                   \
-  if (oa == OA && ob == OB)                                                                     \
-    hipLaunchKernelGGL((k_gemm<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k, alpha, A, \
-                       lda, B, ldb, beta, C, ldc, klen, part);
+#define CHS_GEMM_CASE(OA, OB)                                                                  \
+  if (oa == OA && ob == OB) {                                                                  \
+    if (ctx->impl->use_mfma)                                                                   \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k,   \
+                         alpha, A, lda, B, ldb, beta, C, ldc, klen, part);                     \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_gemm<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k, alpha, \
+                         A, lda, B, ldb, beta, C, ldc, klen, part);                            \
+  }
   CHS_GEMM_CASE(0, 0) CHS_GEMM_CASE(0, 1) CHS_GEMM_CASE(0, 2)
   CHS_GEMM_CASE(1, 0) CHS_GEMM_CASE(1, 1) CHS_GEMM_CASE(1, 2)
   CHS_GEMM_CASE(2, 0) CHS_GEMM_CASE(2, 1) CHS_GEMM_CASE(2, 2)

@@ -60,6 +60,9 @@ struct chs_context::Impl {
   // when CHS_NO_GRAPHS is set to a non-zero value: some kernel tracers
   // cannot follow graph capture, and direct launches run the same kernels.
   bool use_graphs = true;
+  // Complex GEMMs on the matrix cores (MFMA); CHS_GEMM_SIMT=1 selects the
+  // vector-FMA kernel of the same tiling instead.
+  bool use_mfma = true;
 };
 
 namespace chs {
