@@ -109,7 +109,7 @@ int chs_context_destroy(chs_context* ctx) {
                     &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
                     &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws,
                     &im->ham.st_mem, &im->ham.veff_f, &im->ham.beta_f, &im->ham.sinv_m,
-                    &im->ham.sinv_m_f};
+                    &im->ham.sinv_m_f, &im->ham.bmat, &im->ham.bmat_f};
   for (DevBuf* b : bufs) release(*b);
   (void)hipEventDestroy(im->ev0);
   (void)hipEventDestroy(im->ev1);

@@ -45,6 +45,11 @@ struct Hamiltonian {
   DevBuf beta_f;                 // c64 copy
   DevBuf sinv_m;                 // c128 P x P (Q^-1 + beta^H beta)^-1
   DevBuf sinv_m_f;               // c64 copy
+  DevBuf bmat;                   // c128 P x P beta^H beta
+  DevBuf bmat_f;                 // c64 copy
+  // projector coefficients carried through a Chebyshev filter: slots of
+  // W_{k-1}, W_k and the scratch G = beta^H O in the filter workspace
+  int nl_wp = 0, nl_w = 1, nl_g = 2;
 };
 
 }  // namespace chs

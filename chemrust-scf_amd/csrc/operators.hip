@@ -30,34 +30,6 @@ int apply_h_impl(chs_context* ctx, int precision, const void* psi, int n_b, void
   return CHS_OK;
 }
 
-template <typename T>
-__global__ void k_cheb_combine(const cplx<T>* __restrict__ ah, const cplx<T>* __restrict__ p,
-                               cplx<T>* out, const cplx<T>* aux0, size_t n, T c, T inv_e,
-                               int first, double* __restrict__ part) {
-  __shared__ double sh[8];
-  double so = 0, sp = 0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const cplx<T> h = ah[i], x = p[i];
-    cplx<T> o;
-    if (first) {
-      o = mk((h.x - c * x.x) * inv_e, (h.y - c * x.y) * inv_e);
-    } else {
-      const cplx<T> q = aux0[i];
-      o = mk((T)2 * (h.x - c * x.x) * inv_e - q.x, (T)2 * (h.y - c * x.y) * inv_e - q.y);
-    }
-    out[i] = o;
-    so += (double)o.x * o.x + (double)o.y * o.y;
-    sp += (double)x.x * x.x + (double)x.y * x.y;
-  }
-  double a = block_sum(so, sh);
-  double b = block_sum(sp, sh);
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = a;
-    part[2 * blockIdx.x + 1] = b;
-  }
-}
-
 __global__ void k_norm_guard2(const double* __restrict__ part, int n_part, int k,
                               double* __restrict__ flag) {
   __shared__ double sh[8];
@@ -77,6 +49,44 @@ __global__ void k_norm_guard2(const double* __restrict__ part, int n_part, int k
       flag[2] = nc;
       flag[3] = growth;
     }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_sqnorm_part(const cplx<T>* __restrict__ x, size_t n,
+                                                             double* __restrict__ part) {
+  __shared__ double sh[8];
+  double a = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const cplx<T> v = x[i];
+    a += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Growth guard on consecutive iterates: flag[4] keeps |Y_{k-1}|^2 of the
+// previous step; from the second step on, growth > 10 sets flag[0..3] as
+// k_norm_guard2 does.
+__global__ void k_norm_guard_roll(const double* __restrict__ part, int n_part, int k, int first,
+                                  double* __restrict__ flag) {
+  __shared__ double sh[8];
+  double a = 0;
+  for (int i = threadIdx.x; i < n_part; i += blockDim.x) a += part[i];
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    if (!first) {
+      const double nc = sqrt(r), np = sqrt(flag[4]);
+      const double growth = np > 0.0 ? nc / np : 1.0;
+      if (flag[0] == 0.0 && growth > 10.0) {
+        flag[0] = k;
+        flag[1] = np;
+        flag[2] = nc;
+        flag[3] = growth;
+      }
+    }
+    flag[4] = r;
   }
 }
 
@@ -100,26 +110,21 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
     }
     return CHS_OK;
   }
-  // projector path: AY into a scratch block, then the combine kernel
-  CHS_TRY(ensure(ctx, im->ws3, ne * csize(precision)));
-  CHS_TRY(apply_h_impl(ctx, precision, yin, n_b, im->ws3.p, nullptr));
-  if (H.has_s) CHS_TRY(sinv_apply(ctx, precision, im->ws3.p, n_b));
-  CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * kRedBlocks + 64));
+  // projector path: the local step with the fused epilogue, then the
+  // projector terms in coefficient space (cheb_nl_step), then the norm
+  CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
+                        nullptr, c, inv_e, nullptr, nullptr));
+  CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, c, 1.0 / inv_e));
+  CHS_TRY(ensure(ctx, im->part, sizeof(double) * kRedBlocks + 64));
   if (precision == CHS_C128)
-    hipLaunchKernelGGL((k_cheb_combine<double>), dim3(kRedBlocks), dim3(kRedThreads), 0,
-                       ctx->stream, (const z64*)im->ws3.p, (const z64*)yin, (z64*)yout,
-                       (const z64*)yprev, ne, c, inv_e, (int)first, (double*)im->part.p);
+    hipLaunchKernelGGL((k_sqnorm_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                       ctx->stream, (const z64*)yout, ne, (double*)im->part.p);
   else
-    hipLaunchKernelGGL((k_cheb_combine<float>), dim3(kRedBlocks), dim3(kRedThreads), 0,
-                       ctx->stream, (const c32*)im->ws3.p, (const c32*)yin, (c32*)yout,
-                       (const c32*)yprev, ne, (float)c, (float)inv_e, (int)first,
-                       (double*)im->part.p);
+    hipLaunchKernelGGL((k_sqnorm_part<float>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+                       ctx->stream, (const c32*)yout, ne, (double*)im->part.p);
+  hipLaunchKernelGGL(k_norm_guard_roll, dim3(1), dim3(256), 0, ctx->stream,
+                     (const double*)im->part.p, kRedBlocks, k, first ? 1 : 0, flag);
   CHS_LAUNCH_CHECK(ctx);
-  if (!first) {
-    hipLaunchKernelGGL(k_norm_guard2, dim3(1), dim3(256), 0, ctx->stream,
-                       (const double*)im->part.p, kRedBlocks, k, flag);
-    CHS_LAUNCH_CHECK(ctx);
-  }
   return CHS_OK;
 }
 
@@ -146,6 +151,7 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
   void* a = im->ws1.p;
   void* b = im->ws2.p;
   CHS_HIP(ctx, hipMemcpyAsync(a, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  if (H.n_proj > 0) CHS_TRY(cheb_nl_begin(ctx, precision, a, n_b));
   CHS_TRY(cheb_step(ctx, precision, a, n_b, b, nullptr, true, c, inv_e, 1, flag));
   for (int k = 2; k <= degree; ++k) {
     CHS_TRY(cheb_step(ctx, precision, b, n_b, a, a, false, c, inv_e, k, flag));

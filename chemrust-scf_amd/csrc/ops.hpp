@@ -15,5 +15,11 @@ int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, vo
                    void* spsi);
 // x <- S^{-1} x = x - beta M beta^H x (Woodbury).
 int sinv_apply(chs_context* ctx, int precision, void* x, int n_b);
+// Chebyshev filter with projectors, in projector-coefficient space:
+// cheb_nl_begin sets W_0 = beta^H X_0; cheb_nl_step turns the local-potential
+// step O (in place) into X_{k+1} and advances W.
+int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b);
+int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first, double c,
+                 double e);
 
 }  // namespace chs

@@ -58,13 +58,16 @@ static int ensure_c64_projectors(chs_context* ctx) {
     hipLaunchKernelGGL((k_convert<z64, c32>), dim3(ceil_div(nb, 256)), dim3(256), 0, ctx->stream,
                        (const z64*)H.beta, (c32*)H.beta_f.p, nb);
     CHS_LAUNCH_CHECK(ctx);
+    const size_t pp = (size_t)H.n_proj * H.n_proj;
+    CHS_TRY(ensure(ctx, H.bmat_f, pp * sizeof(c32)));
+    hipLaunchKernelGGL((k_convert<z64, c32>), dim3(ceil_div(pp, 256)), dim3(256), 0, ctx->stream,
+                       (const z64*)H.bmat.p, (c32*)H.bmat_f.p, pp);
     if (H.has_s) {
-      const size_t pp = (size_t)H.n_proj * H.n_proj;
       CHS_TRY(ensure(ctx, H.sinv_m_f, pp * sizeof(c32)));
       hipLaunchKernelGGL((k_convert<z64, c32>), dim3(ceil_div(pp, 256)), dim3(256), 0,
                          ctx->stream, (const z64*)H.sinv_m.p, (c32*)H.sinv_m_f.p, pp);
-      CHS_LAUNCH_CHECK(ctx);
     }
+    CHS_LAUNCH_CHECK(ctx);
   }
   return CHS_OK;
 }
@@ -110,6 +113,100 @@ static int nonlocal_t(chs_context* ctx, const cplx<T>* psi, int n_b, cplx<T>* hp
       CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, QW, P, 1.0, spsi, n_pw));
   }
   return CHS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Projector part of one Chebyshev step in coefficient space. With
+// W_k = beta^H X_k, the step X_{k+1} = 2 (S^{-1} H X_k - c X_k) / e - X_{k-1}
+// splits as X_{k+1} = O + (2/e) beta C, where O is the local-potential step
+// (the fused H|psi> pipeline's Chebyshev epilogue) and
+//   C = D W_k - M (Z + B D W_k),   Z = beta^H H_loc X_k,
+// B = beta^H beta, M = (Q^{-1} + B)^{-1} (C = D W_k when S = 1). Z follows
+// from G = beta^H O: Z = (e/2)(G + W_{k-1}) + c W_k (first step:
+// Z = e G + c W_0), and W_{k+1} = G + (2/e) B C. Two n_pw-long GEMMs per
+// step (G and the beta C update) instead of four, and no H|psi> block.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_cheb_coef(const cplx<T>* __restrict__ G, const cplx<T>* __restrict__ Wp,
+                            const cplx<T>* __restrict__ W, const double* __restrict__ d, int P,
+                            size_t n, T a1, T a2, T c, cplx<T>* __restrict__ Z,
+                            cplx<T>* __restrict__ DW) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<T> w = W[i];
+  if (Z) {
+    // a2 == 0 on the first step, where W_{k-1} does not exist (not read)
+    const cplx<T> g = G[i], wp = a2 != (T)0 ? Wp[i] : mk((T)0, (T)0);
+    Z[i] = mk(a1 * g.x + a2 * wp.x + c * w.x, a1 * g.y + a2 * wp.y + c * w.y);
+  }
+  const T dp = (T)d[(int)(i % P)];
+  DW[i] = mk(dp * w.x, dp * w.y);
+}
+
+template <typename T>
+static cplx<T>* nl_slot(chs_context* ctx, int n_b, int slot) {
+  return (cplx<T>*)ctx->impl->proj_ws.p + (size_t)slot * ctx->impl->ham.n_proj * n_b;
+}
+
+template <typename T>
+static int cheb_nl_begin_t(chs_context* ctx, const cplx<T>* X0, int n_b) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 5 * (size_t)P * n_b * sizeof(cplx<T>)));
+  H.nl_wp = 0;
+  H.nl_w = 1;
+  H.nl_g = 2;
+  const cplx<T>* beta = sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
+  return gemm_any<T>(ctx, 'C', 'N', P, n_b, H.n_pw, 1.0, beta, H.n_pw, X0, H.n_pw, 0.0,
+                     nl_slot<T>(ctx, n_b, H.nl_w), P);
+}
+
+template <typename T>
+static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, double c, double e) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  const bool f32 = sizeof(T) == 4;
+  const cplx<T>* beta = f32 ? (const cplx<T>*)H.beta_f.p : (const cplx<T>*)H.beta;
+  const cplx<T>* B = f32 ? (const cplx<T>*)H.bmat_f.p : (const cplx<T>*)H.bmat.p;
+  const cplx<T>* M = f32 ? (const cplx<T>*)H.sinv_m_f.p : (const cplx<T>*)H.sinv_m.p;
+  cplx<T>* Wp = nl_slot<T>(ctx, n_b, H.nl_wp);
+  cplx<T>* W = nl_slot<T>(ctx, n_b, H.nl_w);
+  cplx<T>* G = nl_slot<T>(ctx, n_b, H.nl_g);
+  cplx<T>* Z = nl_slot<T>(ctx, n_b, 3);
+  cplx<T>* Cc = nl_slot<T>(ctx, n_b, 4);
+  const size_t pb = (size_t)P * n_b;
+  const double f = first ? 1.0 / e : 2.0 / e;
+  // G = beta^H O
+  CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P));
+  const T a1 = (T)(first ? e : 0.5 * e), a2 = (T)(first ? 0.0 : 0.5 * e);
+  hipLaunchKernelGGL((k_cheb_coef<T>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream, G, Wp,
+                     W, H.dion, P, pb, a1, a2, (T)c, H.has_s ? Z : nullptr, Cc);
+  CHS_LAUNCH_CHECK(ctx);
+  if (H.has_s) {
+    // Z += B (D W); C = D W - M Z
+    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, B, P, Cc, P, 1.0, Z, P));
+    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, -1.0, M, P, Z, P, 1.0, Cc, P));
+  }
+  // X_{k+1} = O + f beta C ; W_{k+1} = G + f B C (into G's slot)
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, f, beta, n_pw, Cc, P, 1.0, O, n_pw));
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, f, B, P, Cc, P, 1.0, G, P));
+  const int old_wp = H.nl_wp;
+  H.nl_wp = H.nl_w;
+  H.nl_w = H.nl_g;
+  H.nl_g = old_wp;
+  return CHS_OK;
+}
+
+int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b) {
+  if (precision == CHS_C128) return cheb_nl_begin_t<double>(ctx, (const z64*)X0, n_b);
+  CHS_TRY(ensure_c64_projectors(ctx));
+  return cheb_nl_begin_t<float>(ctx, (const c32*)X0, n_b);
+}
+
+int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first, double c,
+                 double e) {
+  if (precision == CHS_C128) return cheb_nl_step_t<double>(ctx, (z64*)O, n_b, first, c, e);
+  return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, c, e);
 }
 
 int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
@@ -340,6 +437,7 @@ int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, cons
   H.has_s = false;
   release(H.beta_f);
   release(H.sinv_m_f);
+  release(H.bmat_f);
   if (n_proj == 0) return CHS_OK;
   std::vector<double> q(n_proj);
   CHS_HIP(ctx, hipMemcpyAsync(q.data(), d_q, n_proj * sizeof(double), hipMemcpyDeviceToHost,
@@ -353,17 +451,23 @@ int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, cons
   }
   if (any && !all)
     return set_error(ctx, CHS_ERR_INVALID_ARG, "Q must be zero for all or no projectors");
-  if (!any) return CHS_OK;
-  // M = (Q^{-1} + beta^H beta)^{-1} through Cholesky
+  // B = beta^H beta (the filter's coefficient recurrences use it) and, with
+  // Q != 0, M = (Q^{-1} + B)^{-1} through Cholesky
   const int P = n_proj;
   const size_t pp = (size_t)P * P;
+  CHS_TRY(ensure(ctx, H.bmat, pp * sizeof(z64)));
+  CHS_TRY(zgemm(ctx, 'C', 'N', P, P, H.n_pw, mk(1.0, 0.0), (const z64*)d_beta, H.n_pw,
+                (const z64*)d_beta, H.n_pw, mk(0.0, 0.0), (z64*)H.bmat.p, P));
+  CHS_TRY(hermitise(ctx, (z64*)H.bmat.p, P));
+  if (!any) {
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return CHS_OK;
+  }
   CHS_TRY(ensure(ctx, H.sinv_m, pp * sizeof(z64)));
   CHS_TRY(ensure(ctx, ctx->impl->lin_ws, 2 * pp * sizeof(z64)));
   z64* A = (z64*)ctx->impl->lin_ws.p;
   z64* Li = A + pp;
-  CHS_TRY(zgemm(ctx, 'C', 'N', P, P, H.n_pw, mk(1.0, 0.0), (const z64*)d_beta, H.n_pw,
-                (const z64*)d_beta, H.n_pw, mk(0.0, 0.0), A, P));
-  CHS_TRY(hermitise(ctx, A, P));
+  CHS_HIP(ctx, hipMemcpyAsync(A, H.bmat.p, pp * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
   hipLaunchKernelGGL(k_diag_add_inv, dim3(ceil_div(P, 256)), dim3(256), 0, ctx->stream, A, d_q, P);
   CHS_LAUNCH_CHECK(ctx);
   int info = 0;
