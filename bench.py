@@ -36,59 +36,103 @@ HBM_PEAK_GBS = 8000.0
 CONFIGS = {
     "A": dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10),
     "B": dict(n_rep=2, n_grid=72, n_bands=160, cheb_degree=12),
+    "C": dict(n_rep=3, n_grid=108, n_bands=512, cheb_degree=12, n_proj_per_atom=4, q0=0.25),
 }
 WORKLOAD = {
     "A": "8-atom diamond-cubic cell, Gamma, Ecut 15 Ha (2,945 PW), 36^3, 20 bands, "
-         "Chebyshev degree 10, complex128",
+         "Chebyshev degree 10",
     "B": "64-atom diamond-cubic 2x2x2 cell, Gamma, Ecut 15 Ha (23,847 PW), 72^3, 160 bands, "
-         "Chebyshev degree 12, complex128 filter + FP64 Cholesky-QR/Rayleigh-Ritz",
+         "Chebyshev degree 12, FP64 Cholesky-QR/Rayleigh-Ritz",
+    "C": "216-atom diamond-cubic 3x3x3 USPP cell (4 projectors/atom, 864 total, Q=0.25 "
+         "Gaussian augmentation), Gamma, Ecut 15 Ha (80,725 PW), 108^3, 512 bands, "
+         "Chebyshev degree 12, FP64 Cholesky-QR/Rayleigh-Ritz",
 }
+PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
 
 
-def roofline(chs, ctx, cfg):
-    """Time the three H|psi> kernels at the workload's shape; the dominant one
-    (largest time) is reported against HBM peak."""
+def roofline(chs, ctx, cfg, precision):
+    """Per-launch times of one Chebyshev step's kernels at the workload's
+    shape, measured with HIP events on the context stream: the three H|psi>
+    kernels and, with projectors, the two n_pw-long projector GEMMs. The
+    dominant (longest) one is reported against its MI355X peak: HBM 8 TB/s
+    for the H|psi> kernels (algorithmic bytes of the stick/plane design),
+    dense matrix-core TFLOP/s for the GEMMs."""
     import numpy as np
     L = cfg["n_rep"] * 10.26
     n = cfg["n_grid"]
+    prec = chs.C128 if precision == "c128" else chs.C64
     mil, idx, kin = chs.gsphere(ctx, L, 15.0, n)
     rng = np.random.default_rng(0)
     veff = chs.Gpu.from_host(ctx, rng.standard_normal(n ** 3) * 0.1)
     chs.Hamiltonian(ctx, n, mil, kin, veff)
-    prof = chs.profile_hpsi(ctx, cfg["n_bands"], chs.C128, reps=20)
+    nb = cfg["n_bands"]
+    prof = chs.profile_hpsi(ctx, nb, prec, reps=10)
     names = ["k_stick_inv", "k_plane", "k_stick_fwd"]
-    k = int(np.argmax(prof["ms"]))
-    achieved = prof["bytes"][k] / (prof["ms"][k] * 1e-3) / 1e9
-    nb, s = cfg["n_bands"], 16
-    canonical = nb * (4 * s * n ** 3 + 3 * s * prof["n_pw"]) + 8 * n ** 3
-    step_ms = float(prof["ms"].sum())
+    items = [{"kernel": nm, "ms": float(ms), "bound": "hbm", "work": float(b)}
+             for nm, ms, b in zip(names, prof["ms"], prof["bytes"])]
+    P = 8 * cfg["n_rep"] ** 3 * cfg.get("n_proj_per_atom", 0)
+    npw = prof["n_pw"]
+    if P:
+        for nm, (oa, ob, m, nn, k) in (("gemm_betaH_O", ("C", "N", P, nb, npw)),
+                                        ("gemm_O_plus_betaC", ("N", "N", npw, nb, P))):
+            ms = chs.profile_gemm(ctx, prec, oa, ob, m, nn, k, reps=5)
+            items.append({"kernel": nm, "ms": ms, "bound": "mfma", "work": 8.0 * m * nn * k})
+    dom = max(items, key=lambda it: it["ms"])
+    if dom["bound"] == "hbm":
+        achieved, peak, unit = dom["work"] / (dom["ms"] * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved, peak, unit = dom["work"] / (dom["ms"] * 1e-3) / 1e12, PEAK_TFLOPS[precision], \
+            "TFLOP/s"
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{names[k]}_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{dom['kernel']}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            t = json.load(f)
+        if t.get("n_grid") == n and t.get("n_bands") == nb and t.get("precision") == precision:
+            traffic = t.get("bytes_per_launch")
+    s_b = 16 if precision == "c128" else 8
+    canonical = nb * (4 * s_b * n ** 3 + 3 * s_b * npw) + 8 * n ** 3
+    hpsi_ms = float(prof["ms"].sum())
     return {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-        "kernel": names[k], "kernel_ms": round(float(prof["ms"][k]), 4),
-        "algorithmic_bytes": float(prof["bytes"][k]),
-        "step_kernels_ms": [round(float(x), 4) for x in prof["ms"]],
-        "step_canonical_bytes": float(canonical),
-        "step_canonical_gbs": round(canonical / (step_ms * 1e-3) / 1e9, 1),
-        "sticks": prof["sticks"], "active_x": prof["active_x"],
+        "bound": dom["bound"], "achieved": round(achieved, 2), "peak": peak, "unit": unit,
+        "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": dom["kernel"],
+        "kernel_ms": round(dom["ms"], 4), "algorithmic_work_per_launch": dom["work"],
+        "step_kernels_ms": {it["kernel"]: round(it["ms"], 4) for it in items},
+        "hpsi_canonical_bytes": float(canonical),
+        "hpsi_canonical_gbs": round(canonical / (hpsi_ms * 1e-3) / 1e9, 1),
+        "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision,
     }
 
 
-def cpu_baseline(cfg, n_iter_gpu):
-    """FP64 oracle on this host: one SCF iteration (Lanczos k=10, one filter +
-    Rayleigh-Ritz pass, density, V_Hxc), extrapolated to the GPU's iteration
-    count (+3 extra first-iteration passes)."""
+def cpu_baseline(cfg, n_iter_gpu, n_pass_gpu):
+    """FP64 oracle on this host. Small cells: one SCF iteration (Lanczos k=10,
+    one filter + Rayleigh-Ritz pass, density, V_Hxc) timed and extrapolated to
+    the GPU run's iteration count (+3 first-iteration passes). Cells whose
+    iteration would take minutes: the oracle's H|psi> on a sample of 4 bands,
+    scaled to the H applications of the GPU run (n_bands x (degree + 1) per
+    filter + Rayleigh-Ritz pass), a lower bound on the CPU time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle as O
-    p = O.default_params(**cfg)
+    ocfg = {k: v for k, v in cfg.items()}
+    p = O.default_params(**ocfg)
     pos = O.positions(cfg["n_rep"])
     m = O.Model(p, pos)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if cfg["n_grid"] > 72:
+        psi = m.initial_psi(4)
+        m.apply_h(psi)  # first touch
+        t0 = time.perf_counter()
+        m.apply_h(psi, want_s=cfg.get("n_proj_per_atom", 0) > 0)
+        per_band = (time.perf_counter() - t0) / 4
+        applies = n_pass_gpu * cfg["n_bands"] * (cfg["cheb_degree"] + 1)
+        return {
+            "value": round(per_band * applies, 1), "unit": "s", "cores": cores, "kind": "port",
+            "sample": f"oracle H|psi> (+S) on 4 bands of the same cell: {per_band * 1e3:.1f} ms "
+                      f"per band, x {applies} band applications of the GPU run "
+                      f"({n_pass_gpu} filter + Rayleigh-Ritz passes); dense algebra, density "
+                      f"and mixing not counted (lower bound)",
+        }
     n3 = m.n3
     omega = (cfg["n_rep"] * 10.26) ** 3
     rho = np.full(n3, 4.0 * 8 * cfg["n_rep"] ** 3 / omega)
@@ -108,7 +152,6 @@ def cpu_baseline(cfg, n_iter_gpu):
     t_pass = t2 - t1
     t_iter = (t1 - t0) + t_pass + (t3 - t2)
     total = n_iter_gpu * t_iter + 3 * t_pass
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
         "value": round(total, 3), "unit": "s", "cores": cores, "kind": "port",
         "sample": f"one SCF iteration of the same workload with the FP64 oracle "
@@ -118,29 +161,55 @@ def cpu_baseline(cfg, n_iter_gpu):
     }
 
 
+def run_form(chs, ctx, cfg, precision, steps, warmup):
+    """Warm-up and timed SCF runs of one precision form; returns the runs."""
+    pos = chs.synth_positions(cfg["n_rep"])
+    prec = chs.C128 if precision == "c128" else chs.C64
+    params = chs.scf_params(**cfg, max_iter=100, precision=prec)
+    for _ in range(warmup):
+        r = chs.run_scf(ctx, params, pos)
+        print(f"[{precision}] warmup: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
+              f"converged={r['converged']}", file=sys.stderr, flush=True)
+    runs = []
+    for k in range(steps):
+        r = chs.run_scf(ctx, params, pos)
+        runs.append(r)
+        print(f"[{precision}] step {k}: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
+              f"E = {r['energy']:.10f}", file=sys.stderr, flush=True)
+    if not all(r["converged"] for r in runs):
+        print(f"bench.py: SCF ({precision}) did not converge", file=sys.stderr)
+        sys.exit(1)
+    return runs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"],
                     help="filter precision: c128 throughout, or complex64 filter with FP64 "
                          "Rayleigh-Ritz and the last SCF iterations in c128")
+    ap.add_argument("--compare", action="store_true",
+                    help="time both forms; value = the complex64-filter form, with the plain "
+                         "FP64 form's time and the ratio beside it")
     ap.add_argument("--hpsi-only", action="store_true",
-                    help="only time the H|psi> kernels (for counter runs) and print the roofline")
+                    help="only time one Chebyshev step's kernels (for counter runs) and print "
+                         "the roofline")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="with --hpsi-only: also run 4 device-to-device copies of this many "
                          "bytes (the counter scale reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path would shard by k-points/bands, see DESIGN.md)", file=sys.stderr)
+              "(the path would shard by k-points and bands)", file=sys.stderr)
         sys.exit(2)
     import chs
     ctx = chs.Context(0)
     cfg = CONFIGS[args.config]
+    precision = "c64" if args.compare else args.precision
     if args.hpsi_only:
         if args.copy_bytes:
             import numpy as np
@@ -149,46 +218,43 @@ def main():
             for _ in range(4):
                 ctx.check(chs.lib().chs_d2d(ctx.h, b.ptr, a.ptr, args.copy_bytes))
             ctx.synchronize()
-        print(json.dumps(roofline(chs, ctx, cfg)))
+        print(json.dumps(roofline(chs, ctx, cfg, precision)))
         return
-    pos = chs.synth_positions(cfg["n_rep"])
-    prec = chs.C128 if args.precision == "c128" else chs.C64
-    params = chs.scf_params(**cfg, max_iter=100, precision=prec)
-    for _ in range(args.warmup):
-        r = chs.run_scf(ctx, params, pos)
-        print(f"warmup: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
-              f"converged={r['converged']}", file=sys.stderr, flush=True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    runs = []
-    for k in range(args.steps):
-        r = chs.run_scf(ctx, params, pos)
-        runs.append(r)
-        print(f"step {k}: {r['n_iter']} iterations, {r['seconds']:.3f} s, E = {r['energy']:.10f}",
-              file=sys.stderr, flush=True)
+    runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup)
     ctx.synchronize()
     t1 = time.perf_counter()
-    if not all(r["converged"] for r in runs):
-        print("bench.py: SCF did not converge", file=sys.stderr)
-        sys.exit(1)
     value = sum(r["seconds"] for r in runs) / len(runs)
-    rl = roofline(chs, ctx, cfg)
     last = runs[-1]
+    n_atoms = 8 * cfg["n_rep"] ** 3
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round((t1 - t0) / args.steps * 1e3, 2),
+        "ms_per_step": round((t1 - t0) / max(1, args.steps + args.warmup) * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": WORKLOAD[args.config], "atoms": 8 * cfg["n_rep"] ** 3,
+        "dtype": precision, "data": "synthetic",
+        "config": {"workload": WORKLOAD[args.config], "atoms": n_atoms,
                    "n_pw": last["n_pw"], "grid": cfg["n_grid"], "bands": cfg["n_bands"],
+                   "projectors": last["n_proj"], "filter_precision": precision,
                    "scf_iterations": last["n_iter"], "energy_Ha": last["energy"],
                    "phase_s": {k: round(last[k], 4) for k in
                                ("t_filter", "t_rr", "t_density", "t_veff", "t_lanczos")}},
-        "roofline": rl,
     }
+    if args.compare:
+        ref = run_form(chs, ctx, cfg, "c128", args.steps, 0)
+        v64 = sum(r["seconds"] for r in ref) / len(ref)
+        line["config"]["fp64_form"] = {
+            "seconds": round(v64, 4), "scf_iterations": ref[-1]["n_iter"],
+            "energy_Ha": ref[-1]["energy"],
+            "phase_s": {k: round(ref[-1][k], 4) for k in
+                        ("t_filter", "t_rr", "t_density", "t_veff", "t_lanczos")}}
+        line["config"]["speedup_vs_fp64"] = round(v64 / value, 3)
+        line["config"]["energy_diff_Ha_per_atom"] = abs(ref[-1]["energy"] - last["energy"]) / n_atoms
+    line["roofline"] = roofline(chs, ctx, cfg, precision)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(cfg, last["n_iter"])
+        n_pass = last["n_iter"] + 3
+        line["cpu_baseline"] = cpu_baseline(cfg, last["n_iter"], n_pass)
     print(json.dumps(line))
 
 

@@ -126,6 +126,9 @@ def lib():
             "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
             "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
             "chs_band_cg": [vp, vp, i, i, d, i, vp, dp, vp, vp],
+            "chs_cgemm": [vp, C.c_char, C.c_char, i, i, i, C.POINTER(C.c_float), vp, i, vp, i,
+                          C.POINTER(C.c_float), vp, i],
+            "chs_profile_gemm": [vp, i, C.c_char, C.c_char, i, i, i, i, dp],
             "chs_build_system": [vp, C.POINTER(ScfParams), dp, C.POINTER(i), C.POINTER(i), vp, vp,
                                  vp, vp, vp, vp],
             "chs_mlp_create": [vp, u64, C.POINTER(vp)],
@@ -454,6 +457,14 @@ def run_scf(ctx, params, pos, want_rho=False):
     out["rho"] = rho
     out["status"] = rc
     return out
+
+
+def profile_gemm(ctx, precision, opa, opb, m, n, k, reps=5):
+    """Mean ms of a complex GEMM of the given shape (HIP events, context stream)."""
+    ms = C.c_double()
+    ctx.check(lib().chs_profile_gemm(ctx.h, precision, opa.encode(), opb.encode(), m, n, k, reps,
+                                     C.byref(ms)))
+    return ms.value
 
 
 def build_system(ctx, params, pos):

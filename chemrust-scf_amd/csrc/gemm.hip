@@ -8,6 +8,7 @@
 // across blocks; each slice writes its partial tile and k_gemm_reduce sums
 // the slices in a fixed order (deterministic).
 #include <algorithm>
+#include <utility>
 
 #include "impl.hpp"
 #include "linalg.hpp"
@@ -286,4 +287,68 @@ extern "C" int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int
     return set_error(ctx, CHS_ERR_INVALID_ARG, "zgemm: leading dimension too small");
   return zgemm(ctx, opa, opb, m, n, k, mk(alpha[0], alpha[1]), (const z64*)d_a, lda,
                (const z64*)d_b, ldb, mk(beta[0], beta[1]), (z64*)d_c, ldc);
+}
+
+// Synthetic operands for timing (values in [-0.005, 0.005)).
+__global__ void k_gemm_fill(float* __restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    a[i] = ((float)(h & 0xFFFF) / 65536.0f - 0.5f) * 1e-2f;
+  }
+}
+
+extern "C" int chs_cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k,
+                         const float alpha[2], const float* d_a, int lda, const float* d_b, int ldb,
+                         const float beta[2], float* d_c, int ldc) {
+  if (!ctx || !alpha || !beta || (m > 0 && n > 0 && (!d_c || (k > 0 && (!d_a || !d_b)))))
+    return CHS_ERR_INVALID_ARG;
+  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (ldc < std::max(1, m) || lda < std::max(1, oa == 0 ? m : k) ||
+      ldb < std::max(1, ob == 0 ? k : n))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "cgemm: leading dimension too small");
+  return cgemm(ctx, opa, opb, m, n, k, mk(alpha[0], alpha[1]), (const c32*)d_a, lda,
+               (const c32*)d_b, ldb, mk(beta[0], beta[1]), (c32*)d_c, ldc);
+}
+
+// Mean duration (HIP events on the context stream) of `reps` complex GEMMs
+// C = A B (op(A) m x k, op(B) k x n) on synthetic operands.
+extern "C" int chs_profile_gemm(chs_context* ctx, int precision, char opa, char opb, int m, int n,
+                                int k, int reps, double* ms) {
+  if (!ctx || !ms || m <= 0 || n <= 0 || k <= 0 || reps <= 0 ||
+      (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
+  if (oa < 0 || ob < 0) return CHS_ERR_INVALID_ARG;
+  auto* im = ctx->impl;
+  const size_t cs = precision == CHS_C128 ? 16 : 8;
+  const size_t na = (size_t)m * k, nb = (size_t)k * n, nc = (size_t)m * n;
+  CHS_TRY(ensure(ctx, im->ws1, na * cs));
+  CHS_TRY(ensure(ctx, im->ws2, nb * cs));
+  CHS_TRY(ensure(ctx, im->ws3, nc * cs));
+  for (auto pr : {std::make_pair(im->ws1.p, na), std::make_pair(im->ws2.p, nb)}) {
+    const size_t nf = pr.second * cs / sizeof(float);
+    hipLaunchKernelGGL(k_gemm_fill, dim3(ceil_div(nf, 256)), dim3(256), 0, ctx->stream,
+                       (float*)pr.first, nf);
+  }
+  CHS_LAUNCH_CHECK(ctx);
+  const int lda = oa == 0 ? m : k, ldb = ob == 0 ? k : n;
+  auto run = [&]() -> int {
+    if (precision == CHS_C128)
+      return zgemm(ctx, opa, opb, m, n, k, mk(1.0, 0.0), (const z64*)im->ws1.p, lda,
+                   (const z64*)im->ws2.p, ldb, mk(0.0, 0.0), (z64*)im->ws3.p, m);
+    return cgemm(ctx, opa, opb, m, n, k, mk(1.0f, 0.0f), (const c32*)im->ws1.p, lda,
+                 (const c32*)im->ws2.p, ldb, mk(0.0f, 0.0f), (c32*)im->ws3.p, m);
+  };
+  CHS_TRY(run());  // warm-up (workspace growth happens here)
+  CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+  for (int r = 0; r < reps; ++r) CHS_TRY(run());
+  CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+  CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+  float t = 0;
+  CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+  *ms = t / reps;
+  return CHS_OK;
 }

@@ -189,6 +189,11 @@ int chs_band_cg(chs_context* ctx, double* d_psi, int n_b, int max_steps, double 
 int chs_zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const double alpha[2],
               const double* d_a, int lda, const double* d_b, int ldb, const double beta[2],
               double* d_c, int ldc);
+/* complex64 counterpart (FP32 storage and accumulation): the projector
+ * products of the complex64 filter */
+int chs_cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const float alpha[2],
+              const float* d_a, int lda, const float* d_b, int ldb, const float beta[2],
+              float* d_c, int ldc);
 int chs_zdotc(chs_context* ctx, int n, const double* d_x, const double* d_y, double out[2]);
 int chs_zaxpy(chs_context* ctx, int n, const double alpha[2], const double* d_x, double* d_y);
 int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
@@ -203,6 +208,11 @@ int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
  * dims4: {n_grid, n_pw, sticks, active x columns}. */
 int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,
                      double* bytes3, int32_t* dims4);
+/* Mean milliseconds (HIP events on the context stream) of `reps` complex
+ * GEMMs C = op(A) op(B) with op(A) m x k, op(B) k x n, in CHS_C128 or
+ * CHS_C64, on synthetic operands in context workspaces. */
+int chs_profile_gemm(chs_context* ctx, int precision, char opa, char opb, int m, int n, int k,
+                     int reps, double* ms);
 
 /* ---- SCF -------------------------------------------------------------- */
 /* Density from band-major c128 wavefunctions with occupations (host f64,

@@ -81,3 +81,34 @@ def test_scf_uspp_config_a_vs_oracle(ctx):
     assert np.abs(got["rho"] - ref["rho"]).sum() * dv < 1e-6
     # augmented density integrates to N_e
     assert abs(got["rho"].sum() * dv - 32.0) < 1e-8
+
+
+def test_filter_fp32_projectors_vs_oracle(ctx):
+    """complex64 filter with projectors (coefficient-space recurrences in FP32)
+    against the FP64 oracle filter: FP32 accuracy of the degree-8 iterate."""
+    m = model()
+    H = bind(ctx, m)
+    psi = m.initial_psi(20)
+    oup, _ = m.lanczos_bound(10, 4)
+    b4 = O.spectral_bounds(None, oup)
+    rc, ref, _ = m.chebyshev_filter(psi, 8, b4)
+    got = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi.astype(np.complex64)), 8, b4)
+    g = got.sync_to_host().astype(np.complex128)
+    assert np.abs(g - ref).max() / np.abs(ref).max() < 2e-5
+
+
+def test_filter_projectors_without_s(ctx):
+    """Projectors with Q = 0 (S = 1): the coefficient recurrence without the
+    Woodbury term, against the oracle."""
+    p = O.default_params(n_rep=1, n_grid=36, n_bands=16, n_proj_per_atom=4, q0=0.0)
+    m = O.Model(p, O.positions(1))
+    mil, idx, kin = m.pw()
+    H = chs.Hamiltonian(ctx, 36, mil, kin, m.vion())
+    H.bind_projectors(m.beta(), m.d(), np.zeros(m.n_proj))
+    psi = m.initial_psi(16)
+    oup, _ = m.lanczos_bound(10, 4)
+    b4 = O.spectral_bounds(None, oup)
+    rc, ref, _ = m.chebyshev_filter(psi, 9, b4)
+    assert rc == 0
+    got = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 9, b4).sync_to_host()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-10
