@@ -129,6 +129,11 @@ def lib():
             "chs_cgemm": [vp, C.c_char, C.c_char, i, i, i, C.POINTER(C.c_float), vp, i, vp, i,
                           C.POINTER(C.c_float), vp, i],
             "chs_profile_gemm": [vp, i, C.c_char, C.c_char, i, i, i, i, dp],
+            "chs_kset_bind": [vp, i, i, vp, vp, vp, vp, i, vp, vp, vp],
+            "chs_kset_set_veff": [vp, vp],
+            "chs_kset_select": [vp, i],
+            "chs_kset_filter": [vp, i, vp, i, i, dp],
+            "chs_autotune_hpsi": [vp, i, i, C.POINTER(i), C.POINTER(i), dp],
             "chs_build_system": [vp, C.POINTER(ScfParams), dp, C.POINTER(i), C.POINTER(i), vp, vp,
                                  vp, vp, vp, vp],
             "chs_mlp_create": [vp, u64, C.POINTER(vp)],
@@ -564,6 +569,47 @@ def mlp_pairs(ctx, psi, psi_ref, eps_ref, pairs, gap_min=1e-3):
                                   C.byref(used)))
     pairs["n"] = o + n_b * n_pw
     return used.value
+
+
+class KSet:
+    """k-point set of one cell on a context (chs_kset_*): per k a plane-wave
+    set (miller, kin device arrays), optional projectors beta[k]; shared
+    V_eff, D and Q."""
+
+    def __init__(self, ctx, n_grid, millers, kins, veff, betas=None, d=None, q=None):
+        self.ctx = ctx
+        self.millers, self.kins, self.veff = list(millers), list(kins), veff
+        self.betas, self.d, self.q = (list(betas) if betas is not None else None), d, q
+        K = len(self.kins)
+        npw = np.array([k.shape[0] for k in self.kins], np.int32)
+        P = self.betas[0].shape[0] if self.betas else 0
+        arr = lambda xs: (vp * len(xs))(*[x.ptr for x in xs])
+        self.n_pw = npw
+        ctx.check(lib().chs_kset_bind(ctx.h, K, n_grid, npw.ctypes.data_as(vp), arr(self.millers),
+                                      arr(self.kins), veff.ptr, P,
+                                      arr(self.betas) if P else None,
+                                      d.ptr if P else None, q.ptr if P else None))
+        ctx._bound_kset = self
+
+    def filter(self, psis, degree, bounds, precision=None):
+        """In-place filter of every k-point's device block; bounds (K, 4)."""
+        b = np.ascontiguousarray(bounds, np.float64).reshape(-1)
+        prec = precision_of(psis[0].dtype) if precision is None else precision
+        ptrs = (vp * len(psis))(*[p.ptr for p in psis])
+        self.ctx.check(lib().chs_kset_filter(self.ctx.h, prec, ptrs, psis[0].shape[0], degree,
+                                             _dptr(b)))
+
+    def select(self, k):
+        self.ctx.check(lib().chs_kset_select(self.ctx.h, k))
+
+
+def autotune_hpsi(ctx, n_b, precision=C128):
+    """Tune the H|psi> launch shape for the bound Hamiltonian; returns
+    (sticks per block, row-buffer divisor, step ms)."""
+    sb, rd, ms = C.c_int(), C.c_int(), C.c_double()
+    ctx.check(lib().chs_autotune_hpsi(ctx.h, precision, n_b, C.byref(sb), C.byref(rd),
+                                      C.byref(ms)))
+    return sb.value, rd.value, ms.value
 
 
 def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None):

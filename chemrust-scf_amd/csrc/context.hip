@@ -29,6 +29,7 @@ int ensure(chs_context* ctx, DevBuf& b, size_t bytes) {
   size_t want = bytes < 256 ? 256 : bytes;
   CHS_HIP(ctx, hipMalloc(&b.p, want));
   b.bytes = want;
+  ctx->impl->ws_gen += 1;
   return CHS_OK;
 }
 
@@ -107,10 +108,11 @@ int chs_context_destroy(chs_context* ctx) {
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
                     &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
-                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws,
-                    &im->ham.st_mem, &im->ham.veff_f, &im->ham.beta_f, &im->ham.sinv_m,
-                    &im->ham.sinv_m_f, &im->ham.bmat, &im->ham.bmat_f};
+                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->kset_ws};
   for (DevBuf* b : bufs) release(*b);
+  release_hamiltonian(im->ham);
+  for (Hamiltonian& h : im->kset) release_hamiltonian(h);
+  if (im->kset_graph) (void)hipGraphExecDestroy(im->kset_graph);
   (void)hipEventDestroy(im->ev0);
   (void)hipEventDestroy(im->ev1);
   (void)hipStreamDestroy(ctx->stream);

@@ -30,7 +30,9 @@
 
 namespace chs {
 
-constexpr int kSB = 32;  // sticks per block in the z passes
+// Sticks per block of the z passes (SB) and the plane kernel's row-buffer
+// count are tuned per (grid, precision) by vloc_autotune; defaults SB = 32,
+// all row buffers that fit.
 
 struct StickArgs {
   int n_st, n_act, n_pw, rs;  // rs: row stride of the compact plane (= 1 mod 8)
@@ -45,16 +47,16 @@ struct StickArgs {
 template <typename T> __device__ __forceinline__ cplx<T> zero_c() { return mk((T)0, (T)0); }
 
 // --- A: scatter + inverse z-FFT -------------------------------------------
-template <int N, typename T>
+template <int N, typename T, int SB>
 __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ psi, StickArgs st,
                                                    cplx<T>* __restrict__ sbuf,
                                                    const cplx<T>* __restrict__ tw) {
   constexpr int LS = N + 1;
-  __shared__ cplx<T> lds[kSB * LS];
-  const int s0 = blockIdx.x * kSB;
+  __shared__ cplx<T> lds[SB * LS];
+  const int s0 = blockIdx.x * SB;
   const int b = blockIdx.y;
-  const int ns = min(kSB, st.n_st - s0);
-  for (int i = threadIdx.x; i < kSB * LS; i += blockDim.x) lds[i] = zero_c<T>();
+  const int ns = min(SB, st.n_st - s0);
+  for (int i = threadIdx.x; i < SB * LS; i += blockDim.x) lds[i] = zero_c<T>();
   __syncthreads();
   const cplx<T>* pb = psi + (size_t)b * st.n_pw;
   for (int t = threadIdx.x; t < ns * N; t += blockDim.x) {
@@ -73,8 +75,8 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
     for (int sl = g; sl < ns; sl += ng) group_fft_line<N, 1>(lds + sl * LS, 1, lane, tw, PostId());
   __syncthreads();
   cplx<T>* out = sbuf + (size_t)b * N * st.n_st + s0;
-  for (int t = threadIdx.x; t < N * kSB; t += blockDim.x) {
-    const int z = t / kSB, sl = t - z * kSB;
+  for (int t = threadIdx.x; t < N * SB; t += blockDim.x) {
+    const int z = t / SB, sl = t - z * SB;
     if (sl < ns) out[(size_t)z * st.n_st + sl] = lds[sl * LS + z];
   }
 }
@@ -150,22 +152,29 @@ __global__ void __launch_bounds__(256) k_plane(cplx<T>* __restrict__ sbuf, Stick
 }
 
 // --- C: forward z-FFT + gather + kinetic + Chebyshev epilogue --------------
-template <int N, typename T, int MODE>
+// c and inv_e come from `coef` (device, {c, inv_e}) when it is given, so a
+// captured graph of the filter replays with new spectral bounds.
+template <int N, typename T, int MODE, int SB>
 __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ sbuf, StickArgs st,
                                                    const cplx<T>* __restrict__ psi,
                                                    const double* __restrict__ kin,
                                                    cplx<T>* out, const cplx<T>* aux0, T c, T inv_e,
-                                                   T inv_n3, double* __restrict__ part,
+                                                   const double* __restrict__ coef, T inv_n3,
+                                                   double* __restrict__ part,
                                                    const cplx<T>* __restrict__ tw) {
   constexpr int LS = N + 1;
-  __shared__ cplx<T> lds[kSB * LS];
+  __shared__ cplx<T> lds[SB * LS];
   __shared__ double red[2][4];
-  const int s0 = blockIdx.x * kSB;
+  if (coef) {
+    c = (T)coef[0];
+    inv_e = (T)coef[1];
+  }
+  const int s0 = blockIdx.x * SB;
   const int b = blockIdx.y;
-  const int ns = min(kSB, st.n_st - s0);
+  const int ns = min(SB, st.n_st - s0);
   const cplx<T>* in = sbuf + (size_t)b * N * st.n_st + s0;
-  for (int t = threadIdx.x; t < N * kSB; t += blockDim.x) {
-    const int z = t / kSB, sl = t - z * kSB;
+  for (int t = threadIdx.x; t < N * SB; t += blockDim.x) {
+    const int z = t / SB, sl = t - z * SB;
     if (sl < ns) lds[sl * LS + z] = in[(size_t)z * st.n_st + sl];
   }
   __syncthreads();
@@ -378,7 +387,7 @@ struct PlaneLaunch {
   int nrb = 0;
 };
 static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz, int n2,
-                        PlaneLaunch* out) {
+                        int nrb_div, PlaneLaunch* out) {
   const size_t cmp_bytes = (size_t)n * sa.rs * csz;
   const size_t row_bytes = (size_t)(n + 1) * csz;
   const int lim = max_lds_per_block(ctx);
@@ -387,16 +396,54 @@ static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz
                      n, cmp_bytes + row_bytes, lim);
   const int groups = 4 * (64 / n2);  // 256 threads
   int nrb = (int)((lim - cmp_bytes) / row_bytes);
-  nrb = std::min(nrb, groups);
+  nrb = std::max(1, std::min(nrb, groups) / std::max(1, nrb_div));
   out->nrb = nrb;
   out->lds = cmp_bytes + (size_t)nrb * row_bytes;
   return CHS_OK;
 }
 
+// Tuned launch shape of the pipeline for (grid, precision).
+static VlocTune get_tune(chs_context* ctx, int n, int precision) {
+  auto it = ctx->impl->vloc_tune.find(std::make_pair(n, precision));
+  return it == ctx->impl->vloc_tune.end() ? VlocTune{} : it->second;
+}
+
+template <int NN, typename T, int SB>
+static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch& pl, int n_b,
+                       const cplx<T>* psi, cplx<T>* sbuf, const T* veff, const cplx<T>* tw,
+                       const double* kin, cplx<T>* out, const cplx<T>* aux0, int mode, double c,
+                       double inv_e, const double* coef, double inv_n3, double* part,
+                       hipEvent_t* ev) {
+  const int gz = ceil_div(sa.n_st, SB);
+  if (ev) CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+  hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
+                     sbuf, tw);
+  if (ev) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+  hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf, sa,
+                     veff, pl.nrb, tw);
+  if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+#define CHS_FWD(M)                                                                            \
+  hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \
+                     sbuf, sa, psi, kin, out, aux0, (T)c, (T)inv_e, coef, (T)inv_n3, part, tw)
+  if (mode == EPI_HPSI)
+    CHS_FWD(EPI_HPSI);
+  else if (mode == EPI_CHEB1)
+    CHS_FWD(EPI_CHEB1);
+  else
+    CHS_FWD(EPI_CHEBK);
+#undef CHS_FWD
+  if (ev) CHS_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+// Sticks per block that fit the static LDS of the z passes for (N, T).
+static bool sb_fits(int n, size_t csz, int sb) { return (size_t)sb * (n + 1) * csz <= 60000; }
+
 template <typename T>
 static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
-                           int mode, const void* aux0_v, double c, double inv_e, double* part,
-                           int* n_part, int reps, double* ms) {
+                           int mode, const void* aux0_v, double c, double inv_e,
+                           const double* coef, double* part, int* n_part, int reps, double* ms) {
   Hamiltonian& H = ctx->impl->ham;
   const StickTable& st = H.st;
   const int n = H.n_grid;
@@ -412,8 +459,9 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   C* out = (C*)out_v;
   const C* aux0 = (const C*)aux0_v;
   const double inv_n3 = 1.0 / ((double)n * n * n);
-  const int gz = ceil_div(st.n_st, kSB);
-  if (n_part) *n_part = gz * n_b;
+  VlocTune tn = get_tune(ctx, n, precision);
+  if (!sb_fits(n, sizeof(C), tn.sb)) tn.sb = 16;
+  if (n_part) *n_part = (int)ceil_div(st.n_st, tn.sb) * n_b;
   hipEvent_t ev[4];
   if (ms)
     for (auto& e : ev) CHS_HIP(ctx, hipEventCreate(&e));
@@ -421,28 +469,17 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   for (int r = 0; r < std::max(1, reps); ++r) {
     CHS_DISPATCH_N(n, {
       PlaneLaunch pl;
-      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, &pl));
-      if (ms) CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-      hipLaunchKernelGGL((k_stick_inv<NN, T>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
-                         sbuf, tw);
-      if (ms) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-      hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf,
-                         sa, veff, pl.nrb, tw);
-      if (ms) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
-      if (mode == EPI_HPSI)
-        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_HPSI>), dim3(gz, n_b), dim3(256), 0,
-                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
-                           (T)inv_n3, part, tw);
-      else if (mode == EPI_CHEB1)
-        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEB1>), dim3(gz, n_b), dim3(256), 0,
-                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
-                           (T)inv_n3, part, tw);
+      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, &pl));
+      hipEvent_t* evp = ms ? ev : nullptr;
+      if (tn.sb == 16)
+        CHS_TRY((vloc_launch<NN, T, 16>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
+                                        mode, c, inv_e, coef, inv_n3, part, evp)));
+      else if (tn.sb == 64)
+        CHS_TRY((vloc_launch<NN, T, 64>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
+                                        mode, c, inv_e, coef, inv_n3, part, evp)));
       else
-        hipLaunchKernelGGL((k_stick_fwd<NN, T, EPI_CHEBK>), dim3(gz, n_b), dim3(256), 0,
-                           ctx->stream, sbuf, sa, psi, H.kin, out, aux0, (T)c, (T)inv_e,
-                           (T)inv_n3, part, tw);
-      if (ms) CHS_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-      CHS_LAUNCH_CHECK(ctx);
+        CHS_TRY((vloc_launch<NN, T, 32>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
+                                        mode, c, inv_e, coef, inv_n3, part, evp)));
     });
     if (ms) {
       CHS_HIP(ctx, hipEventSynchronize(ev[3]));
@@ -461,13 +498,13 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
 }
 
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
-                  const void* aux0, const void* /*aux1*/, double c, double inv_e,
+                  const void* aux0, const double* d_coef, double c, double inv_e,
                   double* d_norm_part, int* n_part) {
   if (precision == CHS_C128)
-    return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e,
+    return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_coef,
                                    d_norm_part, n_part, 1, nullptr);
-  return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_norm_part,
-                                n_part, 1, nullptr);
+  return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_coef,
+                                d_norm_part, n_part, 1, nullptr);
 }
 
 // The three kernels of one steady-state Chebyshev step `reps` times with HIP
@@ -476,10 +513,42 @@ int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, voi
 int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
                double* ms) {
   if (precision == CHS_C128)
-    return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5,
-                                   0.1, nullptr, nullptr, reps, ms);
-  return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5, 0.1,
+    return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5, 0.1,
+                                   nullptr, nullptr, nullptr, reps, ms);
+  return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, EPI_CHEBK, out, 0.5, 0.1, nullptr,
                                 nullptr, nullptr, reps, ms);
+}
+
+// Launch-shape autotuner of the H|psi> pipeline for the bound grid: every
+// (sticks per block, row-buffer divisor) candidate that fits is timed on
+// n_b bands at the given precision (HIP events, `reps` runs after one
+// warm-up) and the fastest is kept for (grid, precision).
+int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
+                  VlocTune* best_out, double* best_ms) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int n = H.n_grid;
+  const size_t csz = precision == CHS_C128 ? 16 : 8;
+  auto key = std::make_pair(n, precision);
+  VlocTune best{};
+  double best_t = 1e300;
+  for (int sb : {16, 32, 64}) {
+    if (!sb_fits(n, csz, sb)) continue;
+    for (int div : {1, 2}) {
+      ctx->impl->vloc_tune[key] = VlocTune{sb, div};
+      double ms[3];
+      CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, 1, ms));
+      CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, reps, ms));
+      const double t = ms[0] + ms[1] + ms[2];
+      if (t < best_t) {
+        best_t = t;
+        best = VlocTune{sb, div};
+      }
+    }
+  }
+  ctx->impl->vloc_tune[key] = best;
+  if (best_out) *best_out = best;
+  if (best_ms) *best_ms = best_t;
+  return CHS_OK;
 }
 
 int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
@@ -511,12 +580,13 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));
   const z64* tw = (const z64*)twv;
   StickArgs sa = stick_args(st);
-  const int gz = ceil_div(st.n_st, kSB);
+  constexpr int SB = 32;
+  const int gz = ceil_div(st.n_st, SB);
   CHS_DISPATCH_N(n, {
     PlaneLaunch pl;
-    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, &pl));
-    hipLaunchKernelGGL((k_stick_inv<NN, double>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi, sa,
-                       sbuf, tw);
+    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, &pl));
+    hipLaunchKernelGGL((k_stick_inv<NN, double, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
+                       sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL((k_density_plane<NN>), dim3(NN, n_chunks), dim3(256), pl.lds, ctx->stream,
                        sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
@@ -560,6 +630,16 @@ extern "C" {
 int chs_bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_miller,
                          const double* d_kin, const double* d_veff) {
   if (!ctx || n_pw <= 0 || !d_miller || !d_kin || !d_veff) return CHS_ERR_INVALID_ARG;
+  kset_swap(ctx, -1);  // a selected k-point goes back to its set
+  return bind_hamiltonian(ctx, n_grid, n_pw, d_miller, d_kin, d_veff);
+}
+
+}  // extern "C"
+
+namespace chs {
+
+int bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_miller,
+                     const double* d_kin, const double* d_veff) {
   if (!fft_length_supported(n_grid))
     return set_error(ctx, CHS_ERR_FFT, "unsupported FFT length %d", n_grid);
   Hamiltonian& H = ctx->impl->ham;
@@ -576,6 +656,10 @@ int chs_bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* 
   H.bound = true;
   return CHS_OK;
 }
+
+}  // namespace chs
+
+extern "C" {
 
 int chs_set_veff(chs_context* ctx, const double* d_veff) {
   if (!ctx || !d_veff) return CHS_ERR_INVALID_ARG;

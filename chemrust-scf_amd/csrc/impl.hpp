@@ -1,6 +1,7 @@
 // Context internals shared by the translation units of libchs_gpu.
 #pragma once
 #include <map>
+#include <vector>
 #include <utility>
 
 #include "common.hpp"
@@ -47,15 +48,32 @@ struct Hamiltonian {
   DevBuf sinv_m_f;               // c64 copy
   DevBuf bmat;                   // c128 P x P beta^H beta
   DevBuf bmat_f;                 // c64 copy
-  // projector coefficients carried through a Chebyshev filter: slots of
-  // W_{k-1}, W_k and the scratch G = beta^H O in the filter workspace
+  // projector coefficients carried through a Chebyshev filter (nl_ws, five
+  // P x n_b slots): slots of W_{k-1}, W_k and G = beta^H O; 3 and 4 scratch
+  DevBuf nl_ws;
   int nl_wp = 0, nl_w = 1, nl_g = 2;
+  DevBuf fscr;                   // second Chebyshev buffer of a k-point set member
+};
+
+inline void release_hamiltonian(Hamiltonian& h) {
+  for (DevBuf* b : {&h.veff_f, &h.st_mem, &h.beta_f, &h.sinv_m, &h.sinv_m_f, &h.bmat, &h.bmat_f,
+                    &h.nl_ws, &h.fscr})
+    release(*b);
+  h = Hamiltonian{};
+}
+
+// Launch shape of the H|psi> pipeline: sticks per block of the z passes and
+// the divisor of the plane kernel's row-buffer count.
+struct VlocTune {
+  int sb = 32;
+  int nrb_div = 1;
 };
 
 }  // namespace chs
 
 struct chs_context::Impl {
   std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
+  std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
   chs::Hamiltonian ham;
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
@@ -68,6 +86,18 @@ struct chs_context::Impl {
   // Complex GEMMs on the matrix cores (MFMA); CHS_GEMM_SIMT=1 selects the
   // vector-FMA kernel of the same tiling instead.
   bool use_mfma = true;
+  // counts workspace (re)allocations: a captured graph is valid only while
+  // the buffers it names are the ones it was captured with
+  uint64_t ws_gen = 0;
+  uint64_t tune_gen = 0;
+  // k-point set: member k lives in kset[k] except while swapped into `ham`
+  // (kset_cur == k; kset[k] then holds the displaced binding)
+  std::vector<chs::Hamiltonian> kset;
+  int kset_cur = -1;
+  chs::DevBuf kset_ws;          // per-k filter windows {c, inv_e} and flags
+  hipGraphExec_t kset_graph = nullptr;
+  std::vector<uintptr_t> kset_key;
+  uint64_t kset_key_gen = 0;
 };
 
 namespace chs {
@@ -78,11 +108,20 @@ int twiddles(chs_context* ctx, int n, int precision, const void** out);
 int fft3d(chs_context* ctx, int precision, void* grid, int n, int batch, int sign);
 // Stick tables for the bound plane-wave set.
 int build_sticks(chs_context* ctx);
+// Bind a plane-wave set + V_eff as the current Hamiltonian (no k-set swap).
+int bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_miller,
+                     const double* d_kin, const double* d_veff);
+// Make k-point k of the bound set the current Hamiltonian (k < 0: none).
+void kset_swap(chs_context* ctx, int k);
 
 // Local-potential H|psi> pipeline pieces (hamiltonian.hip).
+// d_coef: device {c, inv_e} read by the kernels instead of c, inv_e (so a
+// captured graph replays with new bounds); nullptr to use the values.
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
-                  const void* aux0, const void* aux1, double c, double inv_e, double* d_norm_part,
-                  int* n_part);
+                  const void* aux0, const double* d_coef, double c, double inv_e,
+                  double* d_norm_part, int* n_part);
+int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
+                  VlocTune* best, double* best_ms);
 
 // Modes of the stick epilogue.
 enum EpiMode {

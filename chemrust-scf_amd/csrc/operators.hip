@@ -93,16 +93,16 @@ __global__ void k_norm_guard_roll(const double* __restrict__ part, int n_part, i
 // One Chebyshev step Y_out = [2] (A Y_in - c Y_in)/e [- Y_prev] with the
 // growth guard; A = H (fused epilogue) or S^{-1} H (projector path).
 static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, void* yout,
-                     const void* yprev, bool first, double c, double inv_e, int k, double* flag) {
+                     const void* yprev, bool first, const double* d_coef, int k, double* flag) {
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
   const size_t ne = (size_t)n_b * H.n_pw;
   if (H.n_proj == 0) {
-    const int gz = ceil_div(H.st.n_st, 32);
+    const int gz = ceil_div(H.st.n_st, 16);  // partials of the smallest stick chunk
     CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * (size_t)gz * n_b + 64));
     int n_part = 0;
     CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
-                          nullptr, c, inv_e, (double*)im->part.p, &n_part));
+                          d_coef, 0.0, 0.0, (double*)im->part.p, &n_part));
     if (!first) {
       hipLaunchKernelGGL(k_norm_guard2, dim3(1), dim3(256), 0, ctx->stream,
                          (const double*)im->part.p, n_part, k, flag);
@@ -113,8 +113,8 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
   // projector path: the local step with the fused epilogue, then the
   // projector terms in coefficient space (cheb_nl_step), then the norm
   CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
-                        nullptr, c, inv_e, nullptr, nullptr));
-  CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, c, 1.0 / inv_e));
+                        d_coef, 0.0, 0.0, nullptr, nullptr));
+  CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, d_coef));
   CHS_TRY(ensure(ctx, im->part, sizeof(double) * kRedBlocks + 64));
   if (precision == CHS_C128)
     hipLaunchKernelGGL((k_sqnorm_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0,
@@ -125,6 +125,41 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
   hipLaunchKernelGGL(k_norm_guard_roll, dim3(1), dim3(256), 0, ctx->stream,
                      (const double*)im->part.p, kRedBlocks, k, first ? 1 : 0, flag);
   CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+// The degree-m recurrence of the bound Hamiltonian from Y0 in `a` (`b`: the
+// second buffer), window {c, inv_e} in device memory. Stream operations
+// only, so it can be captured once every workspace exists. *res receives
+// the buffer holding Y_m.
+int filter_steps(chs_context* ctx, int precision, void* a, void* b, int n_b, int degree,
+                 const double* d_coef, double* flag, void** res) {
+  Hamiltonian& H = ctx->impl->ham;
+  if (H.n_proj > 0) CHS_TRY(cheb_nl_begin(ctx, precision, a, n_b));
+  CHS_TRY(cheb_step(ctx, precision, a, n_b, b, nullptr, true, d_coef, 1, flag));
+  for (int k = 2; k <= degree; ++k) {
+    CHS_TRY(cheb_step(ctx, precision, b, n_b, a, a, false, d_coef, k, flag));
+    std::swap(a, b);
+  }
+  *res = b;
+  return CHS_OK;
+}
+
+// Divergence report of a filter's flag block (flag[0..3]); synchronises.
+int filter_check(chs_context* ctx, const double* flag) {
+  double h_flag[4];
+  CHS_HIP(ctx, hipMemcpyAsync(h_flag, flag, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h_flag[0] != 0.0) {
+    ctx->diag.iteration = (int)h_flag[0];
+    ctx->diag.norm_previous = h_flag[1];
+    ctx->diag.norm_current = h_flag[2];
+    ctx->diag.growth_factor = h_flag[3];
+    return set_error(ctx, CHS_ERR_CHEBYSHEV_DIVERGED,
+                     "Chebyshev filter diverged at iteration %d: norm %.6e -> %.6e (x%.3f)",
+                     (int)h_flag[0], h_flag[1], h_flag[2], h_flag[3]);
+  }
   return CHS_OK;
 }
 
@@ -141,37 +176,21 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
       CHS_HIP(ctx, hipMemcpyAsync(out, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return CHS_OK;
   }
-  const double c = b4[2], inv_e = 1.0 / b4[3];
   CHS_TRY(ensure(ctx, im->ws1, bytes));
   CHS_TRY(ensure(ctx, im->ws2, bytes));
   CHS_TRY(ensure(ctx, im->small, 64 * sizeof(double)));
   double* flag = (double*)im->small.p;
+  double* d_coef = flag + 8;
+  const double coef[2] = {b4[2], 1.0 / b4[3]};
   CHS_HIP(ctx, hipMemsetAsync(flag, 0, 8 * sizeof(double), ctx->stream));
-  // a <- Y0, b <- Y1
+  CHS_HIP(ctx, hipMemcpyAsync(d_coef, coef, sizeof(coef), hipMemcpyHostToDevice, ctx->stream));
   void* a = im->ws1.p;
   void* b = im->ws2.p;
   CHS_HIP(ctx, hipMemcpyAsync(a, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  if (H.n_proj > 0) CHS_TRY(cheb_nl_begin(ctx, precision, a, n_b));
-  CHS_TRY(cheb_step(ctx, precision, a, n_b, b, nullptr, true, c, inv_e, 1, flag));
-  for (int k = 2; k <= degree; ++k) {
-    CHS_TRY(cheb_step(ctx, precision, b, n_b, a, a, false, c, inv_e, k, flag));
-    std::swap(a, b);
-  }
-  CHS_HIP(ctx, hipMemcpyAsync(out, b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-  double h_flag[4];
-  CHS_HIP(ctx, hipMemcpyAsync(h_flag, flag, 4 * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (h_flag[0] != 0.0) {
-    ctx->diag.iteration = (int)h_flag[0];
-    ctx->diag.norm_previous = h_flag[1];
-    ctx->diag.norm_current = h_flag[2];
-    ctx->diag.growth_factor = h_flag[3];
-    return set_error(ctx, CHS_ERR_CHEBYSHEV_DIVERGED,
-                     "Chebyshev filter diverged at iteration %d: norm %.6e -> %.6e (x%.3f)",
-                     (int)h_flag[0], h_flag[1], h_flag[2], h_flag[3]);
-  }
-  return CHS_OK;
+  void* res = nullptr;
+  CHS_TRY(filter_steps(ctx, precision, a, b, n_b, degree, d_coef, flag, &res));
+  CHS_HIP(ctx, hipMemcpyAsync(out, res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return filter_check(ctx, flag);
 }
 
 // ---- Lanczos -----------------------------------------------------------

@@ -8,6 +8,9 @@ int ensure_precision_ready(chs_context* ctx, int precision);
 int apply_h_impl(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi, void* spsi);
 int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int n_b, int degree,
                           const double* b4, void* out);
+int filter_steps(chs_context* ctx, int precision, void* a, void* b, int n_b, int degree,
+                 const double* d_coef, double* flag, void** res);
+int filter_check(chs_context* ctx, const double* flag);
 int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min);
 // Nonlocal projector terms (projectors.hip): hpsi += beta D beta^H psi and,
 // when spsi != nullptr, spsi = psi + beta Q beta^H psi.
@@ -19,7 +22,8 @@ int sinv_apply(chs_context* ctx, int precision, void* x, int n_b);
 // cheb_nl_begin sets W_0 = beta^H X_0; cheb_nl_step turns the local-potential
 // step O (in place) into X_{k+1} and advances W.
 int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b);
-int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first, double c,
-                 double e);
+// d_coef: device {c, inv_e} of the filter window.
+int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
+                 const double* d_coef);
 
 }  // namespace chs

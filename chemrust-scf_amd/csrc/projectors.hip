@@ -126,70 +126,90 @@ static int nonlocal_t(chs_context* ctx, const cplx<T>* psi, int n_b, cplx<T>* hp
 // Z = e G + c W_0), and W_{k+1} = G + (2/e) B C. Two n_pw-long GEMMs per
 // step (G and the beta C update) instead of four, and no H|psi> block.
 // ---------------------------------------------------------------------------
+// coef = {c, inv_e} on the device (graph-replayable): Z = a1 G + a2 W_{k-1}
+// + c W with (a1, a2) = (e, 0) on the first step and (e/2, e/2) after;
+// DW = D W. Without S the scaled coefficients f D W go straight to Cs.
 template <typename T>
 __global__ void k_cheb_coef(const cplx<T>* __restrict__ G, const cplx<T>* __restrict__ Wp,
-                            const cplx<T>* __restrict__ W, const double* __restrict__ d, int P,
-                            size_t n, T a1, T a2, T c, cplx<T>* __restrict__ Z,
-                            cplx<T>* __restrict__ DW) {
+                            const cplx<T>* __restrict__ W, const double* __restrict__ d,
+                            const double* __restrict__ coef, int first, int P, size_t n,
+                            cplx<T>* __restrict__ Z, cplx<T>* __restrict__ DW) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const T c = (T)coef[0], e = (T)(1.0 / coef[1]);
   const cplx<T> w = W[i];
-  if (Z) {
-    // a2 == 0 on the first step, where W_{k-1} does not exist (not read)
-    const cplx<T> g = G[i], wp = a2 != (T)0 ? Wp[i] : mk((T)0, (T)0);
-    Z[i] = mk(a1 * g.x + a2 * wp.x + c * w.x, a1 * g.y + a2 * wp.y + c * w.y);
-  }
   const T dp = (T)d[(int)(i % P)];
-  DW[i] = mk(dp * w.x, dp * w.y);
+  if (Z) {
+    const T a1 = first ? e : (T)0.5 * e, a2 = first ? (T)0 : (T)0.5 * e;
+    // W_{k-1} does not exist on the first step (not read)
+    const cplx<T> g = G[i], wp = first ? mk((T)0, (T)0) : Wp[i];
+    Z[i] = mk(a1 * g.x + a2 * wp.x + c * w.x, a1 * g.y + a2 * wp.y + c * w.y);
+    DW[i] = mk(dp * w.x, dp * w.y);
+  } else {
+    const T f = (T)(first ? coef[1] : 2.0 * coef[1]);
+    DW[i] = mk(f * dp * w.x, f * dp * w.y);
+  }
+}
+
+// Cs *= f with f = inv_e (first step) or 2 inv_e
+template <typename T>
+__global__ void k_cheb_scale(cplx<T>* __restrict__ x, const double* __restrict__ coef, int first,
+                             size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T f = (T)(first ? coef[1] : 2.0 * coef[1]);
+  x[i] = mk(f * x[i].x, f * x[i].y);
 }
 
 template <typename T>
-static cplx<T>* nl_slot(chs_context* ctx, int n_b, int slot) {
-  return (cplx<T>*)ctx->impl->proj_ws.p + (size_t)slot * ctx->impl->ham.n_proj * n_b;
+static cplx<T>* nl_slot(Hamiltonian& H, int n_b, int slot) {
+  return (cplx<T>*)H.nl_ws.p + (size_t)slot * H.n_proj * n_b;
 }
 
 template <typename T>
 static int cheb_nl_begin_t(chs_context* ctx, const cplx<T>* X0, int n_b) {
   Hamiltonian& H = ctx->impl->ham;
   const int P = H.n_proj;
-  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 5 * (size_t)P * n_b * sizeof(cplx<T>)));
+  CHS_TRY(ensure(ctx, H.nl_ws, 5 * (size_t)P * n_b * sizeof(cplx<T>)));
   H.nl_wp = 0;
   H.nl_w = 1;
   H.nl_g = 2;
   const cplx<T>* beta = sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
   return gemm_any<T>(ctx, 'C', 'N', P, n_b, H.n_pw, 1.0, beta, H.n_pw, X0, H.n_pw, 0.0,
-                     nl_slot<T>(ctx, n_b, H.nl_w), P);
+                     nl_slot<T>(H, n_b, H.nl_w), P);
 }
 
 template <typename T>
-static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, double c, double e) {
+static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, const double* coef) {
   Hamiltonian& H = ctx->impl->ham;
   const int P = H.n_proj, n_pw = H.n_pw;
   const bool f32 = sizeof(T) == 4;
   const cplx<T>* beta = f32 ? (const cplx<T>*)H.beta_f.p : (const cplx<T>*)H.beta;
   const cplx<T>* B = f32 ? (const cplx<T>*)H.bmat_f.p : (const cplx<T>*)H.bmat.p;
   const cplx<T>* M = f32 ? (const cplx<T>*)H.sinv_m_f.p : (const cplx<T>*)H.sinv_m.p;
-  cplx<T>* Wp = nl_slot<T>(ctx, n_b, H.nl_wp);
-  cplx<T>* W = nl_slot<T>(ctx, n_b, H.nl_w);
-  cplx<T>* G = nl_slot<T>(ctx, n_b, H.nl_g);
-  cplx<T>* Z = nl_slot<T>(ctx, n_b, 3);
-  cplx<T>* Cc = nl_slot<T>(ctx, n_b, 4);
+  cplx<T>* Wp = nl_slot<T>(H, n_b, H.nl_wp);
+  cplx<T>* W = nl_slot<T>(H, n_b, H.nl_w);
+  cplx<T>* G = nl_slot<T>(H, n_b, H.nl_g);
+  cplx<T>* Z = nl_slot<T>(H, n_b, 3);
+  cplx<T>* Cs = nl_slot<T>(H, n_b, 4);
   const size_t pb = (size_t)P * n_b;
-  const double f = first ? 1.0 / e : 2.0 / e;
+  const unsigned gp = ceil_div(pb, 256);
   // G = beta^H O
   CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P));
-  const T a1 = (T)(first ? e : 0.5 * e), a2 = (T)(first ? 0.0 : 0.5 * e);
-  hipLaunchKernelGGL((k_cheb_coef<T>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream, G, Wp,
-                     W, H.dion, P, pb, a1, a2, (T)c, H.has_s ? Z : nullptr, Cc);
+  hipLaunchKernelGGL((k_cheb_coef<T>), dim3(gp), dim3(256), 0, ctx->stream, G, Wp, W, H.dion,
+                     coef, (int)first, P, pb, H.has_s ? Z : nullptr, Cs);
   CHS_LAUNCH_CHECK(ctx);
   if (H.has_s) {
-    // Z += B (D W); C = D W - M Z
-    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, B, P, Cc, P, 1.0, Z, P));
-    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, -1.0, M, P, Z, P, 1.0, Cc, P));
+    // Z += B (D W); C = D W - M Z; Cs = f C
+    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, B, P, Cs, P, 1.0, Z, P));
+    CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, -1.0, M, P, Z, P, 1.0, Cs, P));
+    hipLaunchKernelGGL((k_cheb_scale<T>), dim3(gp), dim3(256), 0, ctx->stream, Cs, coef,
+                       (int)first, pb);
+    CHS_LAUNCH_CHECK(ctx);
   }
-  // X_{k+1} = O + f beta C ; W_{k+1} = G + f B C (into G's slot)
-  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, f, beta, n_pw, Cc, P, 1.0, O, n_pw));
-  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, f, B, P, Cc, P, 1.0, G, P));
+  // X_{k+1} = O + beta Cs ; W_{k+1} = G + B Cs (into G's slot)
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, Cs, P, 1.0, O, n_pw));
+  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, B, P, Cs, P, 1.0, G, P));
   const int old_wp = H.nl_wp;
   H.nl_wp = H.nl_w;
   H.nl_w = H.nl_g;
@@ -203,10 +223,10 @@ int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b) {
   return cheb_nl_begin_t<float>(ctx, (const c32*)X0, n_b);
 }
 
-int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first, double c,
-                 double e) {
-  if (precision == CHS_C128) return cheb_nl_step_t<double>(ctx, (z64*)O, n_b, first, c, e);
-  return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, c, e);
+int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
+                 const double* d_coef) {
+  if (precision == CHS_C128) return cheb_nl_step_t<double>(ctx, (z64*)O, n_b, first, d_coef);
+  return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, d_coef);
 }
 
 int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,

@@ -305,6 +305,34 @@ int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref
                   const double* eps_ref, int n_b, double gap_min, float* d_feat, float* d_r,
                   float* d_e, float* d_w, int* n_used);
 
+/* ---- k-point sets (the k loop of ScfIteration::diagonalize,
+ * src/scf.rs:169-265) ---------------------------------------------------
+ * n_k plane-wave sets of one cell (n_pw[k] plane waves, Miller indices and
+ * kinetic energies 0.5 |k + G|^2 per k), one shared V_eff and, with
+ * n_proj > 0, per-k projectors d_beta[k] (n_proj x n_pw[k] c128) with
+ * shared D and Q. */
+int chs_kset_bind(chs_context* ctx, int n_k, int n_grid, const int* n_pw,
+                  const int32_t* const* d_miller, const double* const* d_kin,
+                  const double* d_veff, int n_proj, const double* const* d_beta,
+                  const double* d_d, const double* d_q);
+int chs_kset_set_veff(chs_context* ctx, const double* d_veff);
+/* Make k-point k the bound Hamiltonian for the other entry points
+ * (Rayleigh-Ritz, H|psi>, band CG); binding a new Hamiltonian returns it. */
+int chs_kset_select(chs_context* ctx, int k);
+/* Degree-m Chebyshev filter (chebyshev_filter, src/eigensolver/
+ * chebyshev.rs:626-816) of every k-point's block d_psi[k] (n_b x n_pw[k],
+ * in place) with windows bounds4[4k .. 4k+3] ({lambda_max, eps_cut, c, e}),
+ * all k-points and steps recorded as one HIP graph on the first call for
+ * these buffers and replayed by a single launch afterwards. */
+int chs_kset_filter(chs_context* ctx, int precision, void* const* d_psi, int n_b, int degree,
+                    const double* bounds4);
+/* Launch-shape autotuner of the H|psi> kernels for the bound Hamiltonian
+ * and precision (sticks per block of the z passes, plane row-buffer
+ * divisor): every candidate timed on n_b bands, the fastest kept for this
+ * grid and precision. Outputs the choice and its step time (ms). */
+int chs_autotune_hpsi(chs_context* ctx, int precision, int n_b, int* sticks_per_block,
+                      int* row_div, double* ms);
+
 #ifdef __cplusplus
 }
 #endif
