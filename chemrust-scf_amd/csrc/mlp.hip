@@ -22,6 +22,7 @@
 
 #include "blas1.hpp"
 #include "impl.hpp"
+#include "linalg.hpp"
 #include "ops.hpp"
 
 struct chs_mlp {
@@ -420,21 +421,19 @@ __global__ void __launch_bounds__(256) k_pair_dots(const z64* __restrict__ x, co
   if (threadIdx.x == 0) out[(size_t)blockIdx.x * 8 + slot] = mk(a, b);
 }
 
-// e = ref * phase - psi^ ; r = (hp - eps sp) / sqrt(n)
+// r = (hp - eps sp) / sqrt(n) ; e = t - psi^ with t (in e) the normalised
+// projection of psi^ on the reference eigenspace of the band
 __global__ void k_pair_build(const z64* __restrict__ psi, const z64* __restrict__ hp,
-                             const z64* __restrict__ sp, const z64* __restrict__ ref, int n_pw,
-                             const z64* __restrict__ dots, z64* __restrict__ r, z64* __restrict__ e) {
+                             const z64* __restrict__ sp, int n_pw, const z64* __restrict__ dots,
+                             z64* __restrict__ r, z64* __restrict__ e) {
   const int b = blockIdx.y, g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n_pw) return;
   const z64* d = dots + (size_t)b * 8;
   const double nn = d[0].x, inv = 1.0 / sqrt(nn), eps = d[1].x / nn;
-  const z64 c = d[2];
-  const double cm = sqrt(norm2(c));
-  const z64 ph = cm > 0.0 ? mk(c.x / cm, c.y / cm) : mk(1.0, 0.0);
   const size_t i = (size_t)b * n_pw + g;
   const z64 p = psi[i], h = hp[i], s = sp[i];
   r[i] = mk((h.x - eps * s.x) * inv, (h.y - eps * s.y) * inv);
-  e[i] = ref[i] * ph - mk(p.x * inv, p.y * inv);
+  e[i] = e[i] - mk(p.x * inv, p.y * inv);
 }
 
 // e -= psi^ <S psi^|e>, then the f32 pair arrays and features
@@ -654,7 +653,7 @@ int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref
   if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "no Hamiltonian bound");
   auto* im = ctx->impl;
   const int n_pw = H.n_pw;
-  const size_t ne = (size_t)n_b * n_pw;
+  const size_t ne = (size_t)n_b * n_pw, nn2 = (size_t)n_b * n_b;
   CHS_TRY(ensure(ctx, im->ws4, ne * sizeof(z64)));
   CHS_TRY(ensure(ctx, im->ws5, ne * sizeof(z64)));
   CHS_TRY(ensure(ctx, im->ws3, 2 * ne * sizeof(z64)));
@@ -666,38 +665,71 @@ int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref
   const z64* ref = (const z64*)d_psi_ref;
   CHS_TRY(apply_h_impl(ctx, CHS_C128, psi, n_b, HP, SP));
   // sized before mlp_stats takes pointers into it
-  CHS_TRY(ensure(ctx, im->mlp_ws, (size_t)n_b * (2 * sizeof(double) + 8 * sizeof(z64)) + 512));
+  const size_t dots_off = (((size_t)n_b * 2 * sizeof(double) + 255) / 256) * 256;
+  const size_t ov_off = dots_off + (((size_t)n_b * 8 * sizeof(z64) + 255) / 256) * 256;
+  CHS_TRY(ensure(ctx, im->mlp_ws, ov_off + nn2 * sizeof(z64) + 256));
   double *ekin, *vstat;
   CHS_TRY(mlp_stats(ctx, psi, n_b, &ekin, &vstat));
-  z64* dots = (z64*)((char*)im->mlp_ws.p + (((size_t)n_b * 2 * sizeof(double) + 255) / 256) * 256);
+  z64* dots = (z64*)((char*)im->mlp_ws.p + dots_off);
+  z64* OV = (z64*)((char*)im->mlp_ws.p + ov_off);
   double* wband = ekin + n_b;
   const dim3 eg(ceil_div(n_pw, 256), n_b);
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   hipLaunchKernelGGL(k_pair_dots, dim3(n_b), dim3(256), 0, ctx->stream, psi, SP, n_pw, dots, 0);
   hipLaunchKernelGGL(k_pair_dots, dim3(n_b), dim3(256), 0, ctx->stream, psi, HP, n_pw, dots, 1);
-  hipLaunchKernelGGL(k_pair_dots, dim3(n_b), dim3(256), 0, ctx->stream, ref, SP, n_pw, dots, 2);
-  hipLaunchKernelGGL(k_pair_build, eg, dim3(256), 0, ctx->stream, psi, HP, SP, ref, n_pw, dots, R, E);
+  CHS_LAUNCH_CHECK(ctx);
+  // OV[j][b] = <psi*_j | S | psi_b>
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, ref, n_pw, SP, n_pw, zero, OV, n_b));
+  std::vector<z64> hov(nn2), hd((size_t)n_b * 8);
+  CHS_HIP(ctx, hipMemcpyAsync(hov.data(), OV, nn2 * sizeof(z64), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipMemcpyAsync(hd.data(), dots, hd.size() * sizeof(z64), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += (nn2 + hd.size()) * sizeof(z64);
+  // reference eigenspaces: runs of eigenvalues closer than gap_min; the run
+  // touching the top of the block may be incomplete and is not used
+  std::vector<int> g0(n_b), g1(n_b);
+  for (int b = 0; b < n_b;) {
+    int e = b + 1;
+    while (e < n_b && eps_ref[e] - eps_ref[e - 1] <= gap_min) ++e;
+    for (int j = b; j < e; ++j) {
+      g0[j] = b;
+      g1[j] = e;
+    }
+    b = e;
+  }
+  // t_b = P* psi^_b / |P* psi^_b|: masked, normalised overlap columns
+  std::vector<double> wb(n_b, 0.0);
+  std::vector<char> ok(n_b, 0);
+  for (int b = 0; b < n_b; ++b) {
+    double p2 = 0.0;
+    for (int j = g0[b]; j < g1[b]; ++j) p2 += norm2(hov[(size_t)j + (size_t)b * n_b]);
+    const double nn = hd[(size_t)b * 8].x;
+    ok[b] = g1[b] < n_b && nn > 0.0 && p2 / nn > 0.81;
+    const double sc = p2 > 0.0 ? 1.0 / std::sqrt(p2) : 0.0;
+    for (int j = 0; j < n_b; ++j) {
+      z64& o = hov[(size_t)j + (size_t)b * n_b];
+      o = (j >= g0[b] && j < g1[b]) ? mk(o.x * sc, o.y * sc) : mk(0.0, 0.0);
+    }
+  }
+  CHS_HIP(ctx, hipMemcpyAsync(OV, hov.data(), nn2 * sizeof(z64), hipMemcpyHostToDevice,
+                              ctx->stream));
+  ctx->h2d += nn2 * sizeof(z64);
+  CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, one, ref, n_pw, OV, n_b, zero, E, n_pw));
+  hipLaunchKernelGGL(k_pair_build, eg, dim3(256), 0, ctx->stream, psi, HP, SP, n_pw, dots, R, E);
   hipLaunchKernelGGL(k_pair_dots, dim3(n_b), dim3(256), 0, ctx->stream, SP, E, n_pw, dots, 3);
   // |e|^2 after the projection, for the band weights
   hipLaunchKernelGGL(k_sub_scaled, eg, dim3(256), 0, ctx->stream, E, psi, n_pw, dots, HP);
   hipLaunchKernelGGL(k_pair_dots, dim3(n_b), dim3(256), 0, ctx->stream, HP, HP, n_pw, dots, 4);
   CHS_LAUNCH_CHECK(ctx);
-  std::vector<z64> hd((size_t)n_b * 8);
   CHS_HIP(ctx, hipMemcpyAsync(hd.data(), dots, hd.size() * sizeof(z64), hipMemcpyDeviceToHost,
                               ctx->stream));
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  ctx->d2h += hd.size() * sizeof(z64);
-  // bands used: well separated reference eigenvalue, block band mostly along
-  // the reference one, and a non-zero correction; weights normalise each
-  // band's |e|^2 and average over the used bands
-  std::vector<double> wb(n_b, 0.0);
   int used = 0;
   for (int b = 0; b < n_b; ++b) {
-    const double gl = b > 0 ? eps_ref[b] - eps_ref[b - 1] : 1e300;
-    const double gr = b + 1 < n_b ? eps_ref[b + 1] - eps_ref[b] : 1e300;
-    const double nn = hd[(size_t)b * 8].x;
-    const double ov = std::sqrt(norm2(hd[(size_t)b * 8 + 2]) / nn);
     const double e2 = hd[(size_t)b * 8 + 4].x;
-    if (std::min(gl, gr) > gap_min && ov > 0.9 && e2 > 1e-24) {
+    if (ok[b] && e2 > 1e-24) {
       wb[b] = 1.0 / e2;
       ++used;
     }

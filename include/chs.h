@@ -296,11 +296,12 @@ int chs_mlp_scale_table(chs_context* ctx, chs_mlp* m, const double* d_psi, int n
                         float* d_table);
 /* Training pairs of an unconverged c128 block d_psi against the converged
  * block d_psi_ref (eigenvalues eps_ref, host) on the bound Hamiltonian:
- * r = (H - eps S) psi^ with psi^ = psi / |psi|_S, e = psi_ref (phase
- * aligned) - psi^ with its psi^ component removed. Bands closer than
- * gap_min to a neighbour, or whose overlap with the reference band is
- * below 0.9, get weight 0; the others 1 / (n_used |e_b|^2). Outputs n_b x
- * n_pw pairs in band-major order. */
+ * r = (H - eps S) psi^ with psi^ = psi / |psi|_S, e = t - psi^ with its
+ * psi^ component removed, where t is the normalised projection of psi^ on
+ * the reference eigenspace of band b (reference eigenvalues closer than
+ * gap_min form one eigenspace). Bands whose projection norm is below 0.9,
+ * or whose eigenspace reaches the top of the block, get weight 0; the
+ * others 1 / (n_used |e_b|^2). Outputs n_b x n_pw pairs, band-major. */
 int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref,
                   const double* eps_ref, int n_b, double gap_min, float* d_feat, float* d_r,
                   float* d_e, float* d_w, int* n_used);

@@ -79,6 +79,10 @@ void build_sinv(Projectors& pj, int n_pw) {
   const int P = pj.n_proj;
   pj.sinv_m.assign((size_t)P * P, cd(0));
   if (P == 0) return;
+  // Q = 0: S = 1, and M = (Q^{-1} + beta^H beta)^{-1} -> 0 in the limit
+  bool any_q = false;
+  for (int i = 0; i < P; ++i) any_q = any_q || pj.q[i] != 0.0;
+  if (!any_q) return;
   std::vector<cd> a((size_t)P * P);
 #pragma omp parallel for collapse(2) schedule(static)
   for (int j = 0; j < P; ++j)
