@@ -92,11 +92,11 @@ __global__ void __launch_bounds__(256) k_gemm(int m, int n, int k, cplx<T> alpha
 }
 
 // ---------------------------------------------------------------------------
-// Matrix-core variant: the same 64 x 64 x 16 LDS tiles; each of the 4
-// wavefronts owns a 32 x 32 complex sub-tile as 2 x 2 MFMA 16 x 16 tiles and
-// forms the complex product from four real 16x16x4 MFMAs per k-step
-// (re += ar br - ai bi, im += ar bi + ai br). Operand layout of both
-// v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32: lane l supplies
+// Matrix-core variant: LDS-staged block tiles (below), register prefetch of
+// the next K stage during the MFMAs; the complex product is four real
+// 16x16x4 MFMAs per k-step (re += ar br - ai bi, im += ar bi + ai br).
+// Operand layout of both v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32:
+// lane l supplies
 // A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]. Result layout: lane l
 // holds column l % 16, rows 4 (l / 16) + v (f32) or l / 16 + 4 v (f64).
 // ---------------------------------------------------------------------------
@@ -116,6 +116,17 @@ template <> struct Mfma<float> {
   __device__ __forceinline__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// Block tile of the matrix-core kernel: 128 x 128 x 16 (complex64) or
+// 128 x 64 x 8 (complex128); 4 wavefronts as 2 x 2, each owning
+// WTM x WTN MFMA tiles of 16 x 16.
+template <typename T> struct MfmaTile;
+template <> struct MfmaTile<float> {
+  static constexpr int BM = 128, BN = 128, BK = 16, WTM = 4, WTN = 4;
+};
+template <> struct MfmaTile<double> {
+  static constexpr int BM = 128, BN = 64, BK = 8, WTM = 4, WTN = 2;
+};
+
 template <typename T, int OPA, int OPB>
 __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
                                                    const cplx<T>* __restrict__ A, int lda,
@@ -124,47 +135,75 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
                                                    int klen, cplx<T>* __restrict__ part) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
-  __shared__ cplx<T> As[GBK][GBM + 1];
-  __shared__ cplx<T> Bs[GBK][GBN + 1];
+  constexpr int BM = MfmaTile<T>::BM, BN = MfmaTile<T>::BN, BK = MfmaTile<T>::BK;
+  constexpr int WTM = MfmaTile<T>::WTM, WTN = MfmaTile<T>::WTN;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // staged elements per thread
+  __shared__ cplx<T> As[BK][BM + 4];
+  __shared__ cplx<T> Bs[BK][BN + 4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wr = w >> 1, wc = w & 1;
+  const int wm0 = (w >> 1) * (16 * WTM), wn0 = (w & 1) * (16 * WTN);
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kb = blockIdx.z * klen, ke = min(k, kb + klen);
-  acc_t cre[2][2], cim[2][2];
+  acc_t cre[WTM][WTN], cim[WTM][WTN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < WTN; ++j) {
       cre[i][j] = acc_t{0, 0, 0, 0};
       cim[i][j] = acc_t{0, 0, 0, 0};
     }
-  for (int k0 = kb; k0 < ke; k0 += GBK) {
-    for (int t = threadIdx.x; t < GBM * GBK; t += 256) {
+  cplx<T> ra[NA], rb[NB];
+  auto a_pos = [](int e, int& r, int& kk) {
+    if (OPA == 0) { r = e % BM; kk = e / BM; } else { kk = e % BK; r = e / BK; }
+  };
+  auto b_pos = [](int e, int& c, int& kk) {
+    if (OPB == 0) { kk = e % BK; c = e / BK; } else { c = e % BN; kk = e / BN; }
+  };
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
       int r, kk;
-      if (OPA == 0) { r = t % GBM; kk = t / GBM; } else { kk = t % GBK; r = t / GBK; }
+      a_pos(threadIdx.x + 256 * j, r, kk);
       const int gr = m0 + r, gk = k0 + kk;
-      As[kk][r] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
+      ra[j] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
     }
-    for (int t = threadIdx.x; t < GBK * GBN; t += 256) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
       int c, kk;
-      if (OPB == 0) { kk = t % GBK; c = t / GBK; } else { c = t % GBN; kk = t / GBN; }
+      b_pos(threadIdx.x + 256 * j, c, kk);
       const int gc = n0 + c, gk = k0 + kk;
-      Bs[kk][c] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+      rb[j] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      int r, kk;
+      a_pos(threadIdx.x + 256 * j, r, kk);
+      As[kk][r] = ra[j];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      int c, kk;
+      b_pos(threadIdx.x + 256 * j, c, kk);
+      Bs[kk][c] = rb[j];
     }
     __syncthreads();
+    if (k0 + BK < ke) load(k0 + BK);  // next stage in flight during the MFMAs
 #pragma unroll
-    for (int s = 0; s < GBK / 4; ++s) {
+    for (int s = 0; s < BK / 4; ++s) {
       const int kk = 4 * s + lk;
-      cplx<T> a[2], b[2];
+      cplx<T> a[WTM], b[WTN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk][32 * wr + 16 * i + li];
+      for (int i = 0; i < WTM; ++i) a[i] = As[kk][wm0 + 16 * i + li];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) b[j] = Bs[kk][32 * wc + 16 * j + li];
+      for (int j = 0; j < WTN; ++j) b[j] = Bs[kk][wn0 + 16 * j + li];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < WTM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < WTN; ++j) {
           cre[i][j] = M::mac(a[i].x, b[j].x, cre[i][j]);
           cre[i][j] = M::mac(-a[i].y, b[j].y, cre[i][j]);
           cim[i][j] = M::mac(a[i].x, b[j].y, cim[i][j]);
@@ -174,13 +213,13 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < WTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < WTN; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int r = m0 + 32 * wr + 16 * i + M::row(lane, v);
-        const int c = n0 + 32 * wc + 16 * j + li;
+        const int r = m0 + wm0 + 16 * i + M::row(lane, v);
+        const int c = n0 + wn0 + 16 * j + li;
         if (r < m && c < n) {
           const cplx<T> acc = mk((T)cre[i][j][v], (T)cim[i][j][v]);
           if (part) {
@@ -223,22 +262,25 @@ static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cpl
   const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
   if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
   if (m == 0 || n == 0) return CHS_OK;
-  const int tiles = (int)(ceil_div(m, GBM) * ceil_div(n, GBN));
+  const bool mf = ctx->impl->use_mfma;
+  const int bm = mf ? MfmaTile<T>::BM : GBM, bn = mf ? MfmaTile<T>::BN : GBN;
+  const int bk = mf ? MfmaTile<T>::BK : GBK;
+  const int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
   int ksplit = 1;
-  if (tiles < 512 && k > 4 * GBK) {
+  if (tiles < 512 && k > 4 * bk) {
     ksplit = std::min(512 / tiles + 1, (k + 255) / 256);
     ksplit = std::max(ksplit, 1);
   }
   int klen = (k + ksplit - 1) / ksplit;
-  klen = ((klen + GBK - 1) / GBK) * GBK;
-  if (klen == 0) klen = GBK;
+  klen = ((klen + bk - 1) / bk) * bk;
+  if (klen == 0) klen = bk;
   ksplit = std::max(1, (k + klen - 1) / klen);
   cplx<T>* part = nullptr;
   if (ksplit > 1) {
     CHS_TRY(ensure(ctx, ctx->impl->gemm_ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
     part = (cplx<T>*)ctx->impl->gemm_ws.p;
   }
-  dim3 grid(ceil_div(m, GBM), ceil_div(n, GBN), ksplit);
+  dim3 grid(ceil_div(m, bm), ceil_div(n, bn), ksplit);
 #define CHS_GEMM_CASE(OA, OB)                                                                  \
   if (oa == OA && ob == OB) {                                                                  \
     if (ctx->impl->use_mfma)                                                                   \

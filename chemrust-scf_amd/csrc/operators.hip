@@ -249,48 +249,43 @@ __global__ void k_lanczos_scale(z64* v, z64* sv, const double* __restrict__ ab, 
   if (gen) sv[i] = mk(sv[i].x * s, sv[i].y * s);
 }
 
-// Eigenvalues of the k x k tridiagonal T (alpha, beta) by cyclic Jacobi in
-// one thread; out = {max + beta_{k-1}, min}.
+// Extreme eigenvalues of the k x k Lanczos tridiagonal T (alpha_i = ab[2i],
+// beta_i = ab[2i+1]) by bisection on the Sturm count, one thread each;
+// out = {lambda_max(T) + beta_{k-1}, lambda_min(T)}.
 __global__ void k_lanczos_finish(const double* __restrict__ ab, int k, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double t[16][16];
-  for (int i = 0; i < k; ++i)
-    for (int j = 0; j < k; ++j) t[i][j] = 0;
+  const int which = threadIdx.x;  // 0: largest, 1: smallest
+  if (which > 1 || blockIdx.x != 0) return;
+  double gl = INFINITY, gu = -INFINITY, bmax2 = 0.0;
   for (int i = 0; i < k; ++i) {
-    t[i][i] = ab[2 * i];
-    if (i + 1 < k) t[i][i + 1] = t[i + 1][i] = ab[2 * i + 1];
+    const double bl = i > 0 ? fabs(ab[2 * (i - 1) + 1]) : 0.0, br = i + 1 < k ? fabs(ab[2 * i + 1]) : 0.0;
+    gl = fmin(gl, ab[2 * i] - bl - br);
+    gu = fmax(gu, ab[2 * i] + bl + br);
+    if (i + 1 < k) bmax2 = fmax(bmax2, ab[2 * i + 1] * ab[2 * i + 1]);
   }
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0;
-    for (int i = 0; i < k; ++i)
-      for (int j = i + 1; j < k; ++j) off += t[i][j] * t[i][j];
-    if (off < 1e-30) break;
-    for (int p = 0; p < k - 1; ++p)
-      for (int q = p + 1; q < k; ++q) {
-        const double apq = t[p][q];
-        if (fabs(apq) < 1e-300) continue;
-        const double tau = (t[q][q] - t[p][p]) / (2.0 * apq);
-        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + tt * tt), s = tt * c;
-        for (int r = 0; r < k; ++r) {
-          const double a = t[r][p], b = t[r][q];
-          t[r][p] = c * a - s * b;
-          t[r][q] = s * a + c * b;
-        }
-        for (int r = 0; r < k; ++r) {
-          const double a = t[p][r], b = t[q][r];
-          t[p][r] = c * a - s * b;
-          t[q][r] = s * a + c * b;
-        }
-      }
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, bmax2);
+  const double tn = fmax(fabs(gl), fabs(gu));
+  double lo = gl - 4.0 * 2.220446049250313e-16 * tn * k - 2.0 * pivmin;
+  double hi = gu + 4.0 * 2.220446049250313e-16 * tn * k + 2.0 * pivmin;
+  const int idx = which == 0 ? k - 1 : 0;  // eigenvalue index (ascending)
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    int cnt = 0;  // eigenvalues below mid
+    double q = ab[0] - mid;
+    if (fabs(q) < pivmin) q = -pivmin;
+    cnt += q < 0.0;
+    for (int i = 1; i < k; ++i) {
+      const double b = ab[2 * (i - 1) + 1];
+      q = ab[2 * i] - mid - b * b / q;
+      if (fabs(q) < pivmin) q = -pivmin;
+      cnt += q < 0.0;
+    }
+    if (cnt <= idx) lo = mid;
+    else hi = mid;
   }
-  double mx = t[0][0], mn = t[0][0];
-  for (int i = 1; i < k; ++i) {
-    mx = fmax(mx, t[i][i]);
-    mn = fmin(mn, t[i][i]);
-  }
-  out[0] = mx + ab[2 * (k - 1) + 1];
-  out[1] = mn;
+  const double lam = 0.5 * (lo + hi);
+  if (which == 0) out[0] = lam + ab[2 * (k - 1) + 1];
+  else out[1] = lam;
 }
 
 static int dot_re_into(chs_context* ctx, const z64* a, const z64* b, size_t n, double* dst) {
@@ -375,7 +370,7 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
                        vp, svp, f, sf, ab, j, (int)gen, (size_t)n);
     CHS_LAUNCH_CHECK(ctx);
   }
-  hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(1), 0, ctx->stream, ab, k, red + 4);
+  hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(64), 0, ctx->stream, ab, k, red + 4);
   CHS_LAUNCH_CHECK(ctx);
   double hr[2];
   CHS_HIP(ctx, hipMemcpyAsync(hr, red + 4, 2 * sizeof(double), hipMemcpyDeviceToHost,
