@@ -250,6 +250,79 @@ __device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
   wave_sync();
 }
 
+// One x-row of the plane pass: inverse transform from the compact row (only
+// active columns stored; act_of[x] = compact index of column x, or -1),
+// V_eff multiply, forward transform back into the compact row, with two
+// exchanges through the group's N-element scratch `sc` and everything else
+// in registers:
+//   inverse: lane n2 does DFT_N1 over x[n1 N2 + n2] and twiddles; lane k1
+//            then DFT_N2 over n2, giving y[k1 + N1 k2];
+//   y *= V;  lane k1 does the forward DFT_N2 over k2 and twiddles
+//            w_N^{-k1 m2}; lane m2 then DFT_N1 over k1, giving
+//            Z[m2 + N2 m1], stored for the active columns only.
+template <int N, typename T>
+__device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc,
+                                            const int* __restrict__ act_of,
+                                            const T* __restrict__ vrow, int lane,
+                                            const cplx<T>* __restrict__ tw) {
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int R = (N1 + N2 - 1) / N2;
+  {
+    cplx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      const int ai = act_of[n1 * N2 + lane];
+      a[n1] = ai >= 0 ? crow[ai] : mk((T)0, (T)0);
+    }
+    Dft<N1, 1, T>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * twiddle<1>(tw, lane * k1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) sc[k1 * N2 + lane] = a[k1];
+  }
+  wave_sync();
+  cplx<T> b[R][N2];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[r][n2] = sc[k1 * N2 + n2];
+      Dft<N2, 1, T>::run(b[r]);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const T v = vrow[k1 + N1 * k2];
+        b[r][k2] = mk(b[r][k2].x * v, b[r][k2].y * v);
+      }
+      Dft<N2, -1, T>::run(b[r]);
+#pragma unroll
+      for (int m2 = 1; m2 < N2; ++m2) b[r][m2] = b[r][m2] * twiddle<-1>(tw, k1 * m2);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+#pragma unroll
+      for (int m2 = 0; m2 < N2; ++m2) sc[k1 + N1 * m2] = b[r][m2];
+    }
+  }
+  wave_sync();
+  {
+    cplx<T> c[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) c[k1] = sc[k1 + N1 * lane];
+    Dft<N1, -1, T>::run(c);
+#pragma unroll
+    for (int m1 = 0; m1 < N1; ++m1) {
+      const int ai = act_of[lane + N2 * m1];
+      if (ai >= 0) crow[ai] = c[m1];
+    }
+  }
+  wave_sync();
+}
+
 // Group geometry of a block: groups of N2 lanes packed into each wavefront.
 template <int N> struct Groups {
   static constexpr int N2 = Split<N>::N2;

@@ -10,10 +10,10 @@
 //                    sbuf[b][z][stick] (coalesced over sticks);
 //   B  k_plane     : per (band, z-plane): sticks -> LDS plane in compact
 //                    form [iy][active ix]; inverse y-FFT on the active
-//                    columns; per row: expand to a full x row, inverse
-//                    x-FFT with V_eff(r) applied as the transform's last
-//                    store, forward x-FFT, compact; forward y-FFT; sticks
-//                    back to sbuf (in place);
+//                    columns; per row: inverse x-FFT read straight from the
+//                    compact row, x V_eff(r) and the forward x-FFT in
+//                    registers, active outputs back (row_apply_v); forward
+//                    y-FFT; sticks back to sbuf (in place);
 //   C  k_stick_fwd : per (band, 32 sticks): forward z-FFT, gather the plane
 //                    waves, x 1/N^3, + 0.5|k+G|^2 psi, and the Chebyshev
 //                    three-term update of chebyshev.rs:700-762 in the same
@@ -42,6 +42,7 @@ struct StickArgs {
   const int32_t* cnt;
   const int32_t* iz0;
   const int32_t* act_ix;
+  const int32_t* act_of;  // N: compact index of x column, or -1
 };
 
 template <typename T> __device__ __forceinline__ cplx<T> zero_c() { return mk((T)0, (T)0); }
@@ -81,15 +82,6 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
   }
 }
 
-// V_eff(r) applied to the inverse x-FFT output of one row
-template <typename T> struct MulV {
-  const T* v;
-  __device__ __forceinline__ cplx<T> operator()(int k, cplx<T> c) const {
-    const T s = v[k];
-    return mk(c.x * s, c.y * s);
-  }
-};
-
 // occ * |psi(r)|^2 accumulated into the block's own density plane row
 struct AccRho {
   double* row;
@@ -117,13 +109,15 @@ __global__ void __launch_bounds__(256) k_plane(cplx<T>* __restrict__ sbuf, Stick
                                                const T* __restrict__ veff, int nrb,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  constexpr int LS = N + 1, N2 = Split<N>::N2;
+  constexpr int LS = N + 1;
   const int RS = st.rs;
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
+  int* act_of = reinterpret_cast<int*>(rowbuf + nrb * LS);
   const int z = blockIdx.x, b = blockIdx.y;
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.n_st;
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
+  for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
   __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
   __syncthreads();
@@ -133,16 +127,9 @@ __global__ void __launch_bounds__(256) k_plane(cplx<T>* __restrict__ sbuf, Stick
     for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
   __syncthreads();
   if (g >= 0 && g < nrb) {
-    cplx<T>* rb = rowbuf + g * LS;
+    cplx<T>* sc = rowbuf + g * LS;
     const T* vz = veff + (size_t)z * N * N;
-    for (int iy = g; iy < N; iy += nrb) {
-      cplx<T>* crow = cmp + iy * RS;
-      expand_row<N, T>(crow, rb, st, lane);
-      group_fft_line<N, 1>(rb, 1, lane, tw, MulV<T>{vz + iy * N});
-      group_fft_line<N, -1>(rb, 1, lane, tw, PostId());
-      for (int a = lane; a < st.n_act; a += N2) crow[a] = rb[st.act_ix[a]];
-      wave_sync();
-    }
+    for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, act_of, vz + iy * N, lane, tw);
   }
   __syncthreads();
   if (g >= 0)
@@ -328,9 +315,10 @@ int build_sticks(chs_context* ctx) {
   for (int s = 0; s < n_st; ++s) act[s] = act_of[ix[s]];
   const int n_act = (int)act_ix.size();
   std::vector<int32_t> all;
-  all.reserve(5 * (size_t)n_st + n_act);
+  all.reserve(5 * (size_t)n_st + n_act + n);
   for (auto* v : {&iy, &act, &pw0, &cnt, &iz0}) all.insert(all.end(), v->begin(), v->end());
   all.insert(all.end(), act_ix.begin(), act_ix.end());
+  all.insert(all.end(), act_of.begin(), act_of.end());
   CHS_TRY(ensure(ctx, H.st_mem, all.size() * sizeof(int32_t)));
   CHS_HIP(ctx, hipMemcpyAsync(H.st_mem.p, all.data(), all.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, ctx->stream));
@@ -348,6 +336,7 @@ int build_sticks(chs_context* ctx) {
   st.cnt = d + 3 * (size_t)n_st;
   st.iz0 = d + 4 * (size_t)n_st;
   st.act_ix = d + 5 * (size_t)n_st;
+  st.act_of = d + 5 * (size_t)n_st + n_act;
   st.h_act_ix = act_ix;
   return CHS_OK;
 }
@@ -365,6 +354,7 @@ static StickArgs stick_args(const StickTable& st) {
   a.cnt = st.cnt;
   a.iz0 = st.iz0;
   a.act_ix = st.act_ix;
+  a.act_of = st.act_of;
   return a;
 }
 
@@ -391,14 +381,15 @@ static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz
   const size_t cmp_bytes = (size_t)n * sa.rs * csz;
   const size_t row_bytes = (size_t)(n + 1) * csz;
   const int lim = max_lds_per_block(ctx);
-  if (cmp_bytes + row_bytes > (size_t)lim)
+  if (cmp_bytes + row_bytes + (size_t)n * sizeof(int32_t) > (size_t)lim)
     return set_error(ctx, CHS_ERR_FFT, "plane of %d^3 grid does not fit in LDS (%zu > %d bytes)",
                      n, cmp_bytes + row_bytes, lim);
   const int groups = 4 * (64 / n2);  // 256 threads
-  int nrb = (int)((lim - cmp_bytes) / row_bytes);
+  const size_t tab_bytes = (size_t)n * sizeof(int32_t);  // act_of
+  int nrb = (int)((lim - cmp_bytes - tab_bytes) / row_bytes);
   nrb = std::max(1, std::min(nrb, groups) / std::max(1, nrb_div));
   out->nrb = nrb;
-  out->lds = cmp_bytes + (size_t)nrb * row_bytes;
+  out->lds = cmp_bytes + (size_t)nrb * row_bytes + tab_bytes;
   return CHS_OK;
 }
 
