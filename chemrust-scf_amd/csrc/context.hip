@@ -89,6 +89,7 @@ int chs_context_create(int device, chs_context** out) {
   ctx->impl = new chs_context::Impl;
   if (const char* ng = std::getenv("CHS_NO_GRAPHS")) ctx->impl->use_graphs = !(ng[0] && ng[0] != '0');
   if (const char* gs = std::getenv("CHS_GEMM_SIMT")) ctx->impl->use_mfma = !(gs[0] && gs[0] != '0');
+  if (const char* gt = std::getenv("CHS_GEMM_TUNE")) ctx->impl->gemm_autotune = !(gt[0] == '0');
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev0) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev1) != hipSuccess) {
@@ -108,7 +109,8 @@ int chs_context_destroy(chs_context* ctx) {
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
                     &im->ws4, &im->ws5, &im->ws6, &im->gemm_ws, &im->lin_ws, &im->scf_ws,
-                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->kset_ws};
+                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->kset_ws,
+                    &im->gemm_tune_ws, &im->gemm_tune_c};
   for (DevBuf* b : bufs) release(*b);
   release_hamiltonian(im->ham);
   for (Hamiltonian& h : im->kset) release_hamiltonian(h);

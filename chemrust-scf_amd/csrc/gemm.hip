@@ -116,35 +116,59 @@ template <> struct Mfma<float> {
   __device__ __forceinline__ static int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-// Block tile of the matrix-core kernel: 128 x 128 x 16 (complex64) or
-// 128 x 64 x 8 (complex128); 4 wavefronts as 2 x 2, each owning
-// WTM x WTN MFMA tiles of 16 x 16.
-template <typename T> struct MfmaTile;
-template <> struct MfmaTile<float> {
+// Block tiles of the matrix-core kernel; 4 wavefronts as 2 x 2, each owning
+// WTM x WTN MFMA tiles of 16 x 16. Small: 64 x 64 (both precisions); large:
+// 128 x 128 x 16 (complex64) or 128 x 64 x 8 (complex128). The autotuner
+// picks per GEMM shape.
+template <typename T, int V> struct MfmaTile;
+template <typename T> struct MfmaTile<T, 1> {
+  static constexpr int BM = 64, BN = 64, BK = 16, WTM = 2, WTN = 2;
+};
+template <> struct MfmaTile<float, 2> {
   static constexpr int BM = 128, BN = 128, BK = 16, WTM = 4, WTN = 4;
 };
-template <> struct MfmaTile<double> {
+template <> struct MfmaTile<double, 2> {
   static constexpr int BM = 128, BN = 64, BK = 8, WTM = 4, WTN = 2;
 };
 
-template <typename T, int OPA, int OPB>
+// XCD-aware block order: workgroups are dealt to the eight XCDs round-robin
+// by launch index, so physical block p runs on XCD p % 8. Logical block
+// L = (p % 8) * (P / 8) + p / 8 gives every XCD a contiguous range of
+// logical blocks, ordered (k-slice, column tile, row tile) with the row
+// tile fastest: the tiles that share a k-slice of A and B, and the column
+// tiles that share rows of A, sit in one L2.
+__device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int* bm, int* bn, int* bz) {
+  const int P = gridDim.x, p = blockIdx.x;
+  const int L = (p & 7) * (P >> 3) + (p >> 3);
+  if (L >= mt * nt * kz) return false;
+  *bm = L % mt;
+  *bn = (L / mt) % nt;
+  *bz = L / (mt * nt);
+  return true;
+}
+
+template <typename T, int OPA, int OPB, int V>
 __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
                                                    const cplx<T>* __restrict__ A, int lda,
                                                    const cplx<T>* __restrict__ B, int ldb,
                                                    cplx<T> beta, cplx<T>* __restrict__ C, int ldc,
-                                                   int klen, cplx<T>* __restrict__ part) {
+                                                   int klen, int ksplit,
+                                                   cplx<T>* __restrict__ part) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
-  constexpr int BM = MfmaTile<T>::BM, BN = MfmaTile<T>::BN, BK = MfmaTile<T>::BK;
-  constexpr int WTM = MfmaTile<T>::WTM, WTN = MfmaTile<T>::WTN;
+  using TL = MfmaTile<T, V>;
+  constexpr int BM = TL::BM, BN = TL::BN, BK = TL::BK;
+  constexpr int WTM = TL::WTM, WTN = TL::WTN;
+  int bm, bn, bz;
+  if (!gemm_block((m + BM - 1) / BM, (n + BN - 1) / BN, ksplit, &bm, &bn, &bz)) return;
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // staged elements per thread
   __shared__ cplx<T> As[BK][BM + 4];
   __shared__ cplx<T> Bs[BK][BN + 4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm0 = (w >> 1) * (16 * WTM), wn0 = (w & 1) * (16 * WTN);
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  const int kb = blockIdx.z * klen, ke = min(k, kb + klen);
+  const int m0 = bm * BM, n0 = bn * BN;
+  const int kb = bz * klen, ke = min(k, kb + klen);
   acc_t cre[WTM][WTN], cim[WTM][WTN];
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
@@ -223,7 +247,7 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
         if (r < m && c < n) {
           const cplx<T> acc = mk((T)cre[i][j][v], (T)cim[i][j][v]);
           if (part) {
-            part[(size_t)blockIdx.z * m * n + (size_t)r + (size_t)c * m] = acc;
+            part[(size_t)bz * m * n + (size_t)r + (size_t)c * m] = acc;
           } else {
             cplx<T> val = alpha * acc;
             if (beta.x != (T)0 || beta.y != (T)0) val = val + beta * C[(size_t)r + (size_t)c * ldc];
@@ -231,6 +255,60 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
           }
         }
       }
+}
+
+// ---------------------------------------------------------------------------
+// n = 1 (matrix-vector) in the memory-bound form: y = alpha op(A) x + beta y.
+// op N: one thread per row, A read down its columns (coalesced across rows);
+// op T / C: one wavefront per output element, a reduction along the column.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemv_n(int m, int k, cplx<T> alpha,
+                                                const cplx<T>* __restrict__ A, int lda,
+                                                const cplx<T>* __restrict__ x, cplx<T> beta,
+                                                cplx<T>* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  T re = 0, im = 0;
+  for (int kk = 0; kk < k; ++kk) {
+    const cplx<T> a = A[(size_t)i + (size_t)kk * lda], b = x[kk];
+    re = fma(a.x, b.x, re);
+    re = fma(-a.y, b.y, re);
+    im = fma(a.x, b.y, im);
+    im = fma(a.y, b.x, im);
+  }
+  cplx<T> v = alpha * mk(re, im);
+  if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * y[i];
+  y[i] = v;
+}
+
+template <typename T, int OPA>
+__global__ void __launch_bounds__(256) k_gemv_t(int m, int k, cplx<T> alpha,
+                                                const cplx<T>* __restrict__ A, int lda,
+                                                const cplx<T>* __restrict__ x, cplx<T> beta,
+                                                cplx<T>* __restrict__ y) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= m) return;
+  const cplx<T>* col = A + (size_t)i * lda;
+  T re = 0, im = 0;
+  for (int kk = lane; kk < k; kk += 64) {
+    cplx<T> a = col[kk];
+    if (OPA == 2) a.y = -a.y;
+    const cplx<T> b = x[kk];
+    re = fma(a.x, b.x, re);
+    re = fma(-a.y, b.y, re);
+    im = fma(a.x, b.y, im);
+    im = fma(a.y, b.x, im);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    re += __shfl_down(re, off);
+    im += __shfl_down(im, off);
+  }
+  if (lane == 0) {
+    cplx<T> v = alpha * mk(re, im);
+    if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * y[i];
+    y[i] = v;
+  }
 }
 
 template <typename T>
@@ -255,40 +333,50 @@ int gemm_opcode(char op) {
   }
 }
 
-template <typename T>
-static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cplx<T> alpha,
-                  const cplx<T>* A, int lda, const cplx<T>* B, int ldb, cplx<T> beta, cplx<T>* C,
-                  int ldc) {
-  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
-  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
-  if (m == 0 || n == 0) return CHS_OK;
-  const bool mf = ctx->impl->use_mfma;
-  const int bm = mf ? MfmaTile<T>::BM : GBM, bn = mf ? MfmaTile<T>::BN : GBN;
-  const int bk = mf ? MfmaTile<T>::BK : GBK;
+// Split-K plan of a tile shape: enough blocks to fill the device when the
+// output tiles alone cannot.
+static void split_plan(int m, int n, int k, int bm, int bn, int bk, int* ksplit, int* klen) {
   const int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
-  int ksplit = 1;
-  if (tiles < 512 && k > 4 * bk) {
-    ksplit = std::min(512 / tiles + 1, (k + 255) / 256);
-    ksplit = std::max(ksplit, 1);
+  int ks = 1;
+  if (tiles < 512 && k > 4 * bk) ks = std::max(1, std::min(512 / tiles + 1, (k + 255) / 256));
+  int kl = (k + ks - 1) / ks;
+  kl = ((kl + bk - 1) / bk) * bk;
+  if (kl == 0) kl = bk;
+  *klen = kl;
+  *ksplit = std::max(1, (k + kl - 1) / kl);
+}
+
+template <typename T>
+static int gemm_launch(chs_context* ctx, int variant, int oa, int ob, int m, int n, int k,
+                       cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
+                       cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws) {
+  int bm = GBM, bn = GBN, bk = GBK;
+  if (variant == 1) {
+    bm = MfmaTile<T, 1>::BM; bn = MfmaTile<T, 1>::BN; bk = MfmaTile<T, 1>::BK;
+  } else if (variant == 2) {
+    bm = MfmaTile<T, 2>::BM; bn = MfmaTile<T, 2>::BN; bk = MfmaTile<T, 2>::BK;
   }
-  int klen = (k + ksplit - 1) / ksplit;
-  klen = ((klen + bk - 1) / bk) * bk;
-  if (klen == 0) klen = bk;
-  ksplit = std::max(1, (k + klen - 1) / klen);
+  int ksplit, klen;
+  split_plan(m, n, k, bm, bn, bk, &ksplit, &klen);
   cplx<T>* part = nullptr;
   if (ksplit > 1) {
-    CHS_TRY(ensure(ctx, ctx->impl->gemm_ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
-    part = (cplx<T>*)ctx->impl->gemm_ws.p;
+    CHS_TRY(ensure(ctx, ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
+    part = (cplx<T>*)ws.p;
   }
-  dim3 grid(ceil_div(m, bm), ceil_div(n, bn), ksplit);
-#define CHS_GEMM_CASE(OA, OB)                                                                  \
-  if (oa == OA && ob == OB) {                                                                  \
-    if (ctx->impl->use_mfma)                                                                   \
-      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k,   \
-                         alpha, A, lda, B, ldb, beta, C, ldc, klen, part);                     \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_gemm<T, OA, OB>), grid, dim3(256), 0, ctx->stream, m, n, k, alpha, \
-                         A, lda, B, ldb, beta, C, ldc, klen, part);                            \
+  const unsigned nblk = ceil_div(m, bm) * ceil_div(n, bn) * (unsigned)ksplit;
+  const dim3 g1(((nblk + 7) / 8) * 8);  // multiple of 8 for the XCD-aware order
+  const dim3 g3(ceil_div(m, bm), ceil_div(n, bn), ksplit);
+#define CHS_GEMM_CASE(OA, OB)                                                                    \
+  if (oa == OA && ob == OB) {                                                                    \
+    if (variant == 1)                                                                            \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
+                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, part);               \
+    else if (variant == 2)                                                                       \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
+                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, part);               \
+    else                                                                                         \
+      hipLaunchKernelGGL((k_gemm<T, OA, OB>), g3, dim3(256), 0, ctx->stream, m, n, k, alpha, A,  \
+                         lda, B, ldb, beta, C, ldc, klen, part);                                 \
   }
   CHS_GEMM_CASE(0, 0) CHS_GEMM_CASE(0, 1) CHS_GEMM_CASE(0, 2)
   CHS_GEMM_CASE(1, 0) CHS_GEMM_CASE(1, 1) CHS_GEMM_CASE(1, 2)
@@ -301,6 +389,77 @@ static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cpl
     CHS_LAUNCH_CHECK(ctx);
   }
   return CHS_OK;
+}
+
+// Kernel variant for a GEMM shape: 0 vector-FMA 64 x 64, 1 MFMA 64 x 64,
+// 2 MFMA 128-wide. Chosen once per (precision, ops, m, n, k) by timing every
+// variant into a scratch output (HIP events, context stream); outside a
+// stream capture only, otherwise the MFMA 64 x 64 default.
+template <typename T>
+static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, const cplx<T>* A,
+                        int lda, const cplx<T>* B, int ldb, int* variant) {
+  auto* im = ctx->impl;
+  if (!im->use_mfma) {
+    *variant = 0;
+    return CHS_OK;
+  }
+  const GemmKey key{(int)sizeof(T), oa, ob, m, n, k};
+  auto it = im->gemm_tune.find(key);
+  if (it != im->gemm_tune.end()) {
+    *variant = it->second;
+    return CHS_OK;
+  }
+  *variant = 1;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  CHS_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone || !im->gemm_autotune) return CHS_OK;
+  CHS_TRY(ensure(ctx, im->gemm_tune_c, (size_t)m * n * sizeof(cplx<T>)));
+  cplx<T>* Ct = (cplx<T>*)im->gemm_tune_c.p;
+  const cplx<T> one = mk((T)1, (T)0), zero = mk((T)0, (T)0);
+  double best = 1e300;
+  for (int v = 0; v < 3; ++v) {
+    CHS_TRY(gemm_launch<T>(ctx, v, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
+                           im->gemm_tune_ws));  // warm-up, workspace growth
+    CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+    CHS_TRY(gemm_launch<T>(ctx, v, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
+                           im->gemm_tune_ws));
+    CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+    CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+    float t = 0;
+    CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+    if (t < best) {
+      best = t;
+      *variant = v;
+    }
+  }
+  im->gemm_tune[key] = *variant;
+  return CHS_OK;
+}
+
+template <typename T>
+static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cplx<T> alpha,
+                  const cplx<T>* A, int lda, const cplx<T>* B, int ldb, cplx<T> beta, cplx<T>* C,
+                  int ldc) {
+  const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
+  if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
+  if (m == 0 || n == 0) return CHS_OK;
+  if (n == 1 && ob == 0 && k > 0) {  // matrix-vector
+    if (oa == 0)
+      hipLaunchKernelGGL((k_gemv_n<T>), dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, m, k,
+                         alpha, A, lda, B, beta, C);
+    else if (oa == 1)
+      hipLaunchKernelGGL((k_gemv_t<T, 1>), dim3(ceil_div(m, 4)), dim3(256), 0, ctx->stream, m, k,
+                         alpha, A, lda, B, beta, C);
+    else
+      hipLaunchKernelGGL((k_gemv_t<T, 2>), dim3(ceil_div(m, 4)), dim3(256), 0, ctx->stream, m, k,
+                         alpha, A, lda, B, beta, C);
+    CHS_LAUNCH_CHECK(ctx);
+    return CHS_OK;
+  }
+  int variant = 1;
+  CHS_TRY(gemm_variant<T>(ctx, oa, ob, m, n, k, A, lda, B, ldb, &variant));
+  return gemm_launch<T>(ctx, variant, oa, ob, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                        ctx->impl->gemm_ws);
 }
 
 int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,

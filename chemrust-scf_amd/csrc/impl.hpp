@@ -63,6 +63,17 @@ inline void release_hamiltonian(Hamiltonian& h) {
   h = Hamiltonian{};
 }
 
+// GEMM shape key of the kernel-variant cache.
+struct GemmKey {
+  int tsize, oa, ob, m, n, k;
+  bool operator<(const GemmKey& o) const {
+    const int a[6] = {tsize, oa, ob, m, n, k}, b[6] = {o.tsize, o.oa, o.ob, o.m, o.n, o.k};
+    for (int i = 0; i < 6; ++i)
+      if (a[i] != b[i]) return a[i] < b[i];
+    return false;
+  }
+};
+
 // Launch shape of the H|psi> pipeline: sticks per block of the z passes and
 // the divisor of the plane kernel's row-buffer count.
 struct VlocTune {
@@ -87,6 +98,10 @@ struct chs_context::Impl {
   // Complex GEMMs on the matrix cores (MFMA); CHS_GEMM_SIMT=1 selects the
   // vector-FMA kernel of the same tiling instead.
   bool use_mfma = true;
+  // per-shape GEMM kernel variant, timed on first use (CHS_GEMM_TUNE=0: off)
+  bool gemm_autotune = true;
+  std::map<chs::GemmKey, int> gemm_tune;
+  chs::DevBuf gemm_tune_ws, gemm_tune_c;
   // counts workspace (re)allocations: a captured graph is valid only while
   // the buffers it names are the ones it was captured with
   uint64_t ws_gen = 0;
