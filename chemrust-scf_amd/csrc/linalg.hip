@@ -558,7 +558,14 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   z64* HY = (z64*)im->ws4.p;
   z64* SY = (z64*)im->ws5.p;
   const bool gen = H.has_s;
-  CHS_TRY(apply_h_impl(ctx, CHS_C128, Y, n_b, HY, gen ? SY : nullptr));
+  // with projectors and no H|Y> wanted: the local H|Y> only, the projector
+  // parts of H_sub and S_sub in coefficient space (subspace_nl)
+  const bool nl = H.n_proj > 0 && !HY_out;
+  if (nl)
+    CHS_TRY(vloc_pipeline(ctx, CHS_C128, Y, n_b, HY, EPI_HPSI, nullptr, nullptr, 0.0, 0.0, nullptr,
+                          nullptr));
+  else
+    CHS_TRY(apply_h_impl(ctx, CHS_C128, Y, n_b, HY, gen ? SY : nullptr));
   const size_t nn = (size_t)n_b * n_b;
   CHS_TRY(ensure(ctx, im->scf_ws, (2 * nn + n_b) * sizeof(z64) + 64));
   z64* Hs = (z64*)im->scf_ws.p;
@@ -566,7 +573,9 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   double* w = (double*)(Ss + nn);
   const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, HY, n_pw, zero, Hs, n_b));
-  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen ? SY : Y, n_pw, zero, Ss, n_b));
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen && !nl ? SY : Y, n_pw, zero, Ss,
+                n_b));
+  if (nl) CHS_TRY(subspace_nl(ctx, Y, n_b, Hs, gen ? Ss : nullptr));
   CHS_TRY(hermitise(ctx, Hs, n_b));
   CHS_TRY(hermitise(ctx, Ss, n_b));
   CHS_TRY(zhegv(ctx, Hs, Ss, n_b, w));

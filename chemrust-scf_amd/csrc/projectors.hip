@@ -229,6 +229,28 @@ int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
   return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, d_coef);
 }
 
+// Projector parts of the Rayleigh-Ritz subspace matrices of a c128 block Y:
+// with W = beta^H Y (P x n_b), Hs += W^H D W and, when Ss is given,
+// Ss += W^H Q W. One n_pw-long GEMM instead of the two (beta D W, beta Q W)
+// a full H|Y>, S|Y> would need.
+int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  const size_t pb = (size_t)P * n_b;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 3 * pb * sizeof(z64)));
+  z64* W = (z64*)ctx->impl->proj_ws.p;
+  z64* DW = W + pb;
+  z64* QW = DW + pb;
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, one, (const z64*)H.beta, n_pw, Y, n_pw, zero, W, P));
+  hipLaunchKernelGGL((k_row_scale2<double>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream, W,
+                     P, n_b, H.dion, H.qion, DW, Ss ? QW : nullptr);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, P, one, W, P, DW, P, one, Hs, n_b));
+  if (Ss) CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, P, one, W, P, QW, P, one, Ss, n_b));
+  return CHS_OK;
+}
+
 int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
                    void* spsi) {
   Hamiltonian& H = ctx->impl->ham;
