@@ -140,8 +140,8 @@ def lib():
             "chs_mlp_destroy": [vp],
             "chs_mlp_set_params": [vp, vp, vp],
             "chs_mlp_get_params": [vp, vp, vp],
-            "chs_mlp_loss_grad": [vp, vp, vp, vp, vp, vp, C.c_int64, dp, vp],
-            "chs_mlp_train": [vp, vp, vp, vp, vp, vp, C.c_int64, i, C.c_int64, C.c_float, dp],
+            "chs_mlp_loss_grad": [vp, vp, vp, vp, vp, vp, C.c_int64, i, dp, vp],
+            "chs_mlp_train": [vp, vp, vp, vp, vp, vp, C.c_int64, i, i, i, C.c_float, dp],
             "chs_mlp_scale_table": [vp, vp, vp, i, vp],
             "chs_mlp_pairs": [vp, vp, vp, dp, i, d, vp, vp, vp, vp, C.POINTER(i)],
         }
@@ -524,20 +524,23 @@ class Mlp:
         self.ctx.check(lib().chs_mlp_set_params(self.ctx.h, self.h, th.ctypes.data_as(vp)))
 
     def loss_grad(self, pairs, n=None):
+        """Band loss and its gradient over the first n pairs (whole bands)."""
         n = pairs["n"] if n is None else n
         loss = C.c_double()
         g = np.zeros(self.N_PARAMS, np.float32)
         self.ctx.check(lib().chs_mlp_loss_grad(self.ctx.h, self.h, pairs["feat"].ptr,
                                                pairs["r"].ptr, pairs["e"].ptr, pairs["w"].ptr, n,
-                                               C.byref(loss), g.ctypes.data_as(vp)))
+                                               pairs["band_len"], C.byref(loss),
+                                               g.ctypes.data_as(vp)))
         return loss.value, g
 
-    def train(self, pairs, steps, batch, lr, n=None):
+    def train(self, pairs, steps, batch_bands, lr, n=None):
+        """Adam steps on minibatches of whole bands; returns the loss trace."""
         n = pairs["n"] if n is None else n
         trace = np.zeros(max(steps, 1))
         self.ctx.check(lib().chs_mlp_train(self.ctx.h, self.h, pairs["feat"].ptr, pairs["r"].ptr,
-                                           pairs["e"].ptr, pairs["w"].ptr, n, steps, batch, lr,
-                                           _dptr(trace)))
+                                           pairs["e"].ptr, pairs["w"].ptr, n, pairs["band_len"],
+                                           steps, batch_bands, lr, _dptr(trace)))
         return trace[:steps]
 
     def scale_table(self, psi, out=None):
@@ -551,7 +554,7 @@ def mlp_pair_buffers(ctx, n):
     """Device arrays for n training pairs (feat n x 4, r / e complex64, w)."""
     return {"feat": Gpu(ctx, (n, 4), np.float32), "r": Gpu(ctx, (n,), np.complex64),
             "e": Gpu(ctx, (n,), np.complex64), "w": Gpu(ctx, (n,), np.float32), "n": 0,
-            "cap": n}
+            "cap": n, "band_len": 0}
 
 
 def mlp_pairs(ctx, psi, psi_ref, eps_ref, pairs, gap_min=1e-3):
@@ -561,6 +564,9 @@ def mlp_pairs(ctx, psi, psi_ref, eps_ref, pairs, gap_min=1e-3):
     o = pairs["n"]
     if o + n_b * n_pw > pairs["cap"]:
         raise ValueError("pair buffers full")
+    if pairs["band_len"] not in (0, n_pw):
+        raise ValueError("pairs of one buffer set must share the band length")
+    pairs["band_len"] = n_pw
     used = C.c_int()
     er = np.ascontiguousarray(eps_ref, np.float64)
     at = lambda k, sz: C.c_void_p(pairs[k].ptr.value + o * sz)

@@ -78,7 +78,7 @@ class Sample:
                 break
         return psi, eps
 
-    def rough(self, passes=2, degree=6):
+    def rough(self, passes=3, degree=8):
         """An unconverged block: `passes` low-degree passes from a fresh start."""
         psi = chs.Gpu.from_host(self.ctx, start_block(self.n_b, self.kin, 2000 + self.seed))
         eps = self.chfsi(psi, passes, degree)
@@ -91,15 +91,16 @@ def copy_block(ctx, psi):
     return out
 
 
-def collect_pairs(ctx, samples, pairs, cg_steps=(1, 3), gap_min=1e-3):
+def collect_pairs(ctx, samples, pairs, passes=(2, 3), cg_steps=(2, 5), gap_min=1e-3):
     """Training pairs of every sample: its early subspace-iteration blocks
-    (1 and 2 passes) and band-CG (TPA) from them after each of cg_steps."""
+    (`passes` passes of degree 8) and band-CG (TPA) from them after each of
+    cg_steps steps."""
     used = 0
     for s in samples:
         s.bind()
         ref, eps_ref = s.converged()
-        for passes in (1, 2):
-            blk, _ = s.rough(passes)
+        for p in passes:
+            blk, _ = s.rough(p)
             used += chs.mlp_pairs(ctx, blk, ref, eps_ref, pairs, gap_min)
             for k in cg_steps:
                 b2 = copy_block(ctx, blk)
@@ -108,12 +109,13 @@ def collect_pairs(ctx, samples, pairs, cg_steps=(1, 3), gap_min=1e-3):
     return used
 
 
-def evaluate(ctx, sample, net, tol=1e-9, max_steps=300, uspp_term=False):
+def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False):
     """Band-CG from the same rough block with TPA and with the network.
-    Returns dict with total steps, max |eps - eps*| for both."""
+    Returns dict with total steps, converged bands and max |eps - eps*| for
+    both."""
     sample.bind()
     ref, eps_ref = sample.converged()
-    blk, _ = sample.rough(2)
+    blk, _ = sample.rough()
     out = {}
     for name in ("tpa", "mlp"):
         b = copy_block(ctx, blk)

@@ -8,14 +8,21 @@
 //   f = [kin_g / E_b, log E_b, V_mean, V_std],  s = t(kin_g / E_b) e^o / E_b,
 // with E_b the band's kinetic energy and V the bound local potential, so an
 // untrained (zero-output) network is the per-band TPA preconditioner.
-// Training minimises L = sum_i w_i |s_i r_i + e_i|^2 over residual /
-// correction pairs (r, e) with Adam on FP32 master weights; the forward pass
-// always uses the bfloat16-rounded weights.
+// Training pairs come per band (residual r and correction e over the band's
+// plane waves). The loss is the band-mean misalignment of the
+// preconditioned residual with the correction,
+//   L = (1/n_bands) sum_b [1 - (sum_g s_g a_g)^2 / (sum_g s_g^2 c_g sum_g |e_g|^2)],
+//   a = -Re(conj(r) e), c = |r|^2,
+// which is scale-free per band as band CG's line search is (one
+// preconditioned step along s r leaves an error of relative size L_b);
+// minimised by Adam on FP32 master weights, the forward pass always with
+// the bfloat16-rounded weights.
 //
-// Kernels: k_mlp_grad (forward + backward over 32-sample tiles in LDS,
-// per-block gradient partials), k_mlp_reduce (fixed-order sum), k_mlp_adam,
-// k_mlp_table (one thread per (band, plane wave) inference), and the
-// training-pair builder.
+// Kernels: k_mlp_pass<0> (forward over band-aligned 32-sample tiles in LDS,
+// per-tile sums), k_mlp_band_sums, k_mlp_pass<1> (forward + backward with
+// the band coefficients, per-block gradient partials), k_mlp_reduce
+// (fixed-order sums), k_mlp_adam, k_mlp_table (one thread per (band, plane
+// wave) inference), and the training-pair builder.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -26,7 +33,7 @@
 #include "ops.hpp"
 
 struct chs_mlp {
-  chs::DevBuf theta, m, v, grad, wbf, part, lpart;
+  chs::DevBuf theta, m, v, grad, wbf, part, tsum, bcoef;
   int t = 0;
 };
 
@@ -93,28 +100,34 @@ __device__ __forceinline__ long long batch_index(long long j, long long off, lon
 // 128 threads: thread (mb, sb) = (t >> 3, t & 7) owns a 4 x 4 output block
 // of every 64 x 32 layer product; for the weight gradients it owns the
 // 4 x 8 block (j, i) = (4 (t >> 3) + q, 8 (t & 7) + p).
-__global__ void __launch_bounds__(128) k_mlp_grad(const unsigned short* __restrict__ wbf,
+// Batch: nbb whole bands; batch band j is band (off + j stride) mod NB of
+// the pair arrays (band length Lb); tile tt = (j, q0) covers samples q0 ..
+// q0 + 31 of band j. PASS 0 writes per-tile sums {sum s a, sum s^2 c,
+// sum |e|^2} to tsum; PASS 1 back-propagates dL/do = g1 (g2 s c - a) s with
+// the band coefficients bcoef[j] = {g1, g2}.
+template <int PASS>
+__global__ void __launch_bounds__(128) k_mlp_pass(const unsigned short* __restrict__ wbf,
                                                   const float* __restrict__ feat,
                                                   const float* __restrict__ rr,
                                                   const float* __restrict__ ee,
-                                                  const float* __restrict__ ww, long long n,
-                                                  long long nb, long long off, long long stride,
-                                                  float* __restrict__ part,
-                                                  double* __restrict__ lpart) {
+                                                  const float* __restrict__ ww, long long NB,
+                                                  int Lb, int nbb, long long off,
+                                                  long long stride, double* __restrict__ tsum,
+                                                  const double* __restrict__ bcoef,
+                                                  float* __restrict__ part) {
   __shared__ MlpLds L;
   __shared__ __attribute__((aligned(16))) float Z1[kH * kTSP], Z2[kH * kTSP], Z3[kH * kTSP];
   __shared__ __attribute__((aligned(16))) float FT[kIn * kTSP];
   __shared__ float dO[kTS];
-  __shared__ double lsh[2];
-  load_weights(L, wbf);
+    load_weights(L, wbf);
   const int t = threadIdx.x, mb = t >> 3, sb = t & 7;
   float g3[4][8], g2[4][8], g1[2];
   for (int q = 0; q < 4; ++q)
     for (int p = 0; p < 8; ++p) g3[q][p] = g2[q][p] = 0.0f;
   g1[0] = g1[1] = 0.0f;
   float gb1 = 0, gb2 = 0, gb3 = 0, gw4 = 0, gb4 = 0;
-  double lacc = 0.0;
-  const long long tiles = (nb + kTS - 1) / kTS;
+  const int tpb = (Lb + kTS - 1) / kTS;
+  const long long tiles = (long long)nbb * tpb;
   // out[m][s] = act_in?(sum_k A(k, m) B[k][s]) for the thread's 4 x 4 block
   auto mm_fwd = [&](const float* W, int ldw, int K, const float* B, bool silu_b, float acc[4][4]) {
     for (int k = 0; k < K; ++k) {
@@ -131,12 +144,12 @@ __global__ void __launch_bounds__(128) k_mlp_grad(const unsigned short* __restri
     }
   };
   for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const long long j0 = tile * kTS;
+    const int jb = (int)(tile / tpb), q0 = (int)(tile % tpb) * kTS;
+    const long long base = batch_index(jb, off, stride, NB) * Lb;  // first pair of the band
     __syncthreads();
     for (int e = t; e < kIn * kTS; e += blockDim.x) {
       const int s = e / kIn, c = e % kIn;
-      const long long j = j0 + s;
-      FT[c * kTSP + s] = j < nb ? feat[batch_index(j, off, stride, n) * kIn + c] : 0.0f;
+      FT[c * kTSP + s] = q0 + s < Lb ? feat[(base + q0 + s) * kIn + c] : 0.0f;
     }
     __syncthreads();
     // forward: Z1 = W1 F + b1, Z2 = W2 silu(Z1) + b2, Z3 = W3 silu(Z2) + b3
@@ -170,23 +183,41 @@ __global__ void __launch_bounds__(128) k_mlp_grad(const unsigned short* __restri
             make_float4(acc[q][0], acc[q][1], acc[q][2], acc[q][3]);
     }
     __syncthreads();
-    // output, loss and dL/do per sample
+    // output s, the band sums (PASS 0) or dL/do (PASS 1) per sample
+    double sa = 0.0, sc = 0.0, se = 0.0;
     if (t < kTS) {
-      const long long j = j0 + t;
       float d = 0.0f;
-      if (j < nb) {
+      const long long i = base + q0 + t;
+      if (q0 + t < Lb && ww[i] > 0.0f) {
         float o = L.b4;
         for (int k = 0; k < kH; ++k) o = fmaf(L.w4[k], silu(Z3[k * kTSP + t]), o);
-        const long long i = batch_index(j, off, stride, n);
         const float x = FT[0 * kTSP + t], eb = expf(FT[1 * kTSP + t]);
         const float s = tpa_t(x) * expf(o) / eb;
         const float r0 = rr[2 * i], r1 = rr[2 * i + 1], e0 = ee[2 * i], e1 = ee[2 * i + 1];
-        const float w = ww[i];
-        const float q0 = s * r0 + e0, q1 = s * r1 + e1;
-        lacc += (double)w * ((double)q0 * q0 + (double)q1 * q1);
-        d = 2.0f * w * (r0 * q0 + r1 * q1) * s;
+        const float a = -(r0 * e0 + r1 * e1), c = r0 * r0 + r1 * r1;
+        if (PASS == 0) {
+          sa = (double)s * a;
+          sc = (double)s * s * c;
+          se = (double)e0 * e0 + (double)e1 * e1;
+        } else {
+          const float g1 = (float)bcoef[2 * jb], g2 = (float)bcoef[2 * jb + 1];
+          d = g1 * (g2 * s * c - a) * s;
+        }
       }
       dO[t] = d;
+    }
+    if (PASS == 0) {
+      for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        sa += __shfl_down(sa, o2);
+        sc += __shfl_down(sc, o2);
+        se += __shfl_down(se, o2);
+      }
+      if (t == 0) {  // samples live in threads 0 .. 31 (wave 0)
+        tsum[3 * tile] = sa;
+        tsum[3 * tile + 1] = sc;
+        tsum[3 * tile + 2] = se;
+      }
+      continue;
     }
     __syncthreads();
     // output layer gradients; dZ3 = dO w4 dsilu(Z3) in place
@@ -270,6 +301,7 @@ __global__ void __launch_bounds__(128) k_mlp_grad(const unsigned short* __restri
     if (t < kH)
       for (int s = 0; s < kTS; ++s) gb1 += Z1[t * kTSP + s];
   }
+  if (PASS == 0) return;
   // block partial in the flat parameter layout
   float* pb = part + (size_t)blockIdx.x * kNP;
   for (int q = 0; q < 4; ++q)
@@ -287,15 +319,31 @@ __global__ void __launch_bounds__(128) k_mlp_grad(const unsigned short* __restri
     pb[oW4 + t] = gw4;
   }
   if (t == 0) pb[oB4] = gb4;
-  // loss partial: the per-sample values live in threads t < kTS
-  for (int off2 = 32; off2 > 0; off2 >>= 1) lacc += __shfl_down(lacc, off2);
-  if ((t & 63) == 0) lsh[t >> 6] = lacc;
-  __syncthreads();
-  if (t == 0) lpart[blockIdx.x] = lsh[0] + lsh[1];
 }
 
-__global__ void k_mlp_reduce(const float* __restrict__ part, int nblk, const double* __restrict__ lpart,
-                             float* __restrict__ grad, double* __restrict__ loss) {
+// Per batch band: A, B, E from its tiles (fixed order); L_b = 1 - A^2/(B E)
+// for used bands (B, E > 0), 0 otherwise; bcoef = {2A/(B E) / nbb, A/B};
+// lossv[j] = L_b / nbb.
+__global__ void k_mlp_band_sums(const double* __restrict__ tsum, int nbb, int tpb,
+                                double* __restrict__ bcoef, double* __restrict__ lossv) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nbb) return;
+  double A = 0, B = 0, E = 0;
+  for (int q = 0; q < tpb; ++q) {
+    const double* p = tsum + 3 * ((size_t)j * tpb + q);
+    A += p[0];
+    B += p[1];
+    E += p[2];
+  }
+  const bool used = B > 0.0 && E > 0.0;
+  bcoef[2 * j] = used ? 2.0 * A / (B * E) / nbb : 0.0;
+  bcoef[2 * j + 1] = used ? A / B : 0.0;
+  lossv[j] = used ? (1.0 - A * A / (B * E)) / nbb : 0.0;
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nblk,
+                             const double* __restrict__ lossv, int nl, float* __restrict__ grad,
+                             double* __restrict__ loss) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < kNP) {
     double s = 0.0;
@@ -304,7 +352,7 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, int nblk, const dou
   }
   if (p == 0) {
     double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += lpart[b];
+    for (int b = 0; b < nl; ++b) s += lossv[b];
     *loss = s;
   }
 }
@@ -491,17 +539,27 @@ static int mlp_stats(chs_context* ctx, const z64* psi, int n_b, double** ekin, d
 }
 
 static int mlp_batch(chs_context* ctx, chs_mlp* m, const float* feat, const float* r,
-                     const float* e, const float* w, long long n, long long nb, long long off,
+                     const float* e, const float* w, long long NB, int Lb, int nbb, long long off,
                      long long stride, double* d_loss) {
+  const int tpb = (Lb + kTS - 1) / kTS;
+  const long long tiles = (long long)nbb * tpb;
   CHS_TRY(ensure(ctx, m->part, (size_t)kMlpBlocks * kNP * sizeof(float)));
-  CHS_TRY(ensure(ctx, m->lpart, (size_t)kMlpBlocks * sizeof(double)));
-  const long long tiles = (nb + kTS - 1) / kTS;
+  CHS_TRY(ensure(ctx, m->tsum, (size_t)tiles * 3 * sizeof(double)));
+  CHS_TRY(ensure(ctx, m->bcoef, (size_t)nbb * 3 * sizeof(double)));
+  double* bcoef = (double*)m->bcoef.p;
+  double* lossv = bcoef + 2 * (size_t)nbb;
   const int nblk = (int)std::min<long long>(kMlpBlocks, std::max<long long>(1, tiles));
-  hipLaunchKernelGGL(k_mlp_grad, dim3(nblk), dim3(128), 0, ctx->stream,
-                     (const unsigned short*)m->wbf.p, feat, r, e, w, n, nb, off, stride,
-                     (float*)m->part.p, (double*)m->lpart.p);
+  const unsigned short* wbf = (const unsigned short*)m->wbf.p;
+  hipLaunchKernelGGL(k_mlp_pass<0>, dim3(nblk), dim3(128), 0, ctx->stream, wbf, feat, r, e, w, NB,
+                     Lb, nbb, off, stride, (double*)m->tsum.p, (const double*)nullptr,
+                     (float*)nullptr);
+  hipLaunchKernelGGL(k_mlp_band_sums, dim3(ceil_div(nbb, 256)), dim3(256), 0, ctx->stream,
+                     (const double*)m->tsum.p, nbb, tpb, bcoef, lossv);
+  hipLaunchKernelGGL(k_mlp_pass<1>, dim3(nblk), dim3(128), 0, ctx->stream, wbf, feat, r, e, w, NB,
+                     Lb, nbb, off, stride, (double*)nullptr, (const double*)bcoef,
+                     (float*)m->part.p);
   hipLaunchKernelGGL(k_mlp_reduce, dim3(ceil_div(kNP, 256)), dim3(256), 0, ctx->stream,
-                     (const float*)m->part.p, nblk, (const double*)m->lpart.p, (float*)m->grad.p,
+                     (const float*)m->part.p, nblk, (const double*)lossv, nbb, (float*)m->grad.p,
                      d_loss);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
@@ -554,7 +612,8 @@ int chs_mlp_create(chs_context* ctx, uint64_t seed, chs_mlp** out) {
 
 int chs_mlp_destroy(chs_mlp* m) {
   if (!m) return CHS_ERR_INVALID_ARG;
-  for (DevBuf* b : {&m->theta, &m->m, &m->v, &m->grad, &m->wbf, &m->part, &m->lpart}) release(*b);
+  for (DevBuf* b : {&m->theta, &m->m, &m->v, &m->grad, &m->wbf, &m->part, &m->tsum, &m->bcoef})
+    release(*b);
   delete m;
   return CHS_OK;
 }
@@ -587,11 +646,16 @@ int chs_mlp_get_params(chs_context* ctx, chs_mlp* m, float* theta) {
 }
 
 int chs_mlp_loss_grad(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
-                      const float* d_e, const float* d_w, int64_t n, double* loss, float* grad) {
-  if (!ctx || !m || !d_feat || !d_r || !d_e || !d_w || n <= 0) return CHS_ERR_INVALID_ARG;
+                      const float* d_e, const float* d_w, int64_t n, int band_len, double* loss,
+                      float* grad) {
+  if (!ctx || !m || !d_feat || !d_r || !d_e || !d_w || n <= 0 || band_len <= 0 ||
+      n % band_len != 0)
+    return CHS_ERR_INVALID_ARG;
+  const long long NB = n / band_len;
+  if (NB > (1 << 30)) return CHS_ERR_INVALID_ARG;
   CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
   double* dl = (double*)ctx->impl->small.p + 32;
-  CHS_TRY(mlp_batch(ctx, m, d_feat, d_r, d_e, d_w, n, n, 0, 1, dl));
+  CHS_TRY(mlp_batch(ctx, m, d_feat, d_r, d_e, d_w, NB, band_len, (int)NB, 0, 1, dl));
   double h = 0.0;
   CHS_HIP(ctx, hipMemcpyAsync(&h, dl, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   if (grad)
@@ -603,20 +667,20 @@ int chs_mlp_loss_grad(chs_context* ctx, chs_mlp* m, const float* d_feat, const f
 }
 
 int chs_mlp_train(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
-                  const float* d_e, const float* d_w, int64_t n, int steps, int64_t batch,
-                  float lr, double* loss_trace) {
-  if (!ctx || !m || !d_feat || !d_r || !d_e || !d_w || n <= 0 || steps < 0 || batch <= 0 ||
-      !(lr > 0))
+                  const float* d_e, const float* d_w, int64_t n, int band_len, int steps,
+                  int batch_bands, float lr, double* loss_trace) {
+  if (!ctx || !m || !d_feat || !d_r || !d_e || !d_w || n <= 0 || band_len <= 0 ||
+      n % band_len != 0 || steps < 0 || batch_bands <= 0 || !(lr > 0))
     return CHS_ERR_INVALID_ARG;
-  const long long nb = std::min<long long>(batch, n);
-  const long long stride = coprime_stride(n);
-  CHS_TRY(ensure(ctx, ctx->impl->small, 64 * sizeof(double)));
+  const long long NB = n / band_len;
+  const int nbb = (int)std::min<long long>(batch_bands, NB);
+  const long long stride = coprime_stride(NB);
   CHS_TRY(ensure(ctx, ctx->impl->mlp_ws, (size_t)std::max(steps, 1) * sizeof(double) + 256));
   double* dtrace = (double*)ctx->impl->mlp_ws.p;
   for (int s = 0; s < steps; ++s) {
-    const long long off = (long long)(((unsigned long long)s * (unsigned long long)nb) %
-                                      (unsigned long long)n) * stride % n;
-    CHS_TRY(mlp_batch(ctx, m, d_feat, d_r, d_e, d_w, n, nb, off, stride, dtrace + s));
+    const long long off = (long long)(((unsigned long long)s * (unsigned long long)nbb) %
+                                      (unsigned long long)NB) * stride % NB;
+    CHS_TRY(mlp_batch(ctx, m, d_feat, d_r, d_e, d_w, NB, band_len, nbb, off, stride, dtrace + s));
     m->t += 1;
     const float bc1 = 1.0f - std::pow(0.9f, (float)m->t), bc2 = 1.0f - std::pow(0.999f, (float)m->t);
     hipLaunchKernelGGL(k_mlp_adam, dim3(ceil_div(kNP, 256)), dim3(256), 0, ctx->stream,

@@ -272,24 +272,29 @@ int chs_build_system(chs_context* ctx, const chs_scf_params* p, const double* po
  * local potential; scale s = t(kin_g / E_b) exp(o) / E_b (t: the TPA
  * polynomial), so a zero-output network is per-band TPA. Parameters (FP32
  * master, CHS_MLP_PARAMS floats): W1[64][4], b1, W2[64][64], b2, W3[64][64],
- * b3, w4[64], b4. Training pairs: features (f32, n x 4), residual r and
- * correction e (interleaved complex f32, n), weight w (f32, n); loss
- * L = sum_i w_i |s_i r_i + e_i|^2, minimised with Adam. */
+ * b3, w4[64], b4. Training pairs, band by band (band_len consecutive pairs
+ * per band): features (f32, n x 4), residual r and correction e
+ * (interleaved complex f32, n), mask w (f32, n; bands with w <= 0 unused).
+ * Loss (band mean, scale-free per band): with a = -Re(conj(r) e),
+ * c = |r|^2, L = mean_b [1 - (sum s a)^2 / (sum s^2 c sum |e|^2)];
+ * minimised with Adam. */
 #define CHS_MLP_PARAMS 8705
 typedef struct chs_mlp chs_mlp;
 int chs_mlp_create(chs_context* ctx, uint64_t seed, chs_mlp** out);
 int chs_mlp_destroy(chs_mlp* m);
 int chs_mlp_set_params(chs_context* ctx, chs_mlp* m, const float* theta); /* resets Adam */
 int chs_mlp_get_params(chs_context* ctx, chs_mlp* m, float* theta);
-/* L and dL/dtheta (host, CHS_MLP_PARAMS floats, may be NULL) over all pairs */
+/* L and dL/dtheta (host, CHS_MLP_PARAMS floats, may be NULL) over all bands
+ * (n a multiple of band_len) */
 int chs_mlp_loss_grad(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
-                      const float* d_e, const float* d_w, int64_t n, double* loss, float* grad);
+                      const float* d_e, const float* d_w, int64_t n, int band_len, double* loss,
+                      float* grad);
 /* `steps` Adam steps (learning rate lr) on consecutive minibatches of
- * `batch` pairs of a fixed permutation; loss_trace (host, steps, may be
- * NULL) receives each minibatch's loss before its update. */
+ * `batch_bands` whole bands of a fixed permutation; loss_trace (host,
+ * steps, may be NULL) receives each minibatch's loss before its update. */
 int chs_mlp_train(chs_context* ctx, chs_mlp* m, const float* d_feat, const float* d_r,
-                  const float* d_e, const float* d_w, int64_t n, int steps, int64_t batch,
-                  float lr, double* loss_trace);
+                  const float* d_e, const float* d_w, int64_t n, int band_len, int steps,
+                  int batch_bands, float lr, double* loss_trace);
 /* Scale table (f32, n_b x n_pw) of a c128 block on the bound Hamiltonian,
  * for chs_band_cg's precond 2. */
 int chs_mlp_scale_table(chs_context* ctx, chs_mlp* m, const double* d_psi, int n_b,
@@ -300,8 +305,9 @@ int chs_mlp_scale_table(chs_context* ctx, chs_mlp* m, const double* d_psi, int n
  * psi^ component removed, where t is the normalised projection of psi^ on
  * the reference eigenspace of band b (reference eigenvalues closer than
  * gap_min form one eigenspace). Bands whose projection norm is below 0.9,
- * or whose eigenspace reaches the top of the block, get weight 0; the
- * others 1 / (n_used |e_b|^2). Outputs n_b x n_pw pairs, band-major. */
+ * or whose eigenspace reaches the top of the block, get w = 0; the others
+ * w = 1 / (n_used |e_b|^2). Outputs n_b x n_pw pairs, band-major (band
+ * length n_pw). */
 int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref,
                   const double* eps_ref, int n_b, double gap_min, float* d_feat, float* d_r,
                   float* d_e, float* d_w, int* n_used);

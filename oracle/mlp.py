@@ -16,8 +16,10 @@ specification it restates is the one the device kernels implement:
   preconditioner scale s = t(x) exp(o) / E_b with x = kin_g / E_b and t the
   Teter-Payne-Allan polynomial (uspp_preconditioner.rs:148-180), so a
   network whose output is 0 is the per-band TPA preconditioner;
-  loss over residual / correction pairs (r, e) with per-pair weight w:
-      L = sum_i w_i |s_i r_i + e_i|^2
+  loss over residual / correction pairs (r, e), band by band (band_len
+  consecutive pairs per band; bands whose mask w <= 0 are unused), with
+  a = -Re(conj(r) e), c = |r|^2:
+      L = (1/n_bands) sum_{used b} [1 - (sum s a)^2 / (sum s^2 c sum |e|^2)]
   and its gradient with respect to the FP32 master parameters (straight
   through the bfloat16 rounding).
 
@@ -129,13 +131,30 @@ def scale(theta, f):
     return tpa(x) * np.exp(o) / eb
 
 
-def loss_grad(theta, f, r, e, w):
-    """L = sum w |s r + e|^2 and dL/dtheta (FP64) for pairs (r, e)."""
+def band_loss(s, r, e, w, band_len):
+    """L and dL/ds of the band-mean misalignment loss for scales s."""
+    nb = len(s) // band_len
+    a = -np.real(np.conj(r) * e)
+    c = np.abs(r) ** 2
+    A = (s * a).reshape(nb, band_len).sum(1)
+    B = (s * s * c).reshape(nb, band_len).sum(1)
+    E = (np.abs(e) ** 2).reshape(nb, band_len).sum(1)
+    used = (w.reshape(nb, band_len)[:, 0] > 0) & (B > 0) & (E > 0)
+    Bs = np.where(used, B, 1.0)
+    Es = np.where(used, E, 1.0)
+    L = float(np.sum(np.where(used, 1.0 - A * A / (Bs * Es), 0.0)) / nb)
+    g1 = np.where(used, 2.0 * A / (Bs * Es), 0.0) / nb
+    g2 = np.where(used, A / Bs, 0.0)
+    dLds = np.repeat(g1, band_len) * (np.repeat(g2, band_len) * s * c - a)
+    return L, dLds
+
+
+def loss_grad(theta, f, r, e, w, band_len):
+    """Band loss L and dL/dtheta (FP64) for pairs (r, e)."""
     o, (p, z1, h1, z2, h2, z3, h3) = forward(theta, f)
     s = tpa(f[:, 0]) * np.exp(o) / np.exp(f[:, 1])
-    res = s * r + e
-    L = float(np.sum(w * np.abs(res) ** 2))
-    do = 2.0 * w * np.real(np.conj(r) * res) * s          # dL/do (ds/do = s)
+    L, dLds = band_loss(s, r, e, w, band_len)
+    do = dLds * s                                       # ds/do = s
     g = {"b4": np.array([do.sum()]), "w4": h3.T @ do}
     dz3 = np.outer(do, p["w4"]) * dsilu(z3)
     g["W3"] = dz3.T @ h2

@@ -20,27 +20,32 @@ def _theta(seed=5):
     return M.to_bf16(th).astype(np.float32)  # master == bf16 copy: same weights as the oracle
 
 
-def _pairs(ctx, n, seed=3):
+def _pairs(ctx, nb, band_len, seed=3):
     rng = np.random.default_rng(seed)
+    n = nb * band_len
     f = np.column_stack([rng.uniform(0, 8, n), rng.uniform(-1, 1, n), rng.uniform(-0.5, 0, n),
                          rng.uniform(0.1, 0.6, n)]).astype(np.float32)
     r = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
-    e = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
-    w = (rng.uniform(0.5, 1.5, n) / n).astype(np.float32)
+    e = (-0.3 * r + 0.2 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    w = np.repeat(rng.uniform(0.5, 1.5, nb), band_len).astype(np.float32)
+    if nb > 2:
+        w[band_len:2 * band_len] = 0.0  # an unused band
     p = {"feat": chs.Gpu.from_host(ctx, f), "r": chs.Gpu.from_host(ctx, r),
-         "e": chs.Gpu.from_host(ctx, e), "w": chs.Gpu.from_host(ctx, w), "n": n}
+         "e": chs.Gpu.from_host(ctx, e), "w": chs.Gpu.from_host(ctx, w), "n": n,
+         "band_len": band_len}
     return p, (f.astype(np.float64), r.astype(np.complex128), e.astype(np.complex128),
                w.astype(np.float64))
 
 
-@pytest.mark.parametrize("n", [1, 37, 5000])
-def test_loss_grad_vs_oracle(ctx, n):
+@pytest.mark.parametrize("nb,band_len", [(1, 5), (3, 37), (40, 129)])
+def test_loss_grad_vs_oracle(ctx, nb, band_len):
+    """Band tiles that end mid-tile, an unused band, one-sample bands."""
     net = chs.Mlp(ctx)
     th = _theta()
     net.params = th
-    p, (f, r, e, w) = _pairs(ctx, n)
+    p, (f, r, e, w) = _pairs(ctx, nb, band_len)
     L, g = net.loss_grad(p)
-    Lo, go = M.loss_grad(th.astype(np.float64), f, r, e, w)
+    Lo, go = M.loss_grad(th.astype(np.float64), f, r, e, w, band_len)
     assert abs(L - Lo) <= 2e-5 * abs(Lo)
     scale = np.abs(go).max()
     assert np.abs(g - go).max() <= 2e-4 * scale, np.abs(g - go).max() / scale
@@ -50,9 +55,9 @@ def test_adam_step_vs_oracle(ctx):
     net = chs.Mlp(ctx)
     th = _theta(7)
     net.params = th
-    p, _ = _pairs(ctx, 4096, 9)
+    p, _ = _pairs(ctx, 32, 128, 9)
     _, g = net.loss_grad(p)
-    net.train(p, 1, 4096, 1e-3)  # full batch: the gradient above
+    net.train(p, 1, 32, 1e-3)  # full batch: the gradient above
     got = net.params
     ref, _, _ = M.adam_step(th.astype(np.float64), np.zeros(M.N_PARAMS), np.zeros(M.N_PARAMS),
                             g.astype(np.float64), 1, 1e-3)
@@ -128,7 +133,7 @@ def test_training_gain_over_tpa(ctx):
     assert used > 0
     net = chs.Mlp(ctx, seed=11)
     L0, _ = net.loss_grad(pairs)
-    net.train(pairs, 1500, 8192, 2e-3)
+    net.train(pairs, 1500, 64, 1e-3)
     L1, _ = net.loss_grad(pairs)
     res = [learned.evaluate(ctx, learned.Sample(ctx, seed=s, n_bands=24), net)
            for s in (900, 901, 902)]
