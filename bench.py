@@ -1,25 +1,33 @@
 #!/usr/bin/env python3
 """SCF time-to-1e-6 Ha/atom on one MI355X (libchs_gpu.so).
 
-Workload (default): the 64-atom diamond-cubic cell (2x2x2 supercell of
-a = 10.26 bohr, atoms displaced by seeded N(0, 0.05 bohr)), V_loc(r) =
--4 erf(r)/r, Gamma point, 0.5|G|^2 <= 15 Ha (23,847 plane waves), 72^3
-grid, 160 bands, Chebyshev degree 12, complex128 throughout with FP64
-Cholesky-QR + Rayleigh-Ritz, Pulay(6)/Kerker mixing, Fermi-Dirac kT = 0.01
-Ha; converged when |dE|/N_atoms < 1e-6 Ha on two consecutive iterations.
+Workload (default, config C): the 216-atom diamond-cubic USPP cell (3x3x3
+supercell of a = 10.26 bohr, atoms displaced by seeded N(0, 0.05 bohr)),
+V_loc(r) = -4 erf(r)/r plus 4 projectors per atom (864) with Q = 0.25
+Gaussian augmentation, Gamma point, 0.5|G|^2 <= 15 Ha (80,725 plane waves),
+108^3 grid, 512 bands, Chebyshev degree 12, FP64 Cholesky-QR + Rayleigh-
+Ritz, Pulay(6)/Kerker mixing, Fermi-Dirac kT = 0.01 Ha; converged when
+|dE|/N_atoms < 1e-6 Ha on two consecutive iterations. Both forms are run:
+the complex64 filter (with the last SCF iterations in complex128) is
+`value`, the plain complex128 form is reported beside it. Config B (64
+atoms, local potential, 72^3, 160 bands) and A (8 atoms) are available
+with --config.
 
 One "step" is one full SCF run from the seeded start to convergence with
 the start wavefunctions already resident in HBM; `value` is the mean
 SCF-loop time of the K timed runs (lower is better).
 
 The JSON line also carries
-  roofline     - the dominant kernel of the Chebyshev H|psi> step, timed
-                 with HIP events on the context stream at the workload's
-                 shape, against MI355X HBM peak (8 TB/s), with the
-                 algorithmic bytes of the stick/plane design;
+  roofline     - the dominant kernel of one Chebyshev step (the three
+                 H|psi> kernels and, with projectors, the two projector
+                 GEMMs), timed with HIP events on the context stream at the
+                 workload's shape, against its MI355X peak (HBM 8 TB/s with
+                 the algorithmic bytes of the stick/plane design, or the
+                 dense FP32 / FP64 matrix peak for the GEMMs);
   cpu_baseline - the FP64 CPU oracle (oracle/) on this host: one SCF
-                 iteration of the same workload timed, extrapolated to the
-                 GPU run's iteration count.
+                 iteration timed and extrapolated (configs A, B), or a
+                 4-band H|psi> sample scaled to the run's band applications
+                 (config C, a lower bound).
 """
 import argparse
 import json
@@ -187,14 +195,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="B", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="C", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--precision", default="c128", choices=["c128", "c64"],
                     help="filter precision: c128 throughout, or complex64 filter with FP64 "
                          "Rayleigh-Ritz and the last SCF iterations in c128")
     ap.add_argument("--compare", action="store_true",
                     help="time both forms; value = the complex64-filter form, with the plain "
-                         "FP64 form's time and the ratio beside it")
+                         "FP64 form's time and the ratio beside it (the default for config C)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="config C: time only the form given by --precision")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only time one Chebyshev step's kernels (for counter runs) and print "
                          "the roofline")
@@ -209,6 +219,8 @@ def main():
     import chs
     ctx = chs.Context(0)
     cfg = CONFIGS[args.config]
+    if args.config == "C" and not args.no_compare:
+        args.compare = True
     precision = "c64" if args.compare else args.precision
     if args.hpsi_only:
         if args.copy_bytes:

@@ -133,7 +133,7 @@ def lib():
             "chs_kset_set_veff": [vp, vp],
             "chs_kset_select": [vp, i],
             "chs_kset_filter": [vp, i, vp, i, i, dp],
-            "chs_autotune_hpsi": [vp, i, i, C.POINTER(i), C.POINTER(i), dp],
+            "chs_autotune_hpsi": [vp, i, i, C.POINTER(i), C.POINTER(i), C.POINTER(i), dp],
             "chs_build_system": [vp, C.POINTER(ScfParams), dp, C.POINTER(i), C.POINTER(i), vp, vp,
                                  vp, vp, vp, vp],
             "chs_mlp_create": [vp, u64, C.POINTER(vp)],
@@ -611,11 +611,11 @@ class KSet:
 
 def autotune_hpsi(ctx, n_b, precision=C128):
     """Tune the H|psi> launch shape for the bound Hamiltonian; returns
-    (sticks per block, row-buffer divisor, step ms)."""
-    sb, rd, ms = C.c_int(), C.c_int(), C.c_double()
+    (sticks per block, row-buffer divisor, plane threads, step ms)."""
+    sb, rd, th, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
     ctx.check(lib().chs_autotune_hpsi(ctx.h, precision, n_b, C.byref(sb), C.byref(rd),
-                                      C.byref(ms)))
-    return sb.value, rd.value, ms.value
+                                      C.byref(th), C.byref(ms)))
+    return sb.value, rd.value, th.value, ms.value
 
 
 def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None):

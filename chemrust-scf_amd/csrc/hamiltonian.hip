@@ -104,8 +104,8 @@ __device__ __forceinline__ void expand_row(const cplx<T>* cmp_row, cplx<T>* rb, 
 }
 
 // --- B: plane pass with V_eff ----------------------------------------------
-template <int N, typename T>
-__global__ void __launch_bounds__(256) k_plane(cplx<T>* __restrict__ sbuf, StickArgs st,
+template <int N, typename T, int TPB>
+__global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, StickArgs st,
                                                const T* __restrict__ veff, int nrb,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
@@ -375,20 +375,22 @@ static int max_lds_per_block(chs_context* ctx) {
 struct PlaneLaunch {
   size_t lds = 0;
   int nrb = 0;
+  int threads = 256;
 };
 static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz, int n2,
-                        int nrb_div, PlaneLaunch* out) {
+                        int nrb_div, int threads, PlaneLaunch* out) {
   const size_t cmp_bytes = (size_t)n * sa.rs * csz;
   const size_t row_bytes = (size_t)(n + 1) * csz;
   const int lim = max_lds_per_block(ctx);
   if (cmp_bytes + row_bytes + (size_t)n * sizeof(int32_t) > (size_t)lim)
     return set_error(ctx, CHS_ERR_FFT, "plane of %d^3 grid does not fit in LDS (%zu > %d bytes)",
                      n, cmp_bytes + row_bytes, lim);
-  const int groups = 4 * (64 / n2);  // 256 threads
+  const int groups = (threads / 64) * (64 / n2);
   const size_t tab_bytes = (size_t)n * sizeof(int32_t);  // act_of
   int nrb = (int)((lim - cmp_bytes - tab_bytes) / row_bytes);
   nrb = std::max(1, std::min(nrb, groups) / std::max(1, nrb_div));
   out->nrb = nrb;
+  out->threads = threads;
   out->lds = cmp_bytes + (size_t)nrb * row_bytes + tab_bytes;
   return CHS_OK;
 }
@@ -410,8 +412,12 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
   hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
                      sbuf, tw);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-  hipLaunchKernelGGL((k_plane<NN, T>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf, sa,
-                     veff, pl.nrb, tw);
+  if (pl.threads == 512)
+    hipLaunchKernelGGL((k_plane<NN, T, 512>), dim3(NN, n_b), dim3(512), pl.lds, ctx->stream, sbuf,
+                       sa, veff, pl.nrb, tw);
+  else
+    hipLaunchKernelGGL((k_plane<NN, T, 256>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf,
+                       sa, veff, pl.nrb, tw);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
 #define CHS_FWD(M)                                                                            \
   hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \
@@ -460,7 +466,7 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   for (int r = 0; r < std::max(1, reps); ++r) {
     CHS_DISPATCH_N(n, {
       PlaneLaunch pl;
-      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, &pl));
+      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, tn.threads, &pl));
       hipEvent_t* evp = ms ? ev : nullptr;
       if (tn.sb == 16)
         CHS_TRY((vloc_launch<NN, T, 16>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
@@ -511,7 +517,8 @@ int vloc_timed(chs_context* ctx, int precision, const void* psi, int n_b, void* 
 }
 
 // Launch-shape autotuner of the H|psi> pipeline for the bound grid: every
-// (sticks per block, row-buffer divisor) candidate that fits is timed on
+// (sticks per block, plane-kernel threads, row-buffer divisor) candidate
+// that fits is timed on
 // n_b bands at the given precision (HIP events, `reps` runs after one
 // warm-up) and the fastest is kept for (grid, precision).
 int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
@@ -524,17 +531,19 @@ int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, voi
   double best_t = 1e300;
   for (int sb : {16, 32, 64}) {
     if (!sb_fits(n, csz, sb)) continue;
-    for (int div : {1, 2}) {
-      ctx->impl->vloc_tune[key] = VlocTune{sb, div};
-      double ms[3];
-      CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, 1, ms));
-      CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, reps, ms));
-      const double t = ms[0] + ms[1] + ms[2];
-      if (t < best_t) {
-        best_t = t;
-        best = VlocTune{sb, div};
+    for (int threads : {256, 512})
+      for (int div : {1, 2, 4}) {
+        const VlocTune cand{sb, div, threads};
+        ctx->impl->vloc_tune[key] = cand;
+        double ms[3];
+        CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, 1, ms));
+        CHS_TRY(vloc_timed(ctx, precision, psi, n_b, out, reps, ms));
+        const double t = ms[0] + ms[1] + ms[2];
+        if (t < best_t) {
+          best_t = t;
+          best = cand;
+        }
       }
-    }
   }
   ctx->impl->vloc_tune[key] = best;
   if (best_out) *best_out = best;
@@ -575,7 +584,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   const int gz = ceil_div(st.n_st, SB);
   CHS_DISPATCH_N(n, {
     PlaneLaunch pl;
-    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, &pl));
+    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, 256, &pl));
     hipLaunchKernelGGL((k_stick_inv<NN, double, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);

@@ -74,11 +74,12 @@ struct GemmKey {
   }
 };
 
-// Launch shape of the H|psi> pipeline: sticks per block of the z passes and
-// the divisor of the plane kernel's row-buffer count.
+// Launch shape of the H|psi> pipeline: sticks per block of the z passes, the
+// divisor of the plane kernel's row-buffer count and its workgroup size.
 struct VlocTune {
   int sb = 32;
   int nrb_div = 1;
+  int threads = 256;  // plane kernel workgroup size (256 or 512)
 };
 
 }  // namespace chs
@@ -86,6 +87,7 @@ struct VlocTune {
 struct chs_context::Impl {
   std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
   std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
+  bool hpsi_autotune = true;  // chs_scf_run tunes new grids (CHS_HPSI_TUNE=0: off)
   chs::Hamiltonian ham;
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;

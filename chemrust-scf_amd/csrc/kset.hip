@@ -200,7 +200,7 @@ int chs_kset_filter(chs_context* ctx, int precision, void* const* d_psi, int n_b
 }
 
 int chs_autotune_hpsi(chs_context* ctx, int precision, int n_b, int* sticks_per_block,
-                      int* row_div, double* ms) {
+                      int* row_div, int* plane_threads, double* ms) {
   if (!ctx || n_b <= 0 || (precision != CHS_C128 && precision != CHS_C64))
     return CHS_ERR_INVALID_ARG;
   auto* im = ctx->impl;
@@ -217,6 +217,7 @@ int chs_autotune_hpsi(chs_context* ctx, int precision, int n_b, int* sticks_per_
   im->tune_gen += 1;
   if (sticks_per_block) *sticks_per_block = best.sb;
   if (row_div) *row_div = best.nrb_div;
+  if (plane_threads) *plane_threads = best.threads;
   if (ms) *ms = t;
   return CHS_OK;
 }

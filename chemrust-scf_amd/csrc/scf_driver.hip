@@ -138,6 +138,14 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     CHS_TRY(chs_bind_projectors(ctx, P, (const double*)S.beta.p, (const double*)S.dscr.p,
                                 (const double*)S.q.p));
   }
+  // launch shapes of the H|psi> kernels, tuned once per (grid, precision)
+  // and context (later runs reuse them)
+  if (ctx->impl->hpsi_autotune)
+    for (int prec : {CHS_C128, CHS_C64}) {
+      if (prec == CHS_C64 && !mixed) continue;
+      if (ctx->impl->vloc_tune.count(std::make_pair(n, prec))) continue;
+      CHS_TRY(chs_autotune_hpsi(ctx, prec, n_b, nullptr, nullptr, nullptr, nullptr));
+    }
   CHS_TRY(ensure(ctx, S.occ, (size_t)n_b * sizeof(double)));
   PulayDev mixer;
   CHS_TRY(mixer.init(ctx, &G, std::max(1, p->pulay_depth), p->mix_alpha, p->kerker_q));

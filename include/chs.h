@@ -334,11 +334,12 @@ int chs_kset_select(chs_context* ctx, int k);
 int chs_kset_filter(chs_context* ctx, int precision, void* const* d_psi, int n_b, int degree,
                     const double* bounds4);
 /* Launch-shape autotuner of the H|psi> kernels for the bound Hamiltonian
- * and precision (sticks per block of the z passes, plane row-buffer
- * divisor): every candidate timed on n_b bands, the fastest kept for this
- * grid and precision. Outputs the choice and its step time (ms). */
+ * and precision (sticks per block of the z passes, plane-kernel workgroup
+ * size and row-buffer divisor): every candidate timed on n_b bands, the
+ * fastest kept for this grid and precision (chs_scf_run tunes on first use
+ * of a grid). Outputs the choice and its step time (ms). */
 int chs_autotune_hpsi(chs_context* ctx, int precision, int n_b, int* sticks_per_block,
-                      int* row_div, double* ms);
+                      int* row_div, int* plane_threads, double* ms);
 
 #ifdef __cplusplus
 }

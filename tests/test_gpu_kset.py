@@ -79,7 +79,7 @@ def test_autotune_keeps_results(ctx):
     psi = m.initial_psi(20)
     b4 = O.spectral_bounds(None, m.lanczos_bound(10, 4)[0])
     before = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 8, b4).sync_to_host()
-    sb, rd, ms = chs.autotune_hpsi(ctx, 20)
-    assert sb in (16, 32, 64) and rd in (1, 2) and ms > 0
+    sb, rd, th, ms = chs.autotune_hpsi(ctx, 20)
+    assert sb in (16, 32, 64) and rd in (1, 2, 4) and th in (256, 512) and ms > 0
     after = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 8, b4).sync_to_host()
     assert np.abs(after - before).max() / np.abs(before).max() < 1e-13
