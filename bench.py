@@ -169,6 +169,60 @@ def cpu_baseline(cfg, n_iter_gpu, n_pass_gpu):
     }
 
 
+def learned_report(chs, ctx):
+    """The learned band-CG preconditioner: trained on the GPU from plain-run
+    pairs of 12 seeded 8-atom samples (36^3, 24 bands), judged on 3 held-out
+    samples by the band loss (relative error left by one optimally scaled
+    preconditioned step) against per-band TPA (the untrained network) and
+    the reference TPA, plus band-CG steps from a common start block."""
+    import numpy as np
+    from chs import learned
+    t0 = time.perf_counter()
+    train = [learned.Sample(ctx, seed=100 + i, n_bands=24) for i in range(12)]
+    held = [learned.Sample(ctx, seed=900 + i, n_bands=24) for i in range(3)]
+    npw = train[0].H.n_pw
+    pairs = chs.mlp_pair_buffers(ctx, 12 * 2 * 3 * 24 * npw)
+    vpairs = chs.mlp_pair_buffers(ctx, 3 * 2 * 3 * 24 * npw)
+    learned.collect_pairs(ctx, train, pairs)
+    learned.collect_pairs(ctx, held, vpairs)
+    t1 = time.perf_counter()
+    net = chs.Mlp(ctx, seed=11)
+    v0, _ = net.loss_grad(vpairs)
+    ctx.synchronize()
+    t2 = time.perf_counter()
+    net.train(pairs, 2000, 64, 1e-3)
+    ctx.synchronize()
+    t3 = time.perf_counter()
+    v1, _ = net.loss_grad(vpairs)
+    # reference TPA on the same held-out pairs (x = kin / mean plane-wave kinetic energy)
+    n, bl = vpairs["n"], vpairs["band_len"]
+    f = vpairs["feat"].sync_to_host()[:n].astype(np.float64)
+    r = vpairs["r"].sync_to_host()[:n].astype(np.complex128)
+    e = vpairs["e"].sync_to_host()[:n].astype(np.complex128)
+    w = vpairs["w"].sync_to_host()[:n]
+    kin = f[:, 0] * np.exp(f[:, 1])
+    x = kin / kin[:bl].mean()
+    t = (27 + 18 * x + 12 * x ** 2 + 8 * x ** 3) / (27 + 18 * x + 12 * x ** 2 + 8 * x ** 3 + 16 * x ** 4)
+    a = -np.real(np.conj(r) * e)
+    nb = n // bl
+    A = (t * a).reshape(nb, bl).sum(1)
+    B = (t * t * np.abs(r) ** 2).reshape(nb, bl).sum(1)
+    E = (np.abs(e) ** 2).reshape(nb, bl).sum(1)
+    used = (w.reshape(nb, bl)[:, 0] > 0) & (B > 0) & (E > 0)
+    vt = float(np.sum(1 - A[used] ** 2 / (B[used] * E[used])) / nb)
+    cg = [learned.evaluate(ctx, s, net) for s in held]
+    return {
+        "held_out_band_loss": {"reference_tpa": round(vt, 5), "per_band_tpa": round(v0, 5),
+                               "trained": round(v1, 5)},
+        "band_cg_steps_from_common_start": {
+            "tpa": sum(c["tpa"]["steps"] for c in cg), "mlp": sum(c["mlp"]["steps"] for c in cg),
+            "converged_bands": [sum(c["tpa"]["converged"] for c in cg),
+                                sum(c["mlp"]["converged"] for c in cg)]},
+        "train_pairs": pairs["n"], "train_steps": 2000,
+        "seconds": {"pairs": round(t1 - t0, 2), "train": round(t3 - t2, 2)},
+    }
+
+
 def run_form(chs, ctx, cfg, precision, steps, warmup):
     """Warm-up and timed SCF runs of one precision form; returns the runs."""
     pos = chs.synth_positions(cfg["n_rep"])
@@ -205,6 +259,8 @@ def main():
                          "FP64 form's time and the ratio beside it (the default for config C)")
     ap.add_argument("--no-compare", action="store_true",
                     help="config C: time only the form given by --precision")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-preconditioner report")
     ap.add_argument("--hpsi-only", action="store_true",
                     help="only time one Chebyshev step's kernels (for counter runs) and print "
                          "the roofline")
@@ -264,6 +320,8 @@ def main():
         line["config"]["speedup_vs_fp64"] = round(v64 / value, 3)
         line["config"]["energy_diff_Ha_per_atom"] = abs(ref[-1]["energy"] - last["energy"]) / n_atoms
     line["roofline"] = roofline(chs, ctx, cfg, precision)
+    if args.compare and not args.no_learned:
+        line["learned_preconditioner"] = learned_report(chs, ctx)
     if not args.no_cpu:
         n_pass = last["n_iter"] + 3
         line["cpu_baseline"] = cpu_baseline(cfg, last["n_iter"], n_pass)
