@@ -305,7 +305,8 @@ def main():
         "config": {"workload": WORKLOAD[args.config], "atoms": n_atoms,
                    "n_pw": last["n_pw"], "grid": cfg["n_grid"], "bands": cfg["n_bands"],
                    "projectors": last["n_proj"], "filter_precision": precision,
-                   "scf_iterations": last["n_iter"], "energy_Ha": last["energy"],
+                   "scf_iterations": last["n_iter"],
+                   "scf_iterations_fp32_filter": last["n_iter_fp32"], "energy_Ha": last["energy"],
                    "phase_s": {k: round(last[k], 4) for k in
                                ("t_filter", "t_rr", "t_density", "t_veff", "t_lanczos")}},
     }

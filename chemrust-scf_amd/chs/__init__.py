@@ -67,6 +67,7 @@ class ScfParams(C.Structure):
         ("mix_alpha", C.c_double), ("kerker_q", C.c_double), ("etol", C.c_double),
         ("smear_kt", C.c_double),
         ("max_iter", C.c_int32), ("precision", C.c_int32), ("fp64_final", C.c_int32),
+        ("fp32_until", C.c_double),
     ]
 
 
@@ -74,7 +75,8 @@ class ScfResult(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("converged", C.c_int32), ("n_pw", C.c_int32),
                 ("n_proj", C.c_int32), ("energy", C.c_double), ("mu", C.c_double),
                 ("seconds", C.c_double), ("t_filter", C.c_double), ("t_rr", C.c_double),
-                ("t_density", C.c_double), ("t_veff", C.c_double), ("t_lanczos", C.c_double)]
+                ("t_density", C.c_double), ("t_veff", C.c_double), ("t_lanczos", C.c_double),
+                ("n_iter_fp32", C.c_int32)]
 
 
 _LIB = None
@@ -432,7 +434,7 @@ def scf_params(**kw):
              n_proj_per_atom=0, beta_rc=1.3, q_rc=1.2, q0=0.0, d_min=0.5, d_max=2.0, seed_d=3,
              seed_psi=2, seed_lanczos=4, cheb_degree=10, first_passes=4, lanczos_k=10,
              pulay_depth=6, mix_alpha=0.5, kerker_q=0.8, etol=1e-6, smear_kt=0.01,
-             max_iter=60, precision=C128, fp64_final=3)
+             max_iter=60, precision=C128, fp64_final=3, fp32_until=0.0)
     d.update(kw)
     for k, v in d.items():
         if k == "kfrac":

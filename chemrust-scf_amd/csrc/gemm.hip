@@ -259,56 +259,145 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
 
 // ---------------------------------------------------------------------------
 // n = 1 (matrix-vector) in the memory-bound form: y = alpha op(A) x + beta y.
+// Both forms cut K into slices (blockIdx.y) so that a few thousand
+// wavefronts stream A, with four independent loads in flight per lane;
+// with more than one slice each writes a partial vector and k_gemv_fin adds
+// them in slice order (deterministic).
 // op N: one thread per row, A read down its columns (coalesced across rows);
 // op T / C: one wavefront per output element, a reduction along the column.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) k_gemv_n(int m, int k, cplx<T> alpha,
+__device__ __forceinline__ void cmac(cplx<T> a, cplx<T> b, T& re, T& im) {
+  re = fma(a.x, b.x, re);
+  re = fma(-a.y, b.y, re);
+  im = fma(a.x, b.y, im);
+  im = fma(a.y, b.x, im);
+}
+
+template <typename T>
+__device__ __forceinline__ void gemv_out(cplx<T> acc, cplx<T> alpha, cplx<T> beta,
+                                         cplx<T>* __restrict__ y, cplx<T>* __restrict__ part,
+                                         int m, int i) {
+  if (part) {
+    part[(size_t)blockIdx.y * m + i] = acc;
+  } else {
+    cplx<T> v = alpha * acc;
+    if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * y[i];
+    y[i] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemv_n(int m, int k, int klen, cplx<T> alpha,
                                                 const cplx<T>* __restrict__ A, int lda,
                                                 const cplx<T>* __restrict__ x, cplx<T> beta,
-                                                cplx<T>* __restrict__ y) {
+                                                cplx<T>* __restrict__ y,
+                                                cplx<T>* __restrict__ part) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  T re = 0, im = 0;
-  for (int kk = 0; kk < k; ++kk) {
-    const cplx<T> a = A[(size_t)i + (size_t)kk * lda], b = x[kk];
-    re = fma(a.x, b.x, re);
-    re = fma(-a.y, b.y, re);
-    im = fma(a.x, b.y, im);
-    im = fma(a.y, b.x, im);
+  const int kb = blockIdx.y * klen, ke = min(k, kb + klen);
+  T re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+  int kk = kb;
+  for (; kk + 4 <= ke; kk += 4) {
+    cplx<T> a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = A[(size_t)i + (size_t)(kk + u) * lda];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cmac(a[u], x[kk + u], re[u], im[u]);
   }
-  cplx<T> v = alpha * mk(re, im);
+  for (; kk < ke; ++kk) cmac(A[(size_t)i + (size_t)kk * lda], x[kk], re[0], im[0]);
+  gemv_out(mk((re[0] + re[1]) + (re[2] + re[3]), (im[0] + im[1]) + (im[2] + im[3])), alpha, beta,
+           y, part, m, i);
+}
+
+template <typename T, int OPA>
+__global__ void __launch_bounds__(256) k_gemv_t(int m, int k, int klen, cplx<T> alpha,
+                                                const cplx<T>* __restrict__ A, int lda,
+                                                const cplx<T>* __restrict__ x, cplx<T> beta,
+                                                cplx<T>* __restrict__ y,
+                                                cplx<T>* __restrict__ part) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= m) return;
+  const int kb = blockIdx.y * klen, ke = min(k, kb + klen);
+  const cplx<T>* col = A + (size_t)i * lda;
+  T re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
+  int kk = kb + lane;
+  for (; kk + 192 < ke; kk += 256) {
+    cplx<T> a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = col[kk + 64 * u];
+      b[u] = x[kk + 64 * u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (OPA == 2) a[u].y = -a[u].y;
+      cmac(a[u], b[u], re[u], im[u]);
+    }
+  }
+  for (; kk < ke; kk += 64) {
+    cplx<T> a = col[kk];
+    if (OPA == 2) a.y = -a.y;
+    cmac(a, x[kk], re[0], im[0]);
+  }
+  T sr = (re[0] + re[1]) + (re[2] + re[3]), si = (im[0] + im[1]) + (im[2] + im[3]);
+  for (int off = 32; off > 0; off >>= 1) {
+    sr += __shfl_down(sr, off);
+    si += __shfl_down(si, off);
+  }
+  if (lane == 0) gemv_out(mk(sr, si), alpha, beta, y, part, m, i);
+}
+
+template <typename T>
+__global__ void k_gemv_fin(int m, int ns, const cplx<T>* __restrict__ part, cplx<T> alpha,
+                           cplx<T> beta, cplx<T>* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  cplx<T> s = part[i];
+  for (int z = 1; z < ns; ++z) s = s + part[(size_t)z * m + i];
+  cplx<T> v = alpha * s;
   if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * y[i];
   y[i] = v;
 }
 
-template <typename T, int OPA>
-__global__ void __launch_bounds__(256) k_gemv_t(int m, int k, cplx<T> alpha,
-                                                const cplx<T>* __restrict__ A, int lda,
-                                                const cplx<T>* __restrict__ x, cplx<T> beta,
-                                                cplx<T>* __restrict__ y) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (i >= m) return;
-  const cplx<T>* col = A + (size_t)i * lda;
-  T re = 0, im = 0;
-  for (int kk = lane; kk < k; kk += 64) {
-    cplx<T> a = col[kk];
-    if (OPA == 2) a.y = -a.y;
-    const cplx<T> b = x[kk];
-    re = fma(a.x, b.x, re);
-    re = fma(-a.y, b.y, re);
-    im = fma(a.x, b.y, im);
-    im = fma(a.y, b.x, im);
+// K slices of a matrix-vector product with `waves` wavefronts per slice:
+// about 4096 wavefronts in all, slices of at least `min_len` elements.
+static void gemv_plan(int k, int waves, int min_len, int* ns, int* klen) {
+  int s = std::max(1, std::min((4096 + waves - 1) / waves, k / min_len));
+  int kl = (k + s - 1) / s;
+  *klen = std::max(1, kl);
+  *ns = (k + *klen - 1) / *klen;
+}
+
+template <typename T>
+static int gemv_launch(chs_context* ctx, int oa, int m, int k, cplx<T> alpha, const cplx<T>* A,
+                       int lda, const cplx<T>* x, cplx<T> beta, cplx<T>* y) {
+  int ns, klen;
+  if (oa == 0)
+    gemv_plan(k, (int)ceil_div(m, 64), 16, &ns, &klen);
+  else
+    gemv_plan(k, m, 512, &ns, &klen);
+  cplx<T>* part = nullptr;
+  if (ns > 1) {
+    CHS_TRY(ensure(ctx, ctx->impl->gemv_ws, (size_t)ns * m * sizeof(cplx<T>)));
+    part = (cplx<T>*)ctx->impl->gemv_ws.p;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    re += __shfl_down(re, off);
-    im += __shfl_down(im, off);
+  if (oa == 0)
+    hipLaunchKernelGGL((k_gemv_n<T>), dim3(ceil_div(m, 256), ns), dim3(256), 0, ctx->stream, m,
+                       k, klen, alpha, A, lda, x, beta, y, part);
+  else if (oa == 1)
+    hipLaunchKernelGGL((k_gemv_t<T, 1>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
+                       k, klen, alpha, A, lda, x, beta, y, part);
+  else
+    hipLaunchKernelGGL((k_gemv_t<T, 2>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
+                       k, klen, alpha, A, lda, x, beta, y, part);
+  CHS_LAUNCH_CHECK(ctx);
+  if (ns > 1) {
+    hipLaunchKernelGGL((k_gemv_fin<T>), dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, m, ns,
+                       (const cplx<T>*)part, alpha, beta, y);
+    CHS_LAUNCH_CHECK(ctx);
   }
-  if (lane == 0) {
-    cplx<T> v = alpha * mk(re, im);
-    if (beta.x != (T)0 || beta.y != (T)0) v = v + beta * y[i];
-    y[i] = v;
-  }
+  return CHS_OK;
 }
 
 template <typename T>
@@ -333,12 +422,28 @@ int gemm_opcode(char op) {
   }
 }
 
-// Split-K plan of a tile shape: enough blocks to fill the device when the
-// output tiles alone cannot.
-static void split_plan(int m, int n, int k, int bm, int bn, int bk, int* ksplit, int* klen) {
+// Split-K plan of a tile shape: when the output tiles alone cannot fill the
+// device, K is cut into slices of whole K tiles until about `target`
+// workgroups are in flight (more resident waves per CU hide the operand
+// latency of the long-K products better; the autotuner picks the target).
+static constexpr int kSplitTargets[3] = {512, 2048, 4096};
+
+template <typename T>
+static void tile_dims(int variant, int* bm, int* bn, int* bk) {
+  *bm = GBM; *bn = GBN; *bk = GBK;
+  if (variant == 1) {
+    *bm = MfmaTile<T, 1>::BM; *bn = MfmaTile<T, 1>::BN; *bk = MfmaTile<T, 1>::BK;
+  } else if (variant == 2) {
+    *bm = MfmaTile<T, 2>::BM; *bn = MfmaTile<T, 2>::BN; *bk = MfmaTile<T, 2>::BK;
+  }
+}
+
+static void split_plan(int m, int n, int k, int bm, int bn, int bk, int target, int* ksplit,
+                       int* klen) {
   const int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
   int ks = 1;
-  if (tiles < 512 && k > 4 * bk) ks = std::max(1, std::min(512 / tiles + 1, (k + 255) / 256));
+  if (tiles < target && k > 4 * bk)
+    ks = std::max(1, std::min(target / tiles + 1, (k + 255) / 256));
   int kl = (k + ks - 1) / ks;
   kl = ((kl + bk - 1) / bk) * bk;
   if (kl == 0) kl = bk;
@@ -347,17 +452,14 @@ static void split_plan(int m, int n, int k, int bm, int bn, int bk, int* ksplit,
 }
 
 template <typename T>
-static int gemm_launch(chs_context* ctx, int variant, int oa, int ob, int m, int n, int k,
+static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int n, int k,
                        cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
                        cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws) {
-  int bm = GBM, bn = GBN, bk = GBK;
-  if (variant == 1) {
-    bm = MfmaTile<T, 1>::BM; bn = MfmaTile<T, 1>::BN; bk = MfmaTile<T, 1>::BK;
-  } else if (variant == 2) {
-    bm = MfmaTile<T, 2>::BM; bn = MfmaTile<T, 2>::BN; bk = MfmaTile<T, 2>::BK;
-  }
+  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % 3];
+  int bm, bn, bk;
+  tile_dims<T>(variant, &bm, &bn, &bk);
   int ksplit, klen;
-  split_plan(m, n, k, bm, bn, bk, &ksplit, &klen);
+  split_plan(m, n, k, bm, bn, bk, target, &ksplit, &klen);
   cplx<T>* part = nullptr;
   if (ksplit > 1) {
     CHS_TRY(ensure(ctx, ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
@@ -391,25 +493,26 @@ static int gemm_launch(chs_context* ctx, int variant, int oa, int ob, int m, int
   return CHS_OK;
 }
 
-// Kernel variant for a GEMM shape: 0 vector-FMA 64 x 64, 1 MFMA 64 x 64,
-// 2 MFMA 128-wide. Chosen once per (precision, ops, m, n, k) by timing every
-// variant into a scratch output (HIP events, context stream); outside a
-// stream capture only, otherwise the MFMA 64 x 64 default.
+// Kernel choice for a GEMM shape: variant (low 4 bits: 0 vector-FMA 64 x 64,
+// 1 MFMA 64 x 64, 2 MFMA 128-wide) and split-K target (bits 4-5, index into
+// kSplitTargets). Chosen once per (precision, ops, m, n, k) by timing every
+// distinct launch into a scratch output (HIP events, context stream);
+// outside a stream capture only, otherwise the MFMA 64 x 64 default.
 template <typename T>
 static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, const cplx<T>* A,
-                        int lda, const cplx<T>* B, int ldb, int* variant) {
+                        int lda, const cplx<T>* B, int ldb, int* choice) {
   auto* im = ctx->impl;
   if (!im->use_mfma) {
-    *variant = 0;
+    *choice = 0;
     return CHS_OK;
   }
   const GemmKey key{(int)sizeof(T), oa, ob, m, n, k};
   auto it = im->gemm_tune.find(key);
   if (it != im->gemm_tune.end()) {
-    *variant = it->second;
+    *choice = it->second;
     return CHS_OK;
   }
-  *variant = 1;
+  *choice = 1;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   CHS_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
   if (cs != hipStreamCaptureStatusNone || !im->gemm_autotune) return CHS_OK;
@@ -418,21 +521,30 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   const cplx<T> one = mk((T)1, (T)0), zero = mk((T)0, (T)0);
   double best = 1e300;
   for (int v = 0; v < 3; ++v) {
-    CHS_TRY(gemm_launch<T>(ctx, v, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
-                           im->gemm_tune_ws));  // warm-up, workspace growth
-    CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
-    CHS_TRY(gemm_launch<T>(ctx, v, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
-                           im->gemm_tune_ws));
-    CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
-    CHS_HIP(ctx, hipEventSynchronize(im->ev1));
-    float t = 0;
-    CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
-    if (t < best) {
-      best = t;
-      *variant = v;
+    int prev_split = -1;
+    for (int ti = 0; ti < 3; ++ti) {
+      int bm, bn, bk, ks, kl;
+      tile_dims<T>(v, &bm, &bn, &bk);
+      split_plan(m, n, k, bm, bn, bk, kSplitTargets[ti], &ks, &kl);
+      if (ks == prev_split) continue;  // same launch as a smaller target
+      prev_split = ks;
+      const int c = v | (ti << 4);
+      CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
+                             im->gemm_tune_ws));  // warm-up, workspace growth
+      CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+      CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
+                             im->gemm_tune_ws));
+      CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+      CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+      float t = 0;
+      CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+      if (t < best) {
+        best = t;
+        *choice = c;
+      }
     }
   }
-  im->gemm_tune[key] = *variant;
+  im->gemm_tune[key] = *choice;
   return CHS_OK;
 }
 
@@ -443,22 +555,11 @@ static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cpl
   const int oa = gemm_opcode(opa), ob = gemm_opcode(opb);
   if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0) return CHS_ERR_INVALID_ARG;
   if (m == 0 || n == 0) return CHS_OK;
-  if (n == 1 && ob == 0 && k > 0) {  // matrix-vector
-    if (oa == 0)
-      hipLaunchKernelGGL((k_gemv_n<T>), dim3(ceil_div(m, 256)), dim3(256), 0, ctx->stream, m, k,
-                         alpha, A, lda, B, beta, C);
-    else if (oa == 1)
-      hipLaunchKernelGGL((k_gemv_t<T, 1>), dim3(ceil_div(m, 4)), dim3(256), 0, ctx->stream, m, k,
-                         alpha, A, lda, B, beta, C);
-    else
-      hipLaunchKernelGGL((k_gemv_t<T, 2>), dim3(ceil_div(m, 4)), dim3(256), 0, ctx->stream, m, k,
-                         alpha, A, lda, B, beta, C);
-    CHS_LAUNCH_CHECK(ctx);
-    return CHS_OK;
-  }
-  int variant = 1;
-  CHS_TRY(gemm_variant<T>(ctx, oa, ob, m, n, k, A, lda, B, ldb, &variant));
-  return gemm_launch<T>(ctx, variant, oa, ob, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+  if (n == 1 && ob == 0 && k > 0)  // matrix-vector
+    return gemv_launch<T>(ctx, oa, m, k, alpha, A, lda, B, beta, C);
+  int choice = 1;
+  CHS_TRY(gemm_variant<T>(ctx, oa, ob, m, n, k, A, lda, B, ldb, &choice));
+  return gemm_launch<T>(ctx, choice, oa, ob, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                         ctx->impl->gemm_ws);
 }
 

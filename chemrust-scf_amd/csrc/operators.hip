@@ -6,6 +6,7 @@
 
 #include "blas1.hpp"
 #include "impl.hpp"
+#include "linalg.hpp"
 #include "ops.hpp"
 
 namespace chs {
@@ -194,59 +195,106 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
 }
 
 // ---- Lanczos -----------------------------------------------------------
-// f = w - alpha v - beta vp ; sf = u - alpha sv - beta svp  (S = I: sv = v)
-__global__ void k_lanczos_f(const z64* __restrict__ w, const z64* __restrict__ u,
-                            const z64* __restrict__ v, const z64* __restrict__ sv,
-                            const z64* __restrict__ vp, const z64* __restrict__ svp,
-                            const double* __restrict__ ab, int j, int gen, size_t n,
-                            z64* __restrict__ f, z64* __restrict__ sf) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// The bound runs on S^{-1} H in the S inner product with the projector
+// coefficients w_x = beta^H x of every Lanczos vector carried along, so a
+// step needs one local H|v>, one beta^H product and one beta product (the
+// Chebyshev step's coefficient-space form):
+//   g = beta^H Hloc v,  y = beta^H H v = g + B D w_v,
+//   c = D w_v - M y  (c = D w_v without S),  S^{-1} H v = Hloc v + beta c,
+//   beta^H S^{-1} H v = g + B c,
+//   alpha = Re<v|Hloc v> + sum_p D_p |w_v,p|^2,
+//   f = S^{-1} H v - alpha v - beta_{j-1} v_prev (same recurrence on w),
+//   beta_j^2 = <f|f> + sum_p Q_p |w_f,p|^2.
+// B = beta^H beta and M = (Q^{-1} + B)^{-1} are Hermitian, so their rows are
+// read as conjugated columns.
+
+// out[p] = a[p] + s sum_q Mat[p][q] x[q], x[q] = (dx ? dx[q] : 1) xin[q],
+// a[p] = (da ? da[p] : 1) ain[p] (0 without ain); no sum without Mat.
+// One wavefront per row.
+__global__ void __launch_bounds__(256) k_herm_mv(int P, const z64* __restrict__ Mat,
+                                                 const z64* __restrict__ xin,
+                                                 const double* __restrict__ dx,
+                                                 const z64* __restrict__ ain,
+                                                 const double* __restrict__ da, double s,
+                                                 z64* __restrict__ out) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (p >= P) return;
+  double re = 0, im = 0;
+  if (Mat) {
+    const z64* col = Mat + (size_t)p * P;
+    for (int q = lane; q < P; q += 64) {
+      const z64 m = col[q];
+      z64 x = xin[q];
+      if (dx) x = mk(dx[q] * x.x, dx[q] * x.y);
+      re += m.x * x.x + m.y * x.y;  // conj(m) x
+      im += m.x * x.y - m.y * x.x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      re += __shfl_down(re, off);
+      im += __shfl_down(im, off);
+    }
+  }
+  if (lane == 0) {
+    z64 a = mk(0.0, 0.0);
+    if (ain) {
+      a = ain[p];
+      if (da) a = mk(da[p] * a.x, da[p] * a.y);
+    }
+    out[p] = mk(a.x + s * re, a.y + s * im);
+  }
+}
+
+// dst = red[0] + sum_p d_p Re(conj(x_p) y_p) (d null: no projector part),
+// then ab[2 j] = dst (store_sqrt 0) or ab[2 j + 1] = sqrt(max(dst, 0)).
+__global__ void k_lanczos_scalar(int P, const double* __restrict__ d, const z64* __restrict__ x,
+                                 const z64* __restrict__ y, const double* __restrict__ red,
+                                 double* __restrict__ ab, int j, int store_sqrt) {
+  __shared__ double sh[8];
+  double a = 0;
+  if (d)
+    for (int p = threadIdx.x; p < P; p += blockDim.x) a += d[p] * (x[p].x * y[p].x + x[p].y * y[p].y);
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) {
+    const double t = red[0] + r;
+    if (store_sqrt) ab[2 * j + 1] = t > 0 ? sqrt(t) : 0.0;
+    else ab[2 * j] = t;
+  }
+}
+
+// f = u - alpha_j v - beta_{j-1} vp over the n plane waves and the P
+// coefficients (indices n .. n + P - 1 address the coefficient arrays).
+__global__ void k_lanczos_f(const z64* __restrict__ u, const z64* __restrict__ v,
+                            const z64* __restrict__ vp, const z64* __restrict__ wu,
+                            const z64* __restrict__ wv, const z64* __restrict__ wvp,
+                            const double* __restrict__ ab, int j, size_t n, int P,
+                            z64* __restrict__ f, z64* __restrict__ wf) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n + (size_t)P) return;
   const double al = ab[2 * j];
-  const double be = j > 0 ? ab[2 * (j - 1) + 1] : 0.0;
-  const z64 vi = v[i], vpi = vp[i];
-  z64 fi = mk(w[i].x - al * vi.x - be * vpi.x, w[i].y - al * vi.y - be * vpi.y);
-  f[i] = fi;
-  if (gen) {
-    const z64 s = sv[i], sp = svp[i];
-    sf[i] = mk(u[i].x - al * s.x - be * sp.x, u[i].y - al * s.y - be * sp.y);
+  const bool in_pw = i < n;
+  const size_t q = in_pw ? i : i - n;
+  const z64 ui = in_pw ? u[q] : wu[q], vi = in_pw ? v[q] : wv[q];
+  z64 fi = mk(ui.x - al * vi.x, ui.y - al * vi.y);
+  if (j > 0) {
+    const double be = ab[2 * (j - 1) + 1];
+    const z64 pi = in_pw ? vp[q] : wvp[q];
+    fi = mk(fi.x - be * pi.x, fi.y - be * pi.y);
   }
+  if (in_pw) f[q] = fi;
+  else wf[q] = fi;
 }
 
-// alpha_j = Re<v|u> from the two-stage reduction result in red[0]
-__global__ void k_store_alpha(const double* __restrict__ red, double* __restrict__ ab, int j) {
-  ab[2 * j] = red[0];
-}
-__global__ void k_store_beta(const double* __restrict__ red, double* __restrict__ ab, int j) {
-  const double x = red[0];
-  ab[2 * j + 1] = x > 0 ? sqrt(x) : 0.0;
-}
-
-// v <- f / beta_j (and sv <- sf / beta_j); vp, svp take the old v, sv.
-__global__ void k_lanczos_next(z64* __restrict__ v, z64* __restrict__ sv, z64* __restrict__ vp,
-                               z64* __restrict__ svp, const z64* __restrict__ f,
-                               const z64* __restrict__ sf, const double* __restrict__ ab, int j,
-                               int gen, size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// x /= ab[2 j + 1] over n plane waves and P coefficients (x untouched when
+// the divisor is 0, an invariant subspace).
+__global__ void k_lanczos_scale(z64* __restrict__ x, z64* __restrict__ wx,
+                                const double* __restrict__ ab, int j, size_t n, int P) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n + (size_t)P) return;
   const double be = ab[2 * j + 1];
-  const double s = be > 0 ? 1.0 / be : 0.0;
-  vp[i] = v[i];
-  if (gen) svp[i] = sv[i];
-  if (be > 0) {
-    v[i] = mk(f[i].x * s, f[i].y * s);
-    if (gen) sv[i] = mk(sf[i].x * s, sf[i].y * s);
-  }
-}
-
-__global__ void k_lanczos_scale(z64* v, z64* sv, const double* __restrict__ ab, int j, int gen,
-                                size_t n) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double be = ab[2 * j + 1];
-  const double s = be > 0 ? 1.0 / be : 1.0;
-  v[i] = mk(v[i].x * s, v[i].y * s);
-  if (gen) sv[i] = mk(sv[i].x * s, sv[i].y * s);
+  if (!(be > 0)) return;
+  const double s = 1.0 / be;
+  z64* e = i < n ? x + i : wx + (i - n);
+  *e = mk(e->x * s, e->y * s);
 }
 
 // Extreme eigenvalues of the k x k Lanczos tridiagonal T (alpha_i = ab[2i],
@@ -307,6 +355,7 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
   const int n = H.n_pw;
+  const int P = H.n_proj;
   const bool gen = H.has_s;
   if (k < 1 || k > 16) return set_error(ctx, CHS_ERR_INVALID_ARG, "Lanczos steps must be 1..16");
   // start vector from the LCG (host, once per call; n_pw complex)
@@ -328,47 +377,75 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
     }
   }
   const size_t vb = (size_t)n * sizeof(z64);
-  CHS_TRY(ensure(ctx, im->lin_ws, 9 * vb + 64 * sizeof(double)));
+  CHS_TRY(ensure(ctx, im->lin_ws, 4 * vb + 7 * (size_t)std::max(P, 1) * sizeof(z64) +
+                                      64 * sizeof(double)));
   z64* base = (z64*)im->lin_ws.p;
-  z64 *v = base, *sv = base + n, *vp = base + 2 * (size_t)n, *svp = base + 3 * (size_t)n,
-      *u = base + 4 * (size_t)n, *w = base + 5 * (size_t)n, *f = base + 6 * (size_t)n,
-      *sf = base + 7 * (size_t)n;
-  double* ab = (double*)(base + 9 * (size_t)n);  // alpha/beta pairs + scratch
+  // three rotating Lanczos vectors (current, previous, next) and H|v>
+  z64* vec[3] = {base, base + n, base + 2 * (size_t)n};
+  z64* u = base + 3 * (size_t)n;
+  z64* pb = base + 4 * (size_t)n;  // projector coefficients
+  const size_t P1 = (size_t)std::max(P, 1);
+  z64* wvec[3] = {pb, pb + P1, pb + 2 * P1};
+  z64 *g = pb + 3 * P1, *y = pb + 4 * P1, *c = pb + 5 * P1, *wu = pb + 6 * P1;
+  double* ab = (double*)(pb + 7 * P1);  // alpha/beta pairs + scratch
   double* red = ab + 40;
-  CHS_HIP(ctx, hipMemcpyAsync(v, h0.data(), vb, hipMemcpyHostToDevice, ctx->stream));
+  const z64* beta = (const z64*)H.beta;
+  const z64* B = (const z64*)H.bmat.p;
+  const z64* M = gen ? (const z64*)H.sinv_m.p : nullptr;
+  const double* Q = gen ? H.qion : nullptr;
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  const unsigned nblk = (unsigned)ceil_div((size_t)n + P, 256);
+  int cur = 0, prv = 1, nxt = 2;
+  CHS_HIP(ctx, hipMemcpyAsync(vec[cur], h0.data(), vb, hipMemcpyHostToDevice, ctx->stream));
   ctx->h2d += vb;
-  CHS_HIP(ctx, hipMemsetAsync(vp, 0, vb, ctx->stream));
-  CHS_HIP(ctx, hipMemsetAsync(svp, 0, vb, ctx->stream));
   CHS_HIP(ctx, hipMemsetAsync(ab, 0, 40 * sizeof(double), ctx->stream));
-  // normalise: v /= sqrt(<v|Sv>)
-  if (gen) {
-    CHS_TRY(apply_h_impl(ctx, CHS_C128, v, 1, u, sv));
-  } else {
-    CHS_HIP(ctx, hipMemcpyAsync(sv, v, vb, hipMemcpyDeviceToDevice, ctx->stream));
-  }
-  CHS_TRY(dot_re_into(ctx, v, sv, n, red));
-  hipLaunchKernelGGL(k_store_beta, dim3(1), dim3(1), 0, ctx->stream, red, ab, 16);
+  // normalise: v /= sqrt(<v|v> + sum Q |w_v|^2)
+  if (P > 0)
+    CHS_TRY(zgemm(ctx, 'C', 'N', P, 1, n, one, beta, n, vec[cur], n, zero, wvec[cur], P));
+  CHS_TRY(dot_re_into(ctx, vec[cur], vec[cur], n, red));
+  hipLaunchKernelGGL(k_lanczos_scalar, dim3(1), dim3(256), 0, ctx->stream, P, Q, wvec[cur],
+                     wvec[cur], red, ab, 16, 1);
   CHS_LAUNCH_CHECK(ctx);
-  // v <- v / sqrt(<v|Sv>) (divisor in the ab[33] slot)
-  hipLaunchKernelGGL(k_lanczos_scale, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, v, sv,
-                     ab, 16, (int)gen, (size_t)n);
+  hipLaunchKernelGGL(k_lanczos_scale, dim3(nblk), dim3(256), 0, ctx->stream, vec[cur], wvec[cur],
+                     ab, 16, (size_t)n, P);
   CHS_LAUNCH_CHECK(ctx);
   for (int j = 0; j < k; ++j) {
-    CHS_TRY(apply_h_impl(ctx, CHS_C128, v, 1, u, nullptr));
-    CHS_HIP(ctx, hipMemcpyAsync(w, u, vb, hipMemcpyDeviceToDevice, ctx->stream));
-    if (gen) CHS_TRY(sinv_apply(ctx, CHS_C128, w, 1));
-    CHS_TRY(dot_re_into(ctx, v, u, n, red));
-    hipLaunchKernelGGL(k_store_alpha, dim3(1), dim3(1), 0, ctx->stream, red, ab, j);
+    CHS_TRY(vloc_pipeline(ctx, CHS_C128, vec[cur], 1, u, EPI_HPSI, nullptr, nullptr, 0, 0, nullptr,
+                          nullptr));
+    CHS_TRY(dot_re_into(ctx, vec[cur], u, n, red));
+    hipLaunchKernelGGL(k_lanczos_scalar, dim3(1), dim3(256), 0, ctx->stream, P, H.dion,
+                       wvec[cur], wvec[cur], red, ab, j, 0);
     CHS_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_lanczos_f, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, w, u, v, sv,
-                       vp, svp, ab, j, (int)gen, (size_t)n, f, sf);
+    if (P > 0) {
+      const unsigned pblk = (unsigned)ceil_div(P, 4);
+      CHS_TRY(zgemm(ctx, 'C', 'N', P, 1, n, one, beta, n, u, n, zero, g, P));
+      hipLaunchKernelGGL(k_herm_mv, dim3(pblk), dim3(256), 0, ctx->stream, P, B, wvec[cur],
+                         H.dion, g, nullptr, 1.0, y);
+      CHS_LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_herm_mv, dim3(pblk), dim3(256), 0, ctx->stream, P, M, y, nullptr,
+                         wvec[cur], H.dion, -1.0, c);
+      CHS_LAUNCH_CHECK(ctx);
+      hipLaunchKernelGGL(k_herm_mv, dim3(pblk), dim3(256), 0, ctx->stream, P, B, c, nullptr, g,
+                         nullptr, 1.0, wu);
+      CHS_LAUNCH_CHECK(ctx);
+      CHS_TRY(zgemm(ctx, 'N', 'N', n, 1, P, one, beta, n, c, P, one, u, n));
+    }
+    hipLaunchKernelGGL(k_lanczos_f, dim3(nblk), dim3(256), 0, ctx->stream, u, vec[cur], vec[prv],
+                       wu, wvec[cur], wvec[prv], ab, j, (size_t)n, P, vec[nxt], wvec[nxt]);
     CHS_LAUNCH_CHECK(ctx);
-    CHS_TRY(dot_re_into(ctx, f, gen ? sf : f, n, red));
-    hipLaunchKernelGGL(k_store_beta, dim3(1), dim3(1), 0, ctx->stream, red, ab, j);
+    CHS_TRY(dot_re_into(ctx, vec[nxt], vec[nxt], n, red));
+    hipLaunchKernelGGL(k_lanczos_scalar, dim3(1), dim3(256), 0, ctx->stream, P, Q, wvec[nxt],
+                       wvec[nxt], red, ab, j, 1);
     CHS_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_lanczos_next, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, v, sv,
-                       vp, svp, f, sf, ab, j, (int)gen, (size_t)n);
-    CHS_LAUNCH_CHECK(ctx);
+    if (j + 1 < k) {
+      hipLaunchKernelGGL(k_lanczos_scale, dim3(nblk), dim3(256), 0, ctx->stream, vec[nxt],
+                         wvec[nxt], ab, j, (size_t)n, P);
+      CHS_LAUNCH_CHECK(ctx);
+      const int t = prv;
+      prv = cur;
+      cur = nxt;
+      nxt = t;
+    }
   }
   hipLaunchKernelGGL(k_lanczos_finish, dim3(1), dim3(64), 0, ctx->stream, ab, k, red + 4);
   CHS_LAUNCH_CHECK(ctx);

@@ -158,7 +158,8 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   double e_prev = 0, mu = 0, etot = 0;
   int small = 0, it = 0, converged = 0;
   bool fp64_phase = !mixed;
-  int fp64_iters = 0;
+  int fp64_iters = 0, fp32_iters = 0;
+  const double fp32_until = p->fp32_until > 0 ? p->fp32_until : 3.0 * p->etol;
   const double t0 = now();
   double t_filter = 0, t_rr = 0, t_den = 0, t_veff = 0, t_lan = 0;
   for (it = 1; it <= p->max_iter; ++it) {
@@ -184,6 +185,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     t_lan += ta - tb;
     const int passes = it == 1 ? std::max(1, p->first_passes) : 1;
     const bool use32 = mixed && !fp64_phase;
+    if (use32) ++fp32_iters;
     for (int ps = 0; ps < passes; ++ps) {
       double b4[4];
       chs_spectral_bounds(eps.data(), (int)eps.size(), lmax, b4);
@@ -226,7 +228,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     if (fp64_phase) ++fp64_iters;
     if (it > 1) {
       const double de = std::abs(etot - e_prev) / n_atoms;
-      if (!fp64_phase && de < 1e-5) {
+      if (!fp64_phase && de < fp32_until) {
         // FP32 filter has reached its useful accuracy: finish in FP64
         fp64_phase = true;
         small = 0;
@@ -261,6 +263,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   res->t_density = t_den;
   res->t_veff = t_veff;
   res->t_lanczos = t_lan;
+  res->n_iter_fp32 = fp32_iters;
   if (eps_out) std::copy(eps.begin(), eps.end(), eps_out);
   if (rho_host) {
     CHS_HIP(ctx, hipMemcpy(rho_host, S.rho_out.p, n3 * sizeof(double), hipMemcpyDeviceToHost));

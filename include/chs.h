@@ -237,12 +237,15 @@ typedef struct chs_scf_params {
   int32_t max_iter;
   int32_t precision;       /* CHS_C128, or CHS_C64 filter with FP64 RR */
   int32_t fp64_final;      /* with CHS_C64: SCF steps forced back to FP64 at the end */
+  double fp32_until;       /* with CHS_C64: |dE|/atom (Ha) below which the filter goes back
+                              to FP64 (<= 0: 3 etol) */
 } chs_scf_params;
 
 typedef struct chs_scf_result {
   int32_t n_iter, converged, n_pw, n_proj;
   double energy, mu, seconds;
   double t_filter, t_rr, t_density, t_veff, t_lanczos; /* seconds per phase */
+  int32_t n_iter_fp32;     /* SCF iterations whose filter ran in complex64 */
 } chs_scf_result;
 
 /* Full SCF (run_scf, src/scf.rs:358-376, with the todo!() transitions

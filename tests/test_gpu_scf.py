@@ -54,3 +54,28 @@ def test_scf_config_a_vs_oracle(ctx):
     dv = omega / 36 ** 3
     assert np.abs(got["rho"] - ref["rho"]).sum() * dv < 1e-6
     assert np.abs(got["energies"] - ref["energies"]).max() / n_at < 1e-7
+
+
+@pytest.mark.parametrize("npa", [0, 4])
+def test_scf_mixed_precision_matches_fp64(ctx, npa):
+    """complex64 filter with FP64 Rayleigh-Ritz, back to complex128 once
+    |dE|/atom < 3 etol (at least fp64_final = 3 FP64 iterations): the same
+    iterations and, within 1e-7 Ha/atom, the same energy as the plain FP64
+    SCF (which the tests above pin to the oracle)."""
+    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=40,
+              n_proj_per_atom=npa, q0=0.25 if npa else 0.0)
+    pos = O.positions(1)
+    r64 = chs.run_scf(ctx, chs.scf_params(**kw), pos, want_rho=True)
+    r32 = chs.run_scf(ctx, chs.scf_params(**kw, precision=chs.C64), pos, want_rho=True)
+    assert r64["converged"] and r32["converged"]
+    assert r64["n_iter_fp32"] == 0
+    assert r32["n_iter_fp32"] >= 3 and r32["n_iter"] - r32["n_iter_fp32"] >= 3
+    de = abs(r32["energy"] - r64["energy"]) / 8
+    deps = np.abs(r32["eps"] - r64["eps"]).max()
+    drho = np.abs(r32["rho"] - r64["rho"]).sum() * 10.26 ** 3 / 36 ** 3
+    print(f"npa={npa}: iterations {r32['n_iter']} ({r32['n_iter_fp32']} FP32) vs "
+          f"{r64['n_iter']}; |dE|/atom {de:.2e}, |d eps| {deps:.2e}, rho L1 {drho:.2e}")
+    assert r32["n_iter"] == r64["n_iter"]
+    assert de < 1e-7
+    assert deps < 1e-6
+    assert drho < 1e-5
