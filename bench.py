@@ -58,7 +58,7 @@ WORKLOAD = {
 PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
 
 
-def roofline(chs, ctx, cfg, precision):
+def roofline(chs, ctx, cfg, precision, tune=False):
     """Per-launch times of one Chebyshev step's kernels at the workload's
     shape, measured with HIP events on the context stream: the three H|psi>
     kernels and, with projectors, the two n_pw-long projector GEMMs. The
@@ -74,6 +74,9 @@ def roofline(chs, ctx, cfg, precision):
     veff = chs.Gpu.from_host(ctx, rng.standard_normal(n ** 3) * 0.1)
     chs.Hamiltonian(ctx, n, mil, kin, veff)
     nb = cfg["n_bands"]
+    tuned = None
+    if tune:  # a fresh context (the SCF tunes its grid on first use)
+        tuned = chs.autotune_hpsi(ctx, nb, prec)[:3]
     prof = chs.profile_hpsi(ctx, nb, prec, reps=10)
     names = ["k_stick_inv", "k_plane", "k_stick_fwd"]
     items = [{"kernel": nm, "ms": float(ms), "bound": "hbm", "work": float(b)}
@@ -109,6 +112,8 @@ def roofline(chs, ctx, cfg, precision):
         "hpsi_canonical_bytes": float(canonical),
         "hpsi_canonical_gbs": round(canonical / (hpsi_ms * 1e-3) / 1e9, 1),
         "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision,
+        **({"hpsi_launch": dict(zip(("sticks_per_block", "row_div", "plane_threads"), tuned))}
+           if tuned else {}),
     }
 
 
@@ -286,7 +291,7 @@ def main():
             for _ in range(4):
                 ctx.check(chs.lib().chs_d2d(ctx.h, b.ptr, a.ptr, args.copy_bytes))
             ctx.synchronize()
-        print(json.dumps(roofline(chs, ctx, cfg, precision)))
+        print(json.dumps(roofline(chs, ctx, cfg, precision, tune=True)))
         return
     ctx.synchronize()
     t0 = time.perf_counter()

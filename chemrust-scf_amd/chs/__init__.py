@@ -434,7 +434,7 @@ def scf_params(**kw):
              n_proj_per_atom=0, beta_rc=1.3, q_rc=1.2, q0=0.0, d_min=0.5, d_max=2.0, seed_d=3,
              seed_psi=2, seed_lanczos=4, cheb_degree=10, first_passes=4, lanczos_k=10,
              pulay_depth=6, mix_alpha=0.5, kerker_q=0.8, etol=1e-6, smear_kt=0.01,
-             max_iter=60, precision=C128, fp64_final=3, fp32_until=0.0)
+             max_iter=60, precision=C128, fp64_final=2, fp32_until=0.0)
     d.update(kw)
     for k, v in d.items():
         if k == "kfrac":

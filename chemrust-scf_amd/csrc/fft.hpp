@@ -251,32 +251,79 @@ __device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
 }
 
 // One x-row of the plane pass: inverse transform from the compact row (only
-// active columns stored; act_of[x] = compact index of column x, or -1),
-// V_eff multiply, forward transform back into the compact row, with two
-// exchanges through the group's N-element scratch `sc` and everything else
-// in registers:
+// active columns stored; column x sits at compact index act_of[x], or is
+// absent), V_eff multiply, forward transform back into the compact row, with
+// two exchanges through the group's N-element scratch `sc` and everything
+// else in registers:
 //   inverse: lane n2 does DFT_N1 over x[n1 N2 + n2] and twiddles; lane k1
 //            then DFT_N2 over n2, giving y[k1 + N1 k2];
 //   y *= V;  lane k1 does the forward DFT_N2 over k2 and twiddles
 //            w_N^{-k1 m2}; lane m2 then DFT_N1 over k1, giving
 //            Z[m2 + N2 m1], stored for the active columns only.
-template <int N, typename T>
-__device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc,
-                                            const int* __restrict__ act_of,
-                                            const T* __restrict__ vrow, int lane,
-                                            const cplx<T>* __restrict__ tw) {
+// A row plan gives what is the same for every row of a lane: the compact
+// indices it reads (in) and writes (out) and its twiddles w1(k1) =
+// w_N^{+lane k1}, w2(r, m2) = w_N^{-k1 m2} with k1 = lane + r N2.
+// RowPlanRegs loads them once per group into registers; RowPlanLoad reads
+// them per row (for launch shapes whose register budget is tight).
+template <int N, typename T> struct RowPlanRegs {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static constexpr int R = (N1 + N2 - 1) / N2;
+  int in_idx[N1];
+  int out_idx[N1];
+  cplx<T> tw1[N1];
+  cplx<T> tw2[R][N2];
+  __device__ __forceinline__ RowPlanRegs(const int* __restrict__ act_of,
+                                         const cplx<T>* __restrict__ tw, int lane) {
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      in_idx[n1] = act_of[n1 * N2 + lane];
+      out_idx[n1] = act_of[lane + N2 * n1];
+      tw1[n1] = twiddle<1>(tw, (lane * n1) % N);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int m2 = 0; m2 < N2; ++m2) {
+        const int k1 = lane + r * N2;
+        tw2[r][m2] = twiddle<-1>(tw, k1 < N1 ? (k1 * m2) % N : 0);
+      }
+  }
+  __device__ __forceinline__ int in(int n1) const { return in_idx[n1]; }
+  __device__ __forceinline__ int out(int m1) const { return out_idx[m1]; }
+  __device__ __forceinline__ cplx<T> w1(int k1) const { return tw1[k1]; }
+  __device__ __forceinline__ cplx<T> w2(int r, int m2) const { return tw2[r][m2]; }
+};
+
+template <int N, typename T> struct RowPlanLoad {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  const int* act_of;
+  const cplx<T>* tw;
+  int lane;
+  __device__ __forceinline__ RowPlanLoad(const int* a, const cplx<T>* t, int l)
+      : act_of(a), tw(t), lane(l) {}
+  __device__ __forceinline__ int in(int n1) const { return act_of[n1 * N2 + lane]; }
+  __device__ __forceinline__ int out(int m1) const { return act_of[lane + N2 * m1]; }
+  __device__ __forceinline__ cplx<T> w1(int k1) const { return twiddle<1>(tw, lane * k1); }
+  __device__ __forceinline__ cplx<T> w2(int r, int m2) const {
+    return twiddle<-1>(tw, (lane + r * N2) * m2);
+  }
+};
+
+template <int N, typename T, typename Plan>
+__device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Plan& rp,
+                                            const T* __restrict__ vrow, int lane) {
   constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
   constexpr int R = (N1 + N2 - 1) / N2;
   {
     cplx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-      const int ai = act_of[n1 * N2 + lane];
+      const int ai = rp.in(n1);
       a[n1] = ai >= 0 ? crow[ai] : mk((T)0, (T)0);
     }
     Dft<N1, 1, T>::run(a);
 #pragma unroll
-    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * twiddle<1>(tw, lane * k1);
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * rp.w1(k1);
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) sc[k1 * N2 + lane] = a[k1];
   }
@@ -296,7 +343,7 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc,
       }
       Dft<N2, -1, T>::run(b[r]);
 #pragma unroll
-      for (int m2 = 1; m2 < N2; ++m2) b[r][m2] = b[r][m2] * twiddle<-1>(tw, k1 * m2);
+      for (int m2 = 1; m2 < N2; ++m2) b[r][m2] = b[r][m2] * rp.w2(r, m2);
     }
   }
   wave_sync();
@@ -316,7 +363,7 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc,
     Dft<N1, -1, T>::run(c);
 #pragma unroll
     for (int m1 = 0; m1 < N1; ++m1) {
-      const int ai = act_of[lane + N2 * m1];
+      const int ai = rp.out(m1);
       if (ai >= 0) crow[ai] = c[m1];
     }
   }

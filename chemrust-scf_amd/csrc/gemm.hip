@@ -93,8 +93,9 @@ __global__ void __launch_bounds__(256) k_gemm(int m, int n, int k, cplx<T> alpha
 
 // ---------------------------------------------------------------------------
 // Matrix-core variant: LDS-staged block tiles (below), register prefetch of
-// the next K stage during the MFMAs; the complex product is four real
-// 16x16x4 MFMAs per k-step (re += ar br - ai bi, im += ar bi + ai br).
+// the next K stage during the MFMAs; the complex product is three real
+// 16x16x4 MFMAs per k-step (the 3M form: P1 += ar br, P2 += ai bi,
+// P3 += (ar + ai)(br + bi); re = P1 - P2, im = P3 - P1 - P2).
 // Operand layout of both v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32:
 // lane l supplies
 // A[l % 16][k0 + l / 16] and B[k0 + l / 16][l % 16]. Result layout: lane l
@@ -120,15 +121,21 @@ template <> struct Mfma<float> {
 // WTM x WTN MFMA tiles of 16 x 16. Small: 64 x 64 (both precisions); large:
 // 128 x 128 x 16 (complex64) or 128 x 64 x 8 (complex128). The autotuner
 // picks per GEMM shape.
+// WAVES: wavefronts per SIMD the register allocation must leave room for
+// (the small tiles keep at least two resident, so one wave's LDS waits and
+// barriers do not idle the matrix core).
 template <typename T, int V> struct MfmaTile;
 template <typename T> struct MfmaTile<T, 1> {
   static constexpr int BM = 64, BN = 64, BK = 16, WTM = 2, WTN = 2;
+  static constexpr int WAVES = sizeof(T) == 8 ? 2 : 4;
 };
 template <> struct MfmaTile<float, 2> {
   static constexpr int BM = 128, BN = 128, BK = 16, WTM = 4, WTN = 4;
+  static constexpr int WAVES = 1;
 };
 template <> struct MfmaTile<double, 2> {
   static constexpr int BM = 128, BN = 64, BK = 8, WTM = 4, WTN = 2;
+  static constexpr int WAVES = 1;
 };
 
 // XCD-aware block order: workgroups are dealt to the eight XCDs round-robin
@@ -148,7 +155,8 @@ __device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int* bm, int*
 }
 
 template <typename T, int OPA, int OPB, int V>
-__global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
                                                    const cplx<T>* __restrict__ A, int lda,
                                                    const cplx<T>* __restrict__ B, int ldb,
                                                    cplx<T> beta, cplx<T>* __restrict__ C, int ldc,
@@ -162,20 +170,26 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
   int bm, bn, bz;
   if (!gemm_block((m + BM - 1) / BM, (n + BN - 1) / BN, ksplit, &bm, &bn, &bz)) return;
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // staged elements per thread
-  __shared__ cplx<T> As[BK][BM + 4];
-  __shared__ cplx<T> Bs[BK][BN + 4];
+  // one padding element per row: the K-major (transposing) stores of T / C
+  // operands then hit distinct banks across each phase of lanes, while the
+  // MFMA operand reads stay contiguous along the row
+  __shared__ cplx<T> As[BK][BM + 1];
+  __shared__ cplx<T> Bs[BK][BN + 1];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm0 = (w >> 1) * (16 * WTM), wn0 = (w & 1) * (16 * WTN);
   const int li = lane & 15, lk = lane >> 4;
   const int m0 = bm * BM, n0 = bn * BN;
   const int kb = bz * klen, ke = min(k, kb + klen);
-  acc_t cre[WTM][WTN], cim[WTM][WTN];
+  // three real products per complex one (Re = P1 - P2, Im = P3 - P1 - P2
+  // with P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar + ai)(br + bi))
+  acc_t p1[WTM][WTN], p2[WTM][WTN], p3[WTM][WTN];
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
 #pragma unroll
     for (int j = 0; j < WTN; ++j) {
-      cre[i][j] = acc_t{0, 0, 0, 0};
-      cim[i][j] = acc_t{0, 0, 0, 0};
+      p1[i][j] = acc_t{0, 0, 0, 0};
+      p2[i][j] = acc_t{0, 0, 0, 0};
+      p3[i][j] = acc_t{0, 0, 0, 0};
     }
   cplx<T> ra[NA], rb[NB];
   auto a_pos = [](int e, int& r, int& kk) {
@@ -224,15 +238,21 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
       for (int i = 0; i < WTM; ++i) a[i] = As[kk][wm0 + 16 * i + li];
 #pragma unroll
       for (int j = 0; j < WTN; ++j) b[j] = Bs[kk][wn0 + 16 * j + li];
+      // independent accumulators back to back: each product class over all
+      // tiles before the next
 #pragma unroll
       for (int i = 0; i < WTM; ++i)
 #pragma unroll
-        for (int j = 0; j < WTN; ++j) {
-          cre[i][j] = M::mac(a[i].x, b[j].x, cre[i][j]);
-          cre[i][j] = M::mac(-a[i].y, b[j].y, cre[i][j]);
-          cim[i][j] = M::mac(a[i].x, b[j].y, cim[i][j]);
-          cim[i][j] = M::mac(a[i].y, b[j].x, cim[i][j]);
-        }
+        for (int j = 0; j < WTN; ++j) p1[i][j] = M::mac(a[i].x, b[j].x, p1[i][j]);
+#pragma unroll
+      for (int i = 0; i < WTM; ++i)
+#pragma unroll
+        for (int j = 0; j < WTN; ++j) p2[i][j] = M::mac(a[i].y, b[j].y, p2[i][j]);
+#pragma unroll
+      for (int i = 0; i < WTM; ++i)
+#pragma unroll
+        for (int j = 0; j < WTN; ++j)
+          p3[i][j] = M::mac(a[i].x + a[i].y, b[j].x + b[j].y, p3[i][j]);
     }
     __syncthreads();
   }
@@ -245,7 +265,8 @@ __global__ void __launch_bounds__(256) k_gemm_mfma(int m, int n, int k, cplx<T> 
         const int r = m0 + wm0 + 16 * i + M::row(lane, v);
         const int c = n0 + wn0 + 16 * j + li;
         if (r < m && c < n) {
-          const cplx<T> acc = mk((T)cre[i][j][v], (T)cim[i][j][v]);
+          const T r1 = (T)p1[i][j][v], r2 = (T)p2[i][j][v];
+          const cplx<T> acc = mk(r1 - r2, (T)p3[i][j][v] - r1 - r2);
           if (part) {
             part[(size_t)bz * m * n + (size_t)r + (size_t)c * m] = acc;
           } else {

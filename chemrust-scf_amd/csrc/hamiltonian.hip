@@ -104,17 +104,25 @@ __device__ __forceinline__ void expand_row(const cplx<T>* cmp_row, cplx<T>* rb, 
 }
 
 // --- B: plane pass with V_eff ----------------------------------------------
+// Block order: workgroups are dealt to the eight XCDs round-robin by launch
+// index (physical block p runs on XCD p % 8); logical block
+// L = (p % 8) (P / 8) + p / 8 gives every XCD a contiguous range of
+// (z-plane, band) pairs, band fastest, so each XCD's L2 holds only its
+// N / 8 planes of V_eff while all bands stream through them.
 template <int N, typename T, int TPB>
 __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, StickArgs st,
-                                               const T* __restrict__ veff, int nrb,
+                                               const T* __restrict__ veff, int nrb, int n_b,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
   constexpr int LS = N + 1;
+  const int P = gridDim.x, p = blockIdx.x;
+  const int L = (p & 7) * (P >> 3) + (p >> 3);
+  if (L >= N * n_b) return;  // grid padding (whole block)
+  const int z = L / n_b, b = L - z * n_b;
   const int RS = st.rs;
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
   int* act_of = reinterpret_cast<int*>(rowbuf + nrb * LS);
-  const int z = blockIdx.x, b = blockIdx.y;
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.n_st;
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
@@ -129,7 +137,13 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   if (g >= 0 && g < nrb) {
     cplx<T>* sc = rowbuf + g * LS;
     const T* vz = veff + (size_t)z * N * N;
-    for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, act_of, vz + iy * N, lane, tw);
+    if (TPB <= 512) {
+      const RowPlanRegs<N, T> rp(act_of, tw, lane);
+      for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+    } else {
+      const RowPlanLoad<N, T> rp(act_of, tw, lane);
+      for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+    }
   }
   __syncthreads();
   if (g >= 0)
@@ -412,12 +426,26 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
   hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
                      sbuf, tw);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
-  if (pl.threads == 512)
-    hipLaunchKernelGGL((k_plane<NN, T, 512>), dim3(NN, n_b), dim3(512), pl.lds, ctx->stream, sbuf,
-                       sa, veff, pl.nrb, tw);
-  else
-    hipLaunchKernelGGL((k_plane<NN, T, 256>), dim3(NN, n_b), dim3(256), pl.lds, ctx->stream, sbuf,
-                       sa, veff, pl.nrb, tw);
+  const unsigned pgrid = (unsigned)(((size_t)NN * n_b + 7) / 8 * 8);
+  // 1024-thread blocks for complex64 only (vloc_pipeline_t never asks for
+  // them with complex128, whose codelets do not fit the 128 registers per
+  // lane that 16 wavefronts per CU leave)
+  bool launched = false;
+  if constexpr (sizeof(T) == 4) {
+    if (pl.threads == 1024) {
+      hipLaunchKernelGGL((k_plane<NN, T, 1024>), dim3(pgrid), dim3(1024), pl.lds, ctx->stream,
+                         sbuf, sa, veff, pl.nrb, n_b, tw);
+      launched = true;
+    }
+  }
+  if (!launched) {
+    if (pl.threads == 512)
+      hipLaunchKernelGGL((k_plane<NN, T, 512>), dim3(pgrid), dim3(512), pl.lds, ctx->stream, sbuf,
+                         sa, veff, pl.nrb, n_b, tw);
+    else
+      hipLaunchKernelGGL((k_plane<NN, T, 256>), dim3(pgrid), dim3(256), pl.lds, ctx->stream, sbuf,
+                         sa, veff, pl.nrb, n_b, tw);
+  }
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
 #define CHS_FWD(M)                                                                            \
   hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \
@@ -458,6 +486,8 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   const double inv_n3 = 1.0 / ((double)n * n * n);
   VlocTune tn = get_tune(ctx, n, precision);
   if (!sb_fits(n, sizeof(C), tn.sb)) tn.sb = 16;
+  if (tn.threads != 256 && tn.threads != 512 && (tn.threads != 1024 || sizeof(T) != 4))
+    tn.threads = 512;
   if (n_part) *n_part = (int)ceil_div(st.n_st, tn.sb) * n_b;
   hipEvent_t ev[4];
   if (ms)
@@ -531,8 +561,9 @@ int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, voi
   double best_t = 1e300;
   for (int sb : {16, 32, 64}) {
     if (!sb_fits(n, csz, sb)) continue;
-    for (int threads : {256, 512})
+    for (int threads : {256, 512, 1024})
       for (int div : {1, 2, 4}) {
+        if (threads == 1024 && csz != 8) continue;
         const VlocTune cand{sb, div, threads};
         ctx->impl->vloc_tune[key] = cand;
         double ms[3];

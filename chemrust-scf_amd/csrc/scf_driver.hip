@@ -159,7 +159,12 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   int small = 0, it = 0, converged = 0;
   bool fp64_phase = !mixed;
   int fp64_iters = 0, fp32_iters = 0;
-  const double fp32_until = p->fp32_until > 0 ? p->fp32_until : 3.0 * p->etol;
+  // complex64 filter until |dE|/atom < fp32_until (default 3 etol, but not
+  // below 3e-7 Ha, near the energy noise the complex64 iterates leave), or
+  // until |dE| stalls (> 0.9 of the previous one) within 10 fp32_until;
+  // convergence is then decided on FP64 energies only
+  const double fp32_until = p->fp32_until > 0 ? p->fp32_until : std::max(3.0 * p->etol, 3e-7);
+  double de_prev = INFINITY;
   const double t0 = now();
   double t_filter = 0, t_rr = 0, t_den = 0, t_veff = 0, t_lan = 0;
   for (it = 1; it <= p->max_iter; ++it) {
@@ -228,13 +233,12 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     if (fp64_phase) ++fp64_iters;
     if (it > 1) {
       const double de = std::abs(etot - e_prev) / n_atoms;
-      if (!fp64_phase && de < fp32_until) {
-        // FP32 filter has reached its useful accuracy: finish in FP64
-        fp64_phase = true;
+      small = de < p->etol ? small + 1 : 0;
+      if (!fp64_phase && (de < fp32_until || (de < 10.0 * fp32_until && de > 0.9 * de_prev))) {
+        fp64_phase = true;  // the FP32 filter has done its part: finish in FP64
         small = 0;
-      } else {
-        small = de < p->etol ? small + 1 : 0;
       }
+      de_prev = de;
     }
     e_prev = etot;
     t_den += now() - tb;

@@ -236,9 +236,12 @@ typedef struct chs_scf_params {
   double mix_alpha, kerker_q, etol, smear_kt;
   int32_t max_iter;
   int32_t precision;       /* CHS_C128, or CHS_C64 filter with FP64 RR */
-  int32_t fp64_final;      /* with CHS_C64: SCF steps forced back to FP64 at the end */
+  int32_t fp64_final;      /* with CHS_C64: at least this many FP64 SCF steps at the end */
   double fp32_until;       /* with CHS_C64: |dE|/atom (Ha) below which the filter goes back
-                              to FP64 (<= 0: 3 etol) */
+                              to FP64 (<= 0: max(3 etol, 3e-7)); it also goes back when |dE|
+                              stalls (> 0.9 x the previous) below 10 fp32_until. The
+                              convergence test (|dE|/atom < etol twice in a row) then runs
+                              on FP64 steps only */
 } chs_scf_params;
 
 typedef struct chs_scf_result {

@@ -59,23 +59,32 @@ def test_scf_config_a_vs_oracle(ctx):
 @pytest.mark.parametrize("npa", [0, 4])
 def test_scf_mixed_precision_matches_fp64(ctx, npa):
     """complex64 filter with FP64 Rayleigh-Ritz, back to complex128 once
-    |dE|/atom < 3 etol (at least fp64_final = 3 FP64 iterations): the same
-    iterations and, within 1e-7 Ha/atom, the same energy as the plain FP64
-    SCF (which the tests above pin to the oracle)."""
-    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=40,
+    |dE|/atom < max(3 etol, 3e-7) (at least fp64_final = 2 FP64 iterations),
+    at etol = 1e-6: the energy within 1e-7 Ha/atom of the plain FP64 SCF at the
+    same tolerance (which the tests above pin to the oracle), and eigenvalues
+    and density no further from the tightly converged FP64 solution
+    (etol = 1e-8) than the plain FP64 run at etol = 1e-6 is."""
+    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=100,
               n_proj_per_atom=npa, q0=0.25 if npa else 0.0)
     pos = O.positions(1)
     r64 = chs.run_scf(ctx, chs.scf_params(**kw), pos, want_rho=True)
     r32 = chs.run_scf(ctx, chs.scf_params(**kw, precision=chs.C64), pos, want_rho=True)
-    assert r64["converged"] and r32["converged"]
+    rt = chs.run_scf(ctx, chs.scf_params(**kw, etol=1e-8), pos, want_rho=True)
+    assert r64["converged"] and r32["converged"] and rt["converged"]
     assert r64["n_iter_fp32"] == 0
-    assert r32["n_iter_fp32"] >= 3 and r32["n_iter"] - r32["n_iter_fp32"] >= 3
-    de = abs(r32["energy"] - r64["energy"]) / 8
-    deps = np.abs(r32["eps"] - r64["eps"]).max()
-    drho = np.abs(r32["rho"] - r64["rho"]).sum() * 10.26 ** 3 / 36 ** 3
+    assert r32["n_iter_fp32"] >= 3 and r32["n_iter"] - r32["n_iter_fp32"] >= 2
+    dv = 10.26 ** 3 / 36 ** 3
+
+    def err(r):
+        return (abs(r["energy"] - rt["energy"]) / 8, np.abs(r["eps"] - rt["eps"]).max(),
+                np.abs(r["rho"] - rt["rho"]).sum() * dv)
+
+    e64, e32 = err(r64), err(r32)
     print(f"npa={npa}: iterations {r32['n_iter']} ({r32['n_iter_fp32']} FP32) vs "
-          f"{r64['n_iter']}; |dE|/atom {de:.2e}, |d eps| {deps:.2e}, rho L1 {drho:.2e}")
-    assert r32["n_iter"] == r64["n_iter"]
-    assert de < 1e-7
-    assert deps < 1e-6
-    assert drho < 1e-5
+          f"{r64['n_iter']} (tight {rt['n_iter']}); distance to the tight solution "
+          f"(E/atom, eps, rho L1): FP64 {e64[0]:.2e} {e64[1]:.2e} {e64[2]:.2e}, "
+          f"mixed {e32[0]:.2e} {e32[1]:.2e} {e32[2]:.2e}")
+    assert r32["n_iter"] <= r64["n_iter"] + 3
+    assert abs(r32["energy"] - r64["energy"]) / 8 < 1e-7
+    assert e32[1] <= 3 * e64[1] + 1e-7
+    assert e32[2] <= 3 * e64[2] + 1e-6
