@@ -309,6 +309,18 @@ template <int N, typename T> struct RowPlanLoad {
   }
 };
 
+// Row scratch of row_apply_v, one per lane group, STRIDE elements apart. The
+// first exchange uses it as [k1][n2] with row pitch PA, the second as
+// [m2][k1] with pitch PB; the pitches and the group stride spread the four
+// exchange patterns of the groups of a wavefront over the LDS banks.
+template <int N, typename T> struct RowScratch {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static constexpr int PA = sizeof(T) == 4 ? N2 + 1 : N2;
+  static constexpr int PB = N1 + 1;
+  static constexpr int SPAN = N1 * PA > N2 * PB ? N1 * PA : N2 * PB;
+  static constexpr int STRIDE = SPAN + (sizeof(T) == 4 ? 17 : 20);
+};
+
 template <int N, typename T, typename Plan>
 __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Plan& rp,
                                             const T* __restrict__ vrow, int lane) {
@@ -325,7 +337,7 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * rp.w1(k1);
 #pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) sc[k1 * N2 + lane] = a[k1];
+    for (int k1 = 0; k1 < N1; ++k1) sc[k1 * RowScratch<N, T>::PA + lane] = a[k1];
   }
   wave_sync();
   cplx<T> b[R][N2];
@@ -334,7 +346,7 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     const int k1 = lane + r * N2;
     if (k1 < N1) {
 #pragma unroll
-      for (int n2 = 0; n2 < N2; ++n2) b[r][n2] = sc[k1 * N2 + n2];
+      for (int n2 = 0; n2 < N2; ++n2) b[r][n2] = sc[k1 * RowScratch<N, T>::PA + n2];
       Dft<N2, 1, T>::run(b[r]);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
@@ -352,14 +364,14 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     const int k1 = lane + r * N2;
     if (k1 < N1) {
 #pragma unroll
-      for (int m2 = 0; m2 < N2; ++m2) sc[k1 + N1 * m2] = b[r][m2];
+      for (int m2 = 0; m2 < N2; ++m2) sc[m2 * RowScratch<N, T>::PB + k1] = b[r][m2];
     }
   }
   wave_sync();
   {
     cplx<T> c[N1];
 #pragma unroll
-    for (int k1 = 0; k1 < N1; ++k1) c[k1] = sc[k1 + N1 * lane];
+    for (int k1 = 0; k1 < N1; ++k1) c[k1] = sc[lane * RowScratch<N, T>::PB + k1];
     Dft<N1, -1, T>::run(c);
 #pragma unroll
     for (int m1 = 0; m1 < N1; ++m1) {

@@ -114,7 +114,6 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
                                                const T* __restrict__ veff, int nrb, int n_b,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  constexpr int LS = N + 1;
   const int P = gridDim.x, p = blockIdx.x;
   const int L = (p & 7) * (P >> 3) + (p >> 3);
   if (L >= N * n_b) return;  // grid padding (whole block)
@@ -122,7 +121,7 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   const int RS = st.rs;
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
-  int* act_of = reinterpret_cast<int*>(rowbuf + nrb * LS);
+  int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.n_st;
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
@@ -135,7 +134,7 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
     for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
   __syncthreads();
   if (g >= 0 && g < nrb) {
-    cplx<T>* sc = rowbuf + g * LS;
+    cplx<T>* sc = rowbuf + g * RowScratch<N, T>::STRIDE;
     const T* vz = veff + (size_t)z * N * N;
     if (TPB <= 512) {
       const RowPlanRegs<N, T> rp(act_of, tw, lane);
@@ -392,9 +391,9 @@ struct PlaneLaunch {
   int threads = 256;
 };
 static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz, int n2,
-                        int nrb_div, int threads, PlaneLaunch* out) {
+                        int nrb_div, int threads, int row_elems, PlaneLaunch* out) {
   const size_t cmp_bytes = (size_t)n * sa.rs * csz;
-  const size_t row_bytes = (size_t)(n + 1) * csz;
+  const size_t row_bytes = (size_t)row_elems * csz;
   const int lim = max_lds_per_block(ctx);
   if (cmp_bytes + row_bytes + (size_t)n * sizeof(int32_t) > (size_t)lim)
     return set_error(ctx, CHS_ERR_FFT, "plane of %d^3 grid does not fit in LDS (%zu > %d bytes)",
@@ -496,7 +495,8 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   for (int r = 0; r < std::max(1, reps); ++r) {
     CHS_DISPATCH_N(n, {
       PlaneLaunch pl;
-      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, tn.threads, &pl));
+      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, tn.threads,
+                           RowScratch<NN, T>::STRIDE, &pl));
       hipEvent_t* evp = ms ? ev : nullptr;
       if (tn.sb == 16)
         CHS_TRY((vloc_launch<NN, T, 16>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
@@ -615,7 +615,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   const int gz = ceil_div(st.n_st, SB);
   CHS_DISPATCH_N(n, {
     PlaneLaunch pl;
-    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, 256, &pl));
+    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, 256, NN + 1, &pl));
     hipLaunchKernelGGL((k_stick_inv<NN, double, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);

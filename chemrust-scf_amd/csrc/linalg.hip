@@ -547,7 +547,7 @@ int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w) {
 
 // Rayleigh-Ritz on a c128 block Y (n_b x n_pw band-major): see file header.
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
-                       double* eps_dev_out) {
+                       double* eps_dev_out, z64* w_out) {
   CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
@@ -583,6 +583,10 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, one, Y, n_pw, Hs, n_b, zero, SY, n_pw));
   CHS_HIP(ctx, hipMemcpyAsync(Y, SY, ne * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
   if (HY_out) CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, one, HY, n_pw, Hs, n_b, zero, HY_out, n_pw));
+  // beta^H (Y V) = (beta^H Y) V: subspace_nl left beta^H Y at the head of proj_ws
+  if (nl && w_out)
+    CHS_TRY(zgemm(ctx, 'N', 'N', H.n_proj, n_b, n_b, one, (const z64*)im->proj_ws.p, H.n_proj, Hs,
+                  n_b, zero, w_out, H.n_proj));
   if (eps_dev_out)
     CHS_HIP(ctx, hipMemcpyAsync(eps_dev_out, w, n_b * sizeof(double), hipMemcpyDeviceToDevice,
                                 ctx->stream));

@@ -16,8 +16,10 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
                 int* sweeps_out);
 int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U);
 int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w);
+// w_out (optional, with projectors and no HY_out): beta^H Y of the rotated
+// block (P x n_b), from the pre-rotation coefficients times the Ritz vectors.
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
-                       double* eps_dev_out);
+                       double* eps_dev_out, z64* w_out = nullptr);
 int cholqr_impl(chs_context* ctx, z64* Y, int n_b);
 
 }  // namespace chs

@@ -407,7 +407,7 @@ int build_beta(chs_context* ctx, const int32_t* miller, int n_pw, double L, cons
 
 int augment_density(chs_context* ctx, const z64* psi, int n_b, const double* d_occ, int n_atoms,
                     int np, double q0, double rq, const z64* d_ph, int n, double L, z64* cgrid,
-                    double* rho) {
+                    double* rho, const z64* w) {
   Hamiltonian& H = ctx->impl->ham;
   const int P = H.n_proj, n_pw = H.n_pw;
   const size_t pb = (size_t)P * n_b;
@@ -415,9 +415,12 @@ int augment_density(chs_context* ctx, const z64* psi, int n_b, const double* d_o
   z64* W = (z64*)ctx->impl->proj_ws.p;
   double* rho_p = (double*)(W + 3 * pb);
   double* ra = rho_p + P;
-  CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0, 0.0), (const z64*)H.beta, n_pw, psi, n_pw,
-                mk(0.0, 0.0), W, P));
-  hipLaunchKernelGGL(k_proj_occ, dim3(ceil_div(P, 256)), dim3(256), 0, ctx->stream, W, P, n_b,
+  if (!w) {
+    CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0, 0.0), (const z64*)H.beta, n_pw, psi, n_pw,
+                  mk(0.0, 0.0), W, P));
+    w = W;
+  }
+  hipLaunchKernelGGL(k_proj_occ, dim3(ceil_div(P, 256)), dim3(256), 0, ctx->stream, w, P, n_b,
                      d_occ, rho_p);
   CHS_LAUNCH_CHECK(ctx);
   // ra[a] = q0 sum_j rho_{a,j}

@@ -56,10 +56,10 @@ static void initial_psi_host(const std::vector<double>& kin, int n_b, uint64_t s
 
 struct ScfState {
   DevBuf miller, idx, kin, vion, veff, vhxc, rho_in, rho_out, rho_next, psi, psi32;
-  DevBuf pos, beta, d0, dscr, q, occ;  // projector part
+  DevBuf pos, beta, d0, dscr, q, occ, wproj;  // projector part
   ~ScfState() {
     for (DevBuf* b : {&miller, &idx, &kin, &vion, &veff, &vhxc, &rho_in, &rho_out, &rho_next, &psi,
-                      &psi32, &pos, &beta, &d0, &dscr, &q, &occ})
+                      &psi32, &pos, &beta, &d0, &dscr, &q, &occ, &wproj})
       release(*b);
   }
 };
@@ -137,6 +137,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     ctx->h2d += 3 * (size_t)n_atoms * sizeof(double) + 3 * (size_t)P * sizeof(double);
     CHS_TRY(chs_bind_projectors(ctx, P, (const double*)S.beta.p, (const double*)S.dscr.p,
                                 (const double*)S.q.p));
+    CHS_TRY(ensure(ctx, S.wproj, (size_t)P * n_b * sizeof(z64)));
   }
   // launch shapes of the H|psi> kernels, tuned once per (grid, precision)
   // and context (later runs reuse them)
@@ -209,7 +210,10 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
       double tf1 = now();
       t_filter += tf1 - tf0;
       eps.resize(n_b);
-      CHS_TRY(rayleigh_ritz_impl(ctx, (z64*)S.psi.p, n_b, eps.data(), nullptr, nullptr));
+      // the last pass also hands over beta^H psi of the rotated block for
+      // the augmentation charge
+      CHS_TRY(rayleigh_ritz_impl(ctx, (z64*)S.psi.p, n_b, eps.data(), nullptr, nullptr,
+                                 P > 0 && ps + 1 == passes ? (z64*)S.wproj.p : nullptr));
       t_rr += now() - tf1;
     }
     tb = now();
@@ -221,7 +225,7 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
                                   hipMemcpyHostToDevice, ctx->stream));
       CHS_TRY(augment_density(ctx, (const z64*)S.psi.p, n_b, (const double*)S.occ.p, n_atoms, np,
                               p->q0, p->q_rc, (const z64*)G.ph.p, n, L, (z64*)G.cgrid.p,
-                              (double*)S.rho_out.p));
+                              (double*)S.rho_out.p, (const z64*)S.wproj.p));
     }
     double eh, exc, dc;
     CHS_TRY(G.vhxc((const double*)S.rho_out.p, (const double*)S.vion.p, nullptr, nullptr,
