@@ -36,6 +36,7 @@ namespace chs {
 
 struct StickArgs {
   int n_st, n_act, n_pw, rs;  // rs: row stride of the compact plane (= 1 mod 8)
+  int ld;                     // stick-buffer row stride: n_st rounded up to 32 (256-B rows)
   const int32_t* iy;
   const int32_t* act;
   const int32_t* pw0;
@@ -43,6 +44,7 @@ struct StickArgs {
   const int32_t* iz0;
   const int32_t* act_ix;
   const int32_t* act_of;  // N: compact index of x column, or -1
+  const int32_t* pw_siz;  // n_pw: (stick << 8) | z index of the plane wave
 };
 
 template <typename T> __device__ __forceinline__ cplx<T> zero_c() { return mk((T)0, (T)0); }
@@ -59,15 +61,12 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
   const int ns = min(SB, st.n_st - s0);
   for (int i = threadIdx.x; i < SB * LS; i += blockDim.x) lds[i] = zero_c<T>();
   __syncthreads();
+  // the block's sticks hold one contiguous plane-wave range
   const cplx<T>* pb = psi + (size_t)b * st.n_pw;
-  for (int t = threadIdx.x; t < ns * N; t += blockDim.x) {
-    const int sl = t / N, j = t - sl * N;
-    const int s = s0 + sl;
-    if (j < st.cnt[s]) {
-      int iz = st.iz0[s] + j;
-      if (iz >= N) iz -= N;
-      lds[sl * LS + iz] = pb[st.pw0[s] + j];
-    }
+  const int p_end = st.pw0[s0 + ns - 1] + st.cnt[s0 + ns - 1];
+  for (int gi = st.pw0[s0] + threadIdx.x; gi < p_end; gi += blockDim.x) {
+    const int si = st.pw_siz[gi];
+    lds[((si >> 8) - s0) * LS + (si & 255)] = pb[gi];
   }
   __syncthreads();
   int lane;
@@ -75,10 +74,10 @@ __global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ p
   if (g >= 0)
     for (int sl = g; sl < ns; sl += ng) group_fft_line<N, 1>(lds + sl * LS, 1, lane, tw, PostId());
   __syncthreads();
-  cplx<T>* out = sbuf + (size_t)b * N * st.n_st + s0;
+  cplx<T>* out = sbuf + (size_t)b * N * st.ld + s0;
   for (int t = threadIdx.x; t < N * SB; t += blockDim.x) {
     const int z = t / SB, sl = t - z * SB;
-    if (sl < ns) out[(size_t)z * st.n_st + sl] = lds[sl * LS + z];
+    if (sl < ns) out[(size_t)z * st.ld + sl] = lds[sl * LS + z];
   }
 }
 
@@ -122,7 +121,7 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
   int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
-  cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.n_st;
+  cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.ld;
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
   __syncthreads();
@@ -172,10 +171,10 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
   const int s0 = blockIdx.x * SB;
   const int b = blockIdx.y;
   const int ns = min(SB, st.n_st - s0);
-  const cplx<T>* in = sbuf + (size_t)b * N * st.n_st + s0;
+  const cplx<T>* in = sbuf + (size_t)b * N * st.ld + s0;
   for (int t = threadIdx.x; t < N * SB; t += blockDim.x) {
     const int z = t / SB, sl = t - z * SB;
-    if (sl < ns) lds[sl * LS + z] = in[(size_t)z * st.n_st + sl];
+    if (sl < ns) lds[sl * LS + z] = in[(size_t)z * st.ld + sl];
   }
   __syncthreads();
   int lane;
@@ -185,30 +184,25 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
   __syncthreads();
   const size_t boff = (size_t)b * st.n_pw;
   double so = 0.0, sp = 0.0;
-  for (int t = threadIdx.x; t < ns * N; t += blockDim.x) {
-    const int sl = t / N, j = t - sl * N;
-    const int s = s0 + sl;
-    if (j < st.cnt[s]) {
-      int iz = st.iz0[s] + j;
-      if (iz >= N) iz -= N;
-      const int gi = st.pw0[s] + j;
-      const cplx<T> v = lds[sl * LS + iz];
-      const cplx<T> p = psi[boff + gi];
-      const T k = (T)kin[gi];
-      cplx<T> h = mk(k * p.x + inv_n3 * v.x, k * p.y + inv_n3 * v.y);
-      cplx<T> o;
-      if (MODE == EPI_HPSI) {
-        o = h;
-      } else if (MODE == EPI_CHEB1) {
-        o = mk((h.x - c * p.x) * inv_e, (h.y - c * p.y) * inv_e);
-      } else {
-        const cplx<T> q = aux0[boff + gi];
-        o = mk((T)2 * (h.x - c * p.x) * inv_e - q.x, (T)2 * (h.y - c * p.y) * inv_e - q.y);
-      }
-      out[boff + gi] = o;
-      so += (double)o.x * o.x + (double)o.y * o.y;
-      sp += (double)p.x * p.x + (double)p.y * p.y;
+  const int p_end = st.pw0[s0 + ns - 1] + st.cnt[s0 + ns - 1];
+  for (int gi = st.pw0[s0] + threadIdx.x; gi < p_end; gi += blockDim.x) {
+    const int si = st.pw_siz[gi];
+    const cplx<T> v = lds[((si >> 8) - s0) * LS + (si & 255)];
+    const cplx<T> p = psi[boff + gi];
+    const T k = (T)kin[gi];
+    cplx<T> h = mk(k * p.x + inv_n3 * v.x, k * p.y + inv_n3 * v.y);
+    cplx<T> o;
+    if (MODE == EPI_HPSI) {
+      o = h;
+    } else if (MODE == EPI_CHEB1) {
+      o = mk((h.x - c * p.x) * inv_e, (h.y - c * p.y) * inv_e);
+    } else {
+      const cplx<T> q = aux0[boff + gi];
+      o = mk((T)2 * (h.x - c * p.x) * inv_e - q.x, (T)2 * (h.y - c * p.y) * inv_e - q.y);
     }
+    out[boff + gi] = o;
+    so += (double)o.x * o.x + (double)o.y * o.y;
+    sp += (double)p.x * p.x + (double)p.y * p.y;
   }
   if (part) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -257,7 +251,7 @@ __global__ void __launch_bounds__(256) k_density_plane(const z64* __restrict__ s
   for (int b = b0; b < b1; ++b) {
     const double f = occ[b];
     if (f == 0.0) continue;
-    const z64* plane = sbuf + ((size_t)b * N + z) * st.n_st;
+    const z64* plane = sbuf + ((size_t)b * N + z) * st.ld;
     __syncthreads();
     for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = mk(0.0, 0.0);
     __syncthreads();
@@ -327,11 +321,20 @@ int build_sticks(chs_context* ctx) {
   std::vector<int32_t> act(n_st);
   for (int s = 0; s < n_st; ++s) act[s] = act_of[ix[s]];
   const int n_act = (int)act_ix.size();
+  // plane wave -> (stick, z index): sticks are consecutive plane-wave ranges
+  std::vector<int32_t> pw_siz(n_pw);
+  for (int s = 0; s < n_st; ++s)
+    for (int j = 0; j < cnt[s]; ++j) {
+      int iz = iz0[s] + j;
+      if (iz >= n) iz -= n;
+      pw_siz[pw0[s] + j] = (s << 8) | iz;
+    }
   std::vector<int32_t> all;
-  all.reserve(5 * (size_t)n_st + n_act + n);
+  all.reserve(5 * (size_t)n_st + n_act + n + n_pw);
   for (auto* v : {&iy, &act, &pw0, &cnt, &iz0}) all.insert(all.end(), v->begin(), v->end());
   all.insert(all.end(), act_ix.begin(), act_ix.end());
   all.insert(all.end(), act_of.begin(), act_of.end());
+  all.insert(all.end(), pw_siz.begin(), pw_siz.end());
   CHS_TRY(ensure(ctx, H.st_mem, all.size() * sizeof(int32_t)));
   CHS_HIP(ctx, hipMemcpyAsync(H.st_mem.p, all.data(), all.size() * sizeof(int32_t),
                               hipMemcpyHostToDevice, ctx->stream));
@@ -350,13 +353,19 @@ int build_sticks(chs_context* ctx) {
   st.iz0 = d + 4 * (size_t)n_st;
   st.act_ix = d + 5 * (size_t)n_st;
   st.act_of = d + 5 * (size_t)n_st + n_act;
+  st.pw_siz = d + 5 * (size_t)n_st + n_act + n;
   st.h_act_ix = act_ix;
   return CHS_OK;
 }
 
+// Row stride of the stick buffer sbuf[b][z][stick]: whole 256-byte lines per
+// z-row, so the SB-stick chunks of the z passes never straddle a line.
+static inline int stick_ld(int n_st) { return (n_st + 31) / 32 * 32; }
+
 static StickArgs stick_args(const StickTable& st) {
   StickArgs a;
   a.n_st = st.n_st;
+  a.ld = stick_ld(st.n_st);
   a.n_act = st.n_act;
   a.n_pw = st.n_pw;
   a.rs = st.n_act + 1;
@@ -368,6 +377,7 @@ static StickArgs stick_args(const StickTable& st) {
   a.iz0 = st.iz0;
   a.act_ix = st.act_ix;
   a.act_of = st.act_of;
+  a.pw_siz = st.pw_siz;
   return a;
 }
 
@@ -472,7 +482,7 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   const StickTable& st = H.st;
   const int n = H.n_grid;
   using C = cplx<T>;
-  CHS_TRY(ensure(ctx, ctx->impl->stick_buf, (size_t)n_b * n * st.n_st * sizeof(C)));
+  CHS_TRY(ensure(ctx, ctx->impl->stick_buf, (size_t)n_b * n * stick_ld(st.n_st) * sizeof(C)));
   C* sbuf = (C*)ctx->impl->stick_buf.p;
   const void* twv = nullptr;
   CHS_TRY(twiddles(ctx, n, precision, &twv));
@@ -605,7 +615,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
     CHS_HIP(ctx, hipMemsetAsync(d_rho, 0, n3 * sizeof(double), ctx->stream));
     return CHS_OK;
   }
-  CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * st.n_st * sizeof(z64)));
+  CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * stick_ld(st.n_st) * sizeof(z64)));
   z64* sbuf = (z64*)im->stick_buf.p;
   const void* twv = nullptr;
   CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));

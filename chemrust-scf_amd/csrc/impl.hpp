@@ -25,6 +25,7 @@ struct StickTable {
   int32_t* iz0 = nullptr;    // FFT z index of the first plane wave
   int32_t* act_ix = nullptr; // n_act: ix of each active column
   int32_t* act_of = nullptr; // n_grid: active index of each x column, or -1
+  int32_t* pw_siz = nullptr; // n_pw: (stick << 8) | FFT z index of each plane wave
   std::vector<int32_t> h_act_ix;
 };
 
