@@ -232,8 +232,8 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
 // plane, inverse y-FFT, then per row: expand and inverse x-FFT whose final
 // store accumulates occ * |psi(r)|^2 into the block's own partial plane
 // part[chunk][z][N^2] (each element owned by one lane: no atomics).
-template <int N>
-__global__ void __launch_bounds__(256) k_density_plane(const z64* __restrict__ sbuf, StickArgs st,
+template <int N, int TPB>
+__global__ void __launch_bounds__(TPB) k_density_plane(const z64* __restrict__ sbuf, StickArgs st,
                                                        const double* __restrict__ occ, int n_b,
                                                        int chunk, int nrb, double* __restrict__ part,
                                                        const z64* __restrict__ tw) {
@@ -603,7 +603,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   int nb = 0;  // bands up to the last occupied one
   for (int b = 0; b < n_b; ++b)
     if (occ_host[b] != 0.0) nb = b + 1;
-  const int chunk = 16;
+  const int chunk = 8;  // bands per block (more blocks in flight; partial planes are cheap)
   const int n_chunks = std::max(1, (nb + chunk - 1) / chunk);
   CHS_TRY(ensure(ctx, im->ws6, (size_t)n_b * sizeof(double) + (size_t)n_chunks * n3 * sizeof(double)));
   double* docc = (double*)im->ws6.p;
@@ -621,17 +621,47 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
   CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));
   const z64* tw = (const z64*)twv;
   StickArgs sa = stick_args(st);
-  constexpr int SB = 32;
+  constexpr int SB = 16;
   const int gz = ceil_div(st.n_st, SB);
+  // workgroup size of the density planes (256 or 512 threads), timed once per
+  // grid on the first call (both give bit-identical partial planes)
+  auto tpb_it = im->density_tpb.find(n);
+  const bool tune = tpb_it == im->density_tpb.end();
   CHS_DISPATCH_N(n, {
-    PlaneLaunch pl;
-    CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, 256, NN + 1, &pl));
     hipLaunchKernelGGL((k_stick_inv<NN, double, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL((k_density_plane<NN>), dim3(NN, n_chunks), dim3(256), pl.lds, ctx->stream,
-                       sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
-    CHS_LAUNCH_CHECK(ctx);
+    auto planes = [&](int tpb) -> int {
+      PlaneLaunch pl;
+      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, tpb, NN + 1, &pl));
+      if (tpb == 512)
+        hipLaunchKernelGGL((k_density_plane<NN, 512>), dim3(NN, n_chunks), dim3(512), pl.lds,
+                           ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+      else
+        hipLaunchKernelGGL((k_density_plane<NN, 256>), dim3(NN, n_chunks), dim3(256), pl.lds,
+                           ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+      CHS_LAUNCH_CHECK(ctx);
+      return CHS_OK;
+    };
+    if (tune) {
+      int best = 256;
+      float best_t = 1e30f;
+      for (int tpb : {256, 512}) {
+        CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+        CHS_TRY(planes(tpb));
+        CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+        CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+        float t = 0;
+        CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+        if (t < best_t) {
+          best_t = t;
+          best = tpb;
+        }
+      }
+      im->density_tpb[n] = best;
+    } else {
+      CHS_TRY(planes(tpb_it->second));
+    }
   });
   hipLaunchKernelGGL(k_density_reduce, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, part,
                      n_chunks, n3, 1.0 / omega, d_rho);

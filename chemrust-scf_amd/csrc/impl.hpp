@@ -88,6 +88,7 @@ struct VlocTune {
 struct chs_context::Impl {
   std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
   std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
+  std::map<int, int> density_tpb;                          // N -> density-plane workgroup size
   bool hpsi_autotune = true;  // chs_scf_run tunes new grids (CHS_HPSI_TUNE=0: off)
   chs::Hamiltonian ham;
   // workspaces
