@@ -116,6 +116,10 @@ int chs_context_destroy(chs_context* ctx) {
   release_hamiltonian(im->ham);
   for (Hamiltonian& h : im->kset) release_hamiltonian(h);
   if (im->kset_graph) (void)hipGraphExecDestroy(im->kset_graph);
+  if (im->eigh_graph.exec) (void)hipGraphExecDestroy(im->eigh_graph.exec);
+  if (im->jacobi.exec) (void)hipGraphExecDestroy(im->jacobi.exec);
+  release(im->jacobi.sched);
+  release(im->jacobi.rot);
   (void)hipEventDestroy(im->ev0);
   (void)hipEventDestroy(im->ev1);
   (void)hipStreamDestroy(ctx->stream);

@@ -358,20 +358,15 @@ __global__ void k_tri_to_complex(const double* __restrict__ Z, int n, z64* __res
 
 // Triangular factor T (b x b, column-major, upper) of the compact WY form
 // H_k0 ... H_{k0+b-1} = I - V T V^H (forward, columnwise):
-// T_cc = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] (V[:, 0:c]^H v_c).
-__global__ void __launch_bounds__(256) k_wy_tmat(const z64* __restrict__ Vst, int n, int k0, int b,
+// T_cc = tau_c, T[0:c, c] = -tau_c T[0:c, 0:c] (V[:, 0:c]^H v_c), with the
+// Gram matrix Gm = V^H V (b x b, column-major) computed beforehand by a GEMM.
+__global__ void __launch_bounds__(256) k_wy_tmat(const z64* __restrict__ Gm, int k0, int b,
                                                  const z64* __restrict__ tau, z64* __restrict__ T) {
   __shared__ z64 G[kWY][kWY + 1];
   __shared__ z64 Ts[kWY][kWY + 1];
   for (int t = threadIdx.x; t < kWY * kWY; t += blockDim.x) {
     const int a = t % kWY, c = t / kWY;
-    z64 g = mk(0.0, 0.0);
-    if (a < c && c < b) {
-      const z64* va = Vst + (size_t)(k0 + a) * n;
-      const z64* vc = Vst + (size_t)(k0 + c) * n;
-      for (int r = k0 + c + 1; r < n; ++r) g = g + cmulc(va[r], vc[r]);  // v_c is 0 above row k0+c+1
-    }
-    G[a][c] = g;
+    G[a][c] = (a < c && c < b) ? Gm[a + (size_t)c * b] : mk(0.0, 0.0);
     Ts[a][c] = mk(0.0, 0.0);
   }
   __syncthreads();
@@ -388,14 +383,6 @@ __global__ void __launch_bounds__(256) k_wy_tmat(const z64* __restrict__ Vst, in
   }
   for (int t = threadIdx.x; t < b * b; t += blockDim.x) T[t] = Ts[t % b][t / b];
 }
-
-struct EighPlan {
-  int n = 0;
-  const z64* gA = nullptr;
-  const void* gws = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
-static EighPlan g_eigh;
 
 // Workspace layout (bytes aligned to 256 by construction of the sizes).
 struct EighWs {
@@ -454,7 +441,7 @@ int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U) {
   EighWs ws;
   CHS_TRY(eigh_ws(ctx, n, &ws));
   // 1. tridiagonal reduction (2 (n - 1) launches; graph-replayed)
-  EighPlan& P = g_eigh;
+  EighGraph& P = ctx->impl->eigh_graph;
   if (ctx->impl->use_graphs && n > 1) {
     if (!P.exec || P.n != n || P.gA != A || P.gws != ctx->impl->eig_ws.p) {
       if (P.exec) (void)hipGraphExecDestroy(P.exec);
@@ -489,10 +476,12 @@ int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U) {
   for (int kb = ((nref - 1) / kWY) * kWY; nref > 0 && kb >= 0; kb -= kWY) {
     const int b = std::min(kWY, nref - kb);
     const int rows = n - kb - 1;
-    hipLaunchKernelGGL(k_wy_tmat, dim3(1), dim3(256), 0, ctx->stream, ws.vst, n, kb, b, ws.tau,
+    const z64* V = ws.vst + (size_t)kb * n + (kb + 1);
+    // V^H V into w2 (free until the T w1 product below)
+    CHS_TRY(zgemm(ctx, 'C', 'N', b, b, rows, one, V, n, V, n, zero, ws.w2, b));
+    hipLaunchKernelGGL(k_wy_tmat, dim3(1), dim3(256), 0, ctx->stream, ws.w2, kb, b, ws.tau,
                        ws.tmat);
     CHS_LAUNCH_CHECK(ctx);
-    const z64* V = ws.vst + (size_t)kb * n + (kb + 1);
     CHS_TRY(zgemm(ctx, 'C', 'N', b, n, rows, one, V, n, U + (kb + 1), n, zero, ws.w1, b));
     CHS_TRY(zgemm(ctx, 'N', 'N', b, n, b, one, ws.tmat, b, ws.w1, b, zero, ws.w2, b));
     CHS_TRY(zgemm(ctx, 'N', 'N', rows, n, b, mone, V, n, ws.w2, b, one, U + (kb + 1), n));

@@ -447,7 +447,7 @@ int gemm_opcode(char op) {
 // device, K is cut into slices of whole K tiles until about `target`
 // workgroups are in flight (more resident waves per CU hide the operand
 // latency of the long-K products better; the autotuner picks the target).
-static constexpr int kSplitTargets[3] = {512, 2048, 4096};
+static constexpr int kSplitTargets[4] = {512, 1024, 2048, 4096};
 
 template <typename T>
 static void tile_dims(int variant, int* bm, int* bn, int* bk) {
@@ -463,8 +463,10 @@ static void split_plan(int m, int n, int k, int bm, int bn, int bk, int target, 
                        int* klen) {
   const int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
   int ks = 1;
+  // at most `target` workgroups: whole rounds of resident workgroups, no
+  // nearly empty last round
   if (tiles < target && k > 4 * bk)
-    ks = std::max(1, std::min(target / tiles + 1, (k + 255) / 256));
+    ks = std::max(1, std::min(target / tiles, (k + 255) / 256));
   int kl = (k + ks - 1) / ks;
   kl = ((kl + bk - 1) / bk) * bk;
   if (kl == 0) kl = bk;
@@ -476,7 +478,7 @@ template <typename T>
 static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int n, int k,
                        cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
                        cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws) {
-  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % 3];
+  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % 4];
   int bm, bn, bk;
   tile_dims<T>(variant, &bm, &bn, &bk);
   int ksplit, klen;
@@ -543,7 +545,7 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   double best = 1e300;
   for (int v = 0; v < 3; ++v) {
     int prev_split = -1;
-    for (int ti = 0; ti < 3; ++ti) {
+    for (int ti = 0; ti < 4; ++ti) {
       int bm, bn, bk, ks, kl;
       tile_dims<T>(v, &bm, &bn, &bk);
       split_plan(m, n, k, bm, bn, bk, kSplitTargets[ti], &ks, &kl);

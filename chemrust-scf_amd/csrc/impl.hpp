@@ -64,6 +64,27 @@ inline void release_hamiltonian(Hamiltonian& h) {
   h = Hamiltonian{};
 }
 
+// Captured launch sequence of the Householder tridiagonal reduction, valid
+// for one (n, matrix, workspace) triple.
+struct EighGraph {
+  int n = 0;
+  const void* gA = nullptr;
+  const void* gws = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+// Round-robin pair schedule and rotation buffer of the Jacobi eigensolver,
+// with one captured sweep for the last (matrix, vectors) pair.
+struct JacobiPlan {
+  int n = 0;
+  int rounds = 0, pairs = 0;
+  DevBuf sched;  // int2 [rounds][pairs]
+  DevBuf rot;    // double4 [pairs]
+  hipGraphExec_t exec = nullptr;
+  const void* gA = nullptr;
+  const void* gV = nullptr;
+};
+
 // GEMM shape key of the kernel-variant cache.
 struct GemmKey {
   int tsize, oa, ob, m, n, k;
@@ -116,6 +137,8 @@ struct chs_context::Impl {
   int kset_cur = -1;
   chs::DevBuf kset_ws;          // per-k filter windows {c, inv_e} and flags
   hipGraphExec_t kset_graph = nullptr;
+  chs::EighGraph eigh_graph;  // captured tridiagonal reduction of the last (n, A, workspace)
+  chs::JacobiPlan jacobi;     // Jacobi eigensolver schedule (CHS_EIGH_JACOBI)
   std::vector<uintptr_t> kset_key;
   uint64_t kset_key_gen = 0;
 };

@@ -352,17 +352,6 @@ __global__ void __launch_bounds__(1024) k_sort_eig(const z64* __restrict__ A, co
   }
 }
 
-struct JacobiPlan {
-  int n = 0;
-  int rounds = 0, pairs = 0;
-  DevBuf sched;  // int2 [rounds][pairs]
-  DevBuf rot;    // double4 [pairs]
-  hipGraphExec_t exec = nullptr;  // one captured sweep for (gA, gV)
-  const z64* gA = nullptr;
-  const z64* gV = nullptr;
-};
-static JacobiPlan g_plan;  // guarded per context device by rebuild on size change
-
 static int jacobi_plan(chs_context* ctx, int n, JacobiPlan& P) {
   if (P.n == n && P.sched.p) return CHS_OK;
   if (P.exec) (void)hipGraphExecDestroy(P.exec);  // the captured sweep bakes in n and buffers
@@ -398,7 +387,7 @@ static int jacobi_plan(chs_context* ctx, int n, JacobiPlan& P) {
 // device w, eigenvectors in U. Work: V (n x n).
 int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int max_sweeps,
                 int* sweeps_out) {
-  JacobiPlan& P = g_plan;
+  JacobiPlan& P = ctx->impl->jacobi;
   CHS_TRY(jacobi_plan(ctx, n, P));
   hipLaunchKernelGGL(k_set_identity, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0, ctx->stream,
                      V, n);
