@@ -131,6 +131,7 @@ def lib():
             "chs_cgemm": [vp, C.c_char, C.c_char, i, i, i, C.POINTER(C.c_float), vp, i, vp, i,
                           C.POINTER(C.c_float), vp, i],
             "chs_profile_gemm": [vp, i, C.c_char, C.c_char, i, i, i, i, dp],
+            "chs_profile_projector_products": [vp, i, i, i, dp],
             "chs_kset_bind": [vp, i, i, vp, vp, vp, vp, i, vp, vp, vp],
             "chs_kset_set_veff": [vp, vp],
             "chs_kset_select": [vp, i],
@@ -464,6 +465,14 @@ def run_scf(ctx, params, pos, want_rho=False):
     out["rho"] = rho
     out["status"] = rc
     return out
+
+
+def profile_projector_products(ctx, n_b, precision=C64, reps=5):
+    """Mean ms of the filter's two projector products (G = beta^H O,
+    O + beta C) on the bound projectors, in the form the filter uses."""
+    ms = np.zeros(2)
+    ctx.check(lib().chs_profile_projector_products(ctx.h, precision, n_b, reps, _dptr(ms)))
+    return ms
 
 
 def profile_gemm(ctx, precision, opa, opb, m, n, k, reps=5):

@@ -141,15 +141,23 @@ template <> struct MfmaTile<double, 2> {
 // XCD-aware block order: workgroups are dealt to the eight XCDs round-robin
 // by launch index, so physical block p runs on XCD p % 8. Logical block
 // L = (p % 8) * (P / 8) + p / 8 gives every XCD a contiguous range of
-// logical blocks, ordered (k-slice, column tile, row tile) with the row
-// tile fastest: the tiles that share a k-slice of A and B, and the column
-// tiles that share rows of A, sit in one L2.
-__device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int* bm, int* bn, int* bz) {
+// logical blocks, ordered (k-slice, then tiles) so that the tiles sharing a
+// k-slice sit in one L2. Within a slice the tiles that share the larger
+// operand are consecutive (nfast: column tile fastest, so the column tiles
+// of one row tile of A run together and A streams from HBM about once;
+// otherwise row tile fastest, for a B wider than A is tall).
+__device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int nfast, int* bm, int* bn,
+                                           int* bz) {
   const int P = gridDim.x, p = blockIdx.x;
   const int L = (p & 7) * (P >> 3) + (p >> 3);
   if (L >= mt * nt * kz) return false;
-  *bm = L % mt;
-  *bn = (L / mt) % nt;
+  if (nfast) {
+    *bn = L % nt;
+    *bm = (L / nt) % mt;
+  } else {
+    *bm = L % mt;
+    *bn = (L / mt) % nt;
+  }
   *bz = L / (mt * nt);
   return true;
 }
@@ -160,7 +168,7 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
                                                    const cplx<T>* __restrict__ A, int lda,
                                                    const cplx<T>* __restrict__ B, int ldb,
                                                    cplx<T> beta, cplx<T>* __restrict__ C, int ldc,
-                                                   int klen, int ksplit,
+                                                   int klen, int ksplit, int nfast,
                                                    cplx<T>* __restrict__ part) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -168,7 +176,7 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
   constexpr int BM = TL::BM, BN = TL::BN, BK = TL::BK;
   constexpr int WTM = TL::WTM, WTN = TL::WTN;
   int bm, bn, bz;
-  if (!gemm_block((m + BM - 1) / BM, (n + BN - 1) / BN, ksplit, &bm, &bn, &bz)) return;
+  if (!gemm_block((m + BM - 1) / BM, (n + BN - 1) / BN, ksplit, nfast, &bm, &bn, &bz)) return;
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // staged elements per thread
   // one padding element per row: the K-major (transposing) stores of T / C
   // operands then hit distinct banks across each phase of lanes, while the
@@ -489,16 +497,17 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
     part = (cplx<T>*)ws.p;
   }
   const unsigned nblk = ceil_div(m, bm) * ceil_div(n, bn) * (unsigned)ksplit;
+  const int nfast = m > n ? 1 : 0;  // tiles sharing the larger operand run together
   const dim3 g1(((nblk + 7) / 8) * 8);  // multiple of 8 for the XCD-aware order
   const dim3 g3(ceil_div(m, bm), ceil_div(n, bn), ksplit);
 #define CHS_GEMM_CASE(OA, OB)                                                                    \
   if (oa == OA && ob == OB) {                                                                    \
     if (variant == 1)                                                                            \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
-                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, part);               \
+                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);        \
     else if (variant == 2)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
-                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, part);               \
+                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);        \
     else                                                                                         \
       hipLaunchKernelGGL((k_gemm<T, OA, OB>), g3, dim3(256), 0, ctx->stream, m, n, k, alpha, A,  \
                          lda, B, ldb, beta, C, ldc, klen, part);                                 \

@@ -29,6 +29,15 @@ struct StickTable {
   std::vector<int32_t> h_act_ix;
 };
 
+// Six fp16 planes [m][k] of a fixed complex operand for the fp16x3 GEMM
+// (ar, ai, ar + ai, each as hi and lo of the value scaled by 2^ea); rows
+// padded to m_pad, row pitch ld (both with zero padding).
+struct H3Planes {
+  DevBuf buf;
+  int m = 0, k = 0, m_pad = 0, ld = 0, ea = 0;
+  size_t plane = 0;  // halfs per plane
+};
+
 struct Hamiltonian {
   bool bound = false;
   int n_grid = 0, n_pw = 0;
@@ -55,11 +64,15 @@ struct Hamiltonian {
   DevBuf nl_ws;
   int nl_wp = 0, nl_w = 1, nl_g = 2;
   DevBuf fscr;                   // second Chebyshev buffer of a k-point set member
+  // fp16x3 split planes of beta^H and beta (complex64 filter), and the
+  // per-column exponents of the two products' right operands (2 n_b ints)
+  H3Planes h3_cn, h3_nn;
+  DevBuf h3_exp;
 };
 
 inline void release_hamiltonian(Hamiltonian& h) {
   for (DevBuf* b : {&h.veff_f, &h.st_mem, &h.beta_f, &h.sinv_m, &h.sinv_m_f, &h.bmat, &h.bmat_f,
-                    &h.nl_ws, &h.fscr})
+                    &h.nl_ws, &h.fscr, &h.h3_cn.buf, &h.h3_nn.buf, &h.h3_exp})
     release(*b);
   h = Hamiltonian{};
 }
@@ -123,6 +136,9 @@ struct chs_context::Impl {
   // Complex GEMMs on the matrix cores (MFMA); CHS_GEMM_SIMT=1 selects the
   // vector-FMA kernel of the same tiling instead.
   bool use_mfma = true;
+  // complex64 filter projector products on the fp16 matrix cores (fp16x3);
+  // opt-in: the present kernel is latency-bound and slower than the FP32 form
+  bool use_h3 = false;
   // per-shape GEMM kernel variant, timed on first use (CHS_GEMM_TUNE=0: off)
   bool gemm_autotune = true;
   std::map<chs::GemmKey, int> gemm_tune;

@@ -21,5 +21,13 @@ int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w);
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
                        double* eps_dev_out, z64* w_out = nullptr);
 int cholqr_impl(chs_context* ctx, z64* Y, int n_b);
+// fp16x3 complex GEMM (gemm_h3.hip): split planes of op(A) for a c128
+// projector block (P x n_pw, lda; cn: op(A) = A^H, else A), per-column
+// exponents of a c64 operand, and C = alpha op(A) B + beta C.
+int h3_split_planes(chs_context* ctx, const z64* A, int lda, int P, int n_pw, bool cn,
+                    H3Planes& out);
+int h3_column_exponents(chs_context* ctx, const c32* B, int k, int n, int ldb, int* d_eb);
+int h3_gemm(chs_context* ctx, const H3Planes& A, int n, c32 alpha, const c32* B, int ldb,
+            const int* d_eb, c32 beta, c32* C, int ldc);
 
 }  // namespace chs

@@ -69,6 +69,10 @@ static int ensure_c64_projectors(chs_context* ctx) {
     }
     CHS_LAUNCH_CHECK(ctx);
   }
+  if (ctx->impl->use_h3 && !H.h3_cn.buf.p) {
+    CHS_TRY(h3_split_planes(ctx, (const z64*)H.beta, H.n_pw, H.n_proj, H.n_pw, true, H.h3_cn));
+    CHS_TRY(h3_split_planes(ctx, (const z64*)H.beta, H.n_pw, H.n_proj, H.n_pw, false, H.h3_nn));
+  }
   return CHS_OK;
 }
 
@@ -175,6 +179,15 @@ static int cheb_nl_begin_t(chs_context* ctx, const cplx<T>* X0, int n_b) {
   H.nl_w = 1;
   H.nl_g = 2;
   const cplx<T>* beta = sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
+  if constexpr (sizeof(T) == 4) {
+    if (ctx->impl->use_h3) {
+      CHS_TRY(ensure(ctx, H.h3_exp, 2 * (size_t)n_b * sizeof(int)));
+      int* eb = (int*)H.h3_exp.p;
+      CHS_TRY(h3_column_exponents(ctx, X0, H.n_pw, n_b, H.n_pw, eb));
+      return h3_gemm(ctx, H.h3_cn, n_b, mk(1.f, 0.f), X0, H.n_pw, eb, mk(0.f, 0.f),
+                     nl_slot<T>(H, n_b, H.nl_w), P);
+    }
+  }
   return gemm_any<T>(ctx, 'C', 'N', P, n_b, H.n_pw, 1.0, beta, H.n_pw, X0, H.n_pw, 0.0,
                      nl_slot<T>(H, n_b, H.nl_w), P);
 }
@@ -194,8 +207,24 @@ static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, con
   cplx<T>* Cs = nl_slot<T>(H, n_b, 4);
   const size_t pb = (size_t)P * n_b;
   const unsigned gp = ceil_div(pb, 256);
+  // complex64 with fp16x3: both n_pw-long products on the fp16 matrix cores
+  bool h3 = false;
+  int* eb = nullptr;
+  if constexpr (sizeof(T) == 4) {
+    h3 = ctx->impl->use_h3;
+    if (h3) {
+      CHS_TRY(ensure(ctx, H.h3_exp, 2 * (size_t)n_b * sizeof(int)));
+      eb = (int*)H.h3_exp.p;
+    }
+  }
   // G = beta^H O
-  CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P));
+  if constexpr (sizeof(T) == 4) {
+    if (h3) {
+      CHS_TRY(h3_column_exponents(ctx, O, n_pw, n_b, n_pw, eb));
+      CHS_TRY(h3_gemm(ctx, H.h3_cn, n_b, mk(1.f, 0.f), O, n_pw, eb, mk(0.f, 0.f), G, P));
+    }
+  }
+  if (!h3) CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P));
   hipLaunchKernelGGL((k_cheb_coef<T>), dim3(gp), dim3(256), 0, ctx->stream, G, Wp, W, H.dion,
                      coef, (int)first, P, pb, H.has_s ? Z : nullptr, Cs);
   CHS_LAUNCH_CHECK(ctx);
@@ -208,7 +237,13 @@ static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, con
     CHS_LAUNCH_CHECK(ctx);
   }
   // X_{k+1} = O + beta Cs ; W_{k+1} = G + B Cs (into G's slot)
-  CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, Cs, P, 1.0, O, n_pw));
+  if constexpr (sizeof(T) == 4) {
+    if (h3) {
+      CHS_TRY(h3_column_exponents(ctx, Cs, P, n_b, P, eb + n_b));
+      CHS_TRY(h3_gemm(ctx, H.h3_nn, n_b, mk(1.f, 0.f), Cs, P, eb + n_b, mk(1.f, 0.f), O, n_pw));
+    }
+  }
+  if (!h3) CHS_TRY(gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, Cs, P, 1.0, O, n_pw));
   CHS_TRY(gemm_any<T>(ctx, 'N', 'N', P, n_b, P, 1.0, B, P, Cs, P, 1.0, G, P));
   const int old_wp = H.nl_wp;
   H.nl_wp = H.nl_w;
@@ -468,7 +503,82 @@ int screen_dion(chs_context* ctx, const double* veff, int n, double L, int n_ato
 
 using namespace chs;
 
+// Synthetic operand values in [-0.005, 0.005) for timing.
+__global__ void k_proj_fill(float* __restrict__ a, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    a[i] = ((float)(h & 0xFFFF) / 65536.0f - 0.5f) * 1e-2f;
+  }
+}
+
+// One filter step's two n_pw-long projector products, G = beta^H O and
+// O + beta C, in the form the filter uses at `precision` (complex64: FP32
+// matrix cores, or fp16x3 with its exponent kernels when enabled).
+template <typename T>
+static int projector_products(chs_context* ctx, cplx<T>* O, cplx<T>* Cs, cplx<T>* G, int n_b,
+                              int which) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  if constexpr (sizeof(T) == 4) {
+    if (ctx->impl->use_h3) {
+      CHS_TRY(ensure(ctx, H.h3_exp, 2 * (size_t)n_b * sizeof(int)));
+      int* eb = (int*)H.h3_exp.p;
+      if (which == 0) {
+        CHS_TRY(h3_column_exponents(ctx, O, n_pw, n_b, n_pw, eb));
+        return h3_gemm(ctx, H.h3_cn, n_b, mk(1.f, 0.f), O, n_pw, eb, mk(0.f, 0.f), G, P);
+      }
+      CHS_TRY(h3_column_exponents(ctx, Cs, P, n_b, P, eb + n_b));
+      return h3_gemm(ctx, H.h3_nn, n_b, mk(1.f, 0.f), Cs, P, eb + n_b, mk(1.f, 0.f), O, n_pw);
+    }
+  }
+  const cplx<T>* beta = sizeof(T) == 4 ? (const cplx<T>*)H.beta_f.p : (const cplx<T>*)H.beta;
+  if (which == 0)
+    return gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P);
+  return gemm_any<T>(ctx, 'N', 'N', n_pw, n_b, P, 1.0, beta, n_pw, Cs, P, 1.0, O, n_pw);
+}
+
 extern "C" {
+
+int chs_profile_projector_products(chs_context* ctx, int precision, int n_b, int reps,
+                                   double* ms2) {
+  if (!ctx || n_b <= 0 || reps <= 0 || !ms2 || (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  auto* im = ctx->impl;
+  Hamiltonian& H = im->ham;
+  if (!H.bound || H.n_proj == 0) return set_error(ctx, CHS_ERR_INVALID_ARG, "no projectors bound");
+  if (precision == CHS_C64) CHS_TRY(ensure_c64_projectors(ctx));
+  const size_t cs = precision == CHS_C128 ? 16 : 8;
+  const size_t no = (size_t)H.n_pw * n_b, nc = (size_t)H.n_proj * n_b;
+  CHS_TRY(ensure(ctx, im->ws1, no * cs));
+  CHS_TRY(ensure(ctx, im->ws2, nc * cs));
+  CHS_TRY(ensure(ctx, im->ws3, nc * cs));
+  for (auto pr : {std::make_pair(im->ws1.p, no), std::make_pair(im->ws2.p, nc)}) {
+    const size_t nf = pr.second * cs / sizeof(float);
+    hipLaunchKernelGGL(k_proj_fill, dim3(ceil_div(nf, 256)), dim3(256), 0, ctx->stream,
+                       (float*)pr.first, nf);
+  }
+  CHS_LAUNCH_CHECK(ctx);
+  for (int which = 0; which < 2; ++which) {
+    auto run = [&]() -> int {
+      if (precision == CHS_C128)
+        return projector_products<double>(ctx, (z64*)im->ws1.p, (z64*)im->ws2.p, (z64*)im->ws3.p,
+                                          n_b, which);
+      return projector_products<float>(ctx, (c32*)im->ws1.p, (c32*)im->ws2.p, (c32*)im->ws3.p,
+                                       n_b, which);
+    };
+    CHS_TRY(run());  // warm-up: tuning and workspace growth
+    CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+    for (int r = 0; r < reps; ++r) CHS_TRY(run());
+    CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+    CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+    float t = 0;
+    CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+    ms2[which] = t / reps;
+  }
+  return CHS_OK;
+}
 
 int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, const double* d_d,
                         const double* d_q) {
@@ -483,6 +593,8 @@ int chs_bind_projectors(chs_context* ctx, int n_proj, const double* d_beta, cons
   release(H.beta_f);
   release(H.sinv_m_f);
   release(H.bmat_f);
+  release(H.h3_cn.buf);
+  release(H.h3_nn.buf);
   if (n_proj == 0) return CHS_OK;
   std::vector<double> q(n_proj);
   CHS_HIP(ctx, hipMemcpyAsync(q.data(), d_q, n_proj * sizeof(double), hipMemcpyDeviceToHost,

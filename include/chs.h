@@ -208,6 +208,13 @@ int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
  * dims4: {n_grid, n_pw, sticks, active x columns}. */
 int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,
                      double* bytes3, int32_t* dims4);
+/* Mean duration (HIP events, context stream) of one Chebyshev step's two
+ * n_pw-long projector products on the bound projectors with synthetic n_b-band
+ * operands: ms2[0] for G = beta^H O, ms2[1] for O + beta C, in the form the
+ * filter uses at `precision` (complex64: the FP32 matrix-core form, or the
+ * fp16x3 form when CHS_GEMM_FP16X3=1). */
+int chs_profile_projector_products(chs_context* ctx, int precision, int n_b, int reps,
+                                   double* ms2);
 /* Mean milliseconds (HIP events on the context stream) of `reps` complex
  * GEMMs C = op(A) op(B) with op(A) m x k, op(B) k x n, in CHS_C128 or
  * CHS_C64, on synthetic operands in context workspaces. */

@@ -112,3 +112,28 @@ def test_filter_projectors_without_s(ctx):
     assert rc == 0
     got = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi), 9, b4).sync_to_host()
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-10
+
+
+def test_filter_fp16x3_projectors_vs_oracle(monkeypatch):
+    """The opt-in fp16x3 projector products (CHS_GEMM_FP16X3=1: split fp16
+    operands on the matrix cores, 22-bit) keep the complex64 filter within the
+    FP32 tolerance of the FP64 oracle, and agree with the FP32 form."""
+    m = model()
+    psi = m.initial_psi(20)
+    oup, _ = m.lanczos_bound(10, 4)
+    b4 = O.spectral_bounds(None, oup)
+    rc, ref, _ = m.chebyshev_filter(psi, 8, b4)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("CHS_GEMM_FP16X3", flag)
+        c = chs.Context(0)  # the switch is read when a context is created
+        H = bind(c, m)  # keeps the bound device arrays alive
+        x = chs.Gpu.from_host(c, psi.astype(np.complex64))
+        got = chs.chebyshev_filter(c, x, 8, b4)
+        out[flag] = got.sync_to_host().astype(np.complex128)
+        c.synchronize()
+        del got, x, H
+        c.close()
+    scale = np.abs(ref).max()
+    assert np.abs(out["1"] - ref).max() / scale < 2e-5
+    assert np.abs(out["1"] - out["0"]).max() / scale < 2e-5
