@@ -13,16 +13,13 @@
 // A is the projector block of the bound Hamiltonian (fixed per binding): it
 // is split once into six fp16 planes [m][k] (ar, ai, ar + ai, each hi and lo)
 // with one exponent. B (the filter iterate or the coefficient block) is split
-// on the fly in the load stage with one exponent per column, so a column far
-// smaller than the others keeps its own 22 bits.
+// per call into the same planes with one exponent per column, so a column
+// far smaller than the others keeps its own 22 bits.
 //
 // Used, when CHS_GEMM_FP16X3=1, by the complex64 Chebyshev filter for its two
 // n_pw-long projector products (G = beta^H O and O + beta C,
-// chebyshev.rs:416-500 as restated in coefficient space). It is opt-in:
-// with two resident wavefronts per SIMD and 576 matrix-core cycles per K
-// stage this kernel cannot hide the operand latency (the matrix cores are
-// busy ~8% of its run) and is slower than the FP32 form (3.5 vs 2.3 ms for
-// 864 x 512 x 80,725); chs_cgemm keeps the plain FP32 matrix-core form.
+// chebyshev.rs:416-500 as restated in coefficient space); chs_cgemm keeps
+// the plain FP32 matrix-core form.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -33,8 +30,6 @@
 namespace chs {
 
 constexpr int kHBM = 64, kHBN = 64, kHBK = 32;
-constexpr int kHP = 40;  // LDS row pitch in halfs (80 B: the 16-B operand reads of 8 lanes
-                         // land on disjoint bank quads)
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -144,129 +139,6 @@ __device__ __forceinline__ bool h3_block(int mt, int nt, int kz, int nfast, int*
   return true;
 }
 
-// 64 x 64 output tile per workgroup of 256 threads (four wavefronts as 2 x 2,
-// each 2 x 2 MFMA tiles of 16 x 16), K stages of 32, register prefetch of the
-// next stage. Operand layout of v_mfma_f32_16x16x32_f16: lane l supplies
-// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16], j < 8; the result
-// holds rows 4 (l / 16) + v, column l % 16.
-__global__ void __launch_bounds__(256) k_gemm_h3(int m, int n, int k, c32 alpha,
-                                                 const _Float16* __restrict__ Ah, int lda_h,
-                                                 size_t plane, int ea, const c32* __restrict__ B,
-                                                 int ldb, const int* __restrict__ eb, c32 beta,
-                                                 c32* __restrict__ C, int ldc, int klen, int ksplit,
-                                                 int nfast, c32* __restrict__ part) {
-  int bm, bn, bz;
-  if (!h3_block((m + kHBM - 1) / kHBM, (n + kHBN - 1) / kHBN, ksplit, nfast, &bm, &bn, &bz))
-    return;
-  __shared__ __attribute__((aligned(16))) _Float16 As[6][kHBM][kHP];
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[6][kHBN][kHP];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int wm0 = (w >> 1) * 32, wn0 = (w & 1) * 32;
-  const int li = lane & 15, lk = lane >> 4;
-  const int m0 = bm * kHBM, n0 = bn * kHBN;
-  const int kb = bz * klen, ke = min(k, kb + klen);
-  // loader coordinates: A row / B column lr, k octet lc
-  const int lr = threadIdx.x >> 2, lc = (threadIdx.x & 3) * 8;
-  const int bc = n0 + lr;
-  const float sb = bc < n ? ldexpf(1.f, eb[bc]) : 0.f;
-  const c32* bcol = B + (size_t)(bc < n ? bc : 0) * ldb;
-  f4 p1[2][2], p2[2][2], p3[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) p1[i][j] = p2[i][j] = p3[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  uint4 ra[6];
-  c32 rb[8];
-  auto load = [&](int k0) {
-    const _Float16* arow = Ah + (size_t)(m0 + lr) * lda_h + k0 + lc;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) ra[q] = *reinterpret_cast<const uint4*>(arow + (size_t)q * plane);
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int kk = k0 + lc + j;
-      rb[j] = (bc < n && kk < ke) ? bcol[kk] : mk(0.f, 0.f);
-    }
-  };
-  if (kb < ke) load(kb);
-  for (int k0 = kb; k0 < ke; k0 += kHBK) {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) *reinterpret_cast<uint4*>(&As[q][lr][lc]) = ra[q];
-    {
-      h8 v[6];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const float re = rb[j].x * sb, im = rb[j].y * sb;
-        _Float16 h, l;
-        h3_split(re, h, l);
-        v[0][j] = h;
-        v[1][j] = l;
-        h3_split(im, h, l);
-        v[2][j] = h;
-        v[3][j] = l;
-        h3_split(re + im, h, l);
-        v[4][j] = h;
-        v[5][j] = l;
-      }
-#pragma unroll
-      for (int q = 0; q < 6; ++q) *reinterpret_cast<h8*>(&Bs[q][lr][lc]) = v[q];
-    }
-    __syncthreads();
-    if (k0 + kHBK < ke) load(k0 + kHBK);  // next stage in flight during the MFMAs
-    // one real product class: acc += hi*hi + hi*lo + lo*hi over planes (q, q + 1)
-    auto product = [&](f4(&acc)[2][2], int q) {
-      h8 ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ah[i] = *reinterpret_cast<const h8*>(&As[q][wm0 + 16 * i + li][8 * lk]);
-        al[i] = *reinterpret_cast<const h8*>(&As[q + 1][wm0 + 16 * i + li][8 * lk]);
-        bh[i] = *reinterpret_cast<const h8*>(&Bs[q][wn0 + 16 * i + li][8 * lk]);
-        bl[i] = *reinterpret_cast<const h8*>(&Bs[q + 1][wn0 + 16 * i + li][8 * lk]);
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[i][j], 0, 0, 0);
-    };
-    product(p1, 0);  // ar br
-    product(p2, 2);  // ai bi
-    product(p3, 4);  // (ar + ai)(br + bi)
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = n0 + wn0 + 16 * j + li;
-      if (c >= n) continue;
-      const float unscale = ldexpf(1.f, -(ea + eb[c]));
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = m0 + wm0 + 16 * i + 4 * lk + v;
-        if (r >= m) continue;
-        const float r1 = p1[i][j][v], r2 = p2[i][j][v];
-        const c32 acc = mk((r1 - r2) * unscale, (p3[i][j][v] - r1 - r2) * unscale);
-        if (part) {
-          part[(size_t)bz * m * n + (size_t)r + (size_t)c * m] = acc;
-        } else {
-          c32 val = alpha * acc;
-          if (beta.x != 0.f || beta.y != 0.f) val = val + beta * C[(size_t)r + (size_t)c * ldc];
-          C[(size_t)r + (size_t)c * ldc] = val;
-        }
-      }
-    }
-}
-
 __global__ void k_h3_reduce(int m, int n, int nz, const c32* __restrict__ part, c32 alpha,
                             c32 beta, c32* __restrict__ C, int ldc) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -277,6 +149,108 @@ __global__ void k_h3_reduce(int m, int n, int nz, const c32* __restrict__ part, 
   c32 v = alpha * s;
   if (beta.x != 0.f || beta.y != 0.f) v = v + beta * C[(size_t)r + (size_t)c * ldc];
   C[(size_t)r + (size_t)c * ldc] = v;
+}
+
+// Six planes [n_pad][ld] of B (c64, k x n, column stride ldb) scaled by
+// 2^eb[column]; padding written as zeros.
+__global__ void k_h3_split_b(const c32* __restrict__ B, int k, int n, int ldb,
+                             const int* __restrict__ eb, int ld, _Float16* __restrict__ planes,
+                             size_t plane) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= plane) return;
+  const int c = (int)(i / ld), kk = (int)(i % ld);
+  float re = 0.f, im = 0.f;
+  if (c < n && kk < k) {
+    const c32 v = B[(size_t)kk + (size_t)c * ldb];
+    const float sc = ldexpf(1.f, eb[c]);
+    re = v.x * sc;
+    im = v.y * sc;
+  }
+  const float x[3] = {re, im, re + im};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    _Float16 hi, lo;
+    h3_split(x[q], hi, lo);
+    planes[(2 * q) * plane + i] = hi;
+    planes[(2 * q + 1) * plane + i] = lo;
+  }
+}
+
+// Register-direct form: both operands as split planes ([rows][k], row
+// pitches multiples of 32 halfs, rows padded to 64, zero padding), so every
+// wavefront loads its MFMA fragments straight from memory (16 B per lane per
+// plane and tile) and no LDS or barrier is needed; the four wavefronts of a
+// workgroup each own a 32 x 32 quarter of the 64 x 64 tile. Occupancy (three
+// wavefronts per SIMD) hides the operand latency.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_gemm_h3r(int m, int n, int k, c32 alpha, const _Float16* __restrict__ Ah, int lda_h,
+           size_t aplane, int ea, const _Float16* __restrict__ Bh, int ldb_h, size_t bplane,
+           const int* __restrict__ eb, c32 beta, c32* __restrict__ C, int ldc, int klen,
+           int ksplit, int nfast, c32* __restrict__ part) {
+  int bm, bn, bz;
+  if (!h3_block((m + kHBM - 1) / kHBM, (n + kHBN - 1) / kHBN, ksplit, nfast, &bm, &bn, &bz))
+    return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm0 = bm * kHBM + (w >> 1) * 32, wn0 = bn * kHBN + (w & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  const int kb = bz * klen, ke = min(k, kb + klen);
+  const _Float16* arow[2];
+  const _Float16* brow[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    arow[i] = Ah + (size_t)(wm0 + 16 * i + li) * lda_h + 8 * lk;
+    brow[i] = Bh + (size_t)(wn0 + 16 * i + li) * ldb_h + 8 * lk;
+  }
+  f4 acc[3][2][2];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[q][i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kb; k0 < ke; k0 += kHBK) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {  // P1 (planes 0/1), P2 (2/3), P3 (4/5)
+      h8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ah[i] = *reinterpret_cast<const h8*>(arow[i] + (size_t)(2 * q) * aplane + k0);
+        al[i] = *reinterpret_cast<const h8*>(arow[i] + (size_t)(2 * q + 1) * aplane + k0);
+        bh[i] = *reinterpret_cast<const h8*>(brow[i] + (size_t)(2 * q) * bplane + k0);
+        bl[i] = *reinterpret_cast<const h8*>(brow[i] + (size_t)(2 * q + 1) * bplane + k0);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bh[j], acc[q][i][j], 0, 0, 0);
+          acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[i], bl[j], acc[q][i][j], 0, 0, 0);
+          acc[q][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[i], bh[j], acc[q][i][j], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = wn0 + 16 * j + li;
+      if (c >= n) continue;
+      const float unscale = ldexpf(1.f, -(ea + eb[c]));
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm0 + 16 * i + 4 * lk + v;
+        if (r >= m) continue;
+        const float r1 = acc[0][i][j][v], r2 = acc[1][i][j][v];
+        const c32 a = mk((r1 - r2) * unscale, (acc[2][i][j][v] - r1 - r2) * unscale);
+        if (part) {
+          part[(size_t)bz * m * n + (size_t)r + (size_t)c * m] = a;
+        } else {
+          c32 val = alpha * a;
+          if (beta.x != 0.f || beta.y != 0.f) val = val + beta * C[(size_t)r + (size_t)c * ldc];
+          C[(size_t)r + (size_t)c * ldc] = val;
+        }
+      }
+    }
 }
 
 // Split planes of op(A) for a c128 projector block (P x n_pw, lda), cn as in
@@ -323,14 +297,22 @@ int h3_column_exponents(chs_context* ctx, const c32* B, int k, int n, int ldb, i
 }
 
 // C = alpha op(A) B + beta C with op(A) from its split planes (m x k), B
-// c64 k x n (column stride ldb) with per-column exponents d_eb (device).
+// c64 k x n (column stride ldb) with per-column exponents d_eb (device): B
+// is split into planes in a context workspace, then the register-direct
+// kernel runs (split K over up to 2048 workgroups when the tiles alone are
+// fewer).
 int h3_gemm(chs_context* ctx, const H3Planes& A, int n, c32 alpha, const c32* B, int ldb,
             const int* d_eb, c32 beta, c32* C, int ldc) {
   const int m = A.m, k = A.k;
   if (m <= 0 || n <= 0) return CHS_OK;
-  const int tiles = ((m + kHBM - 1) / kHBM) * ((n + kHBN - 1) / kHBN);
-  // split K when the tiles alone cannot fill the device: up to 2048
-  // workgroups, slices of whole K stages
+  auto* im = ctx->impl;
+  const int n_pad = (n + kHBN - 1) / kHBN * kHBN;
+  const size_t bplane = (size_t)n_pad * A.ld;
+  CHS_TRY(ensure(ctx, im->h3_bws, 6 * bplane * sizeof(_Float16)));
+  hipLaunchKernelGGL(k_h3_split_b, dim3(ceil_div(bplane, 256)), dim3(256), 0, ctx->stream, B, k, n,
+                     ldb, d_eb, A.ld, (_Float16*)im->h3_bws.p, bplane);
+  CHS_LAUNCH_CHECK(ctx);
+  const int tiles = ((m + kHBM - 1) / kHBM) * (n_pad / kHBN);
   int ks = 1;
   constexpr int kTarget = 2048;
   if (tiles < kTarget && k > 4 * kHBK) ks = std::max(1, std::min(kTarget / tiles, k / 1024));
@@ -339,13 +321,14 @@ int h3_gemm(chs_context* ctx, const H3Planes& A, int n, c32 alpha, const c32* B,
   ks = (k + klen - 1) / klen;
   c32* part = nullptr;
   if (ks > 1) {
-    CHS_TRY(ensure(ctx, ctx->impl->gemm_ws, (size_t)ks * m * n * sizeof(c32)));
-    part = (c32*)ctx->impl->gemm_ws.p;
+    CHS_TRY(ensure(ctx, im->gemm_ws, (size_t)ks * m * n * sizeof(c32)));
+    part = (c32*)im->gemm_ws.p;
   }
   const unsigned nblk = (unsigned)tiles * ks;
-  hipLaunchKernelGGL(k_gemm_h3, dim3((nblk + 7) / 8 * 8), dim3(256), 0, ctx->stream, m, n, k, alpha,
-                     (const _Float16*)A.buf.p, A.ld, A.plane, A.ea, B, ldb, d_eb, beta, C, ldc,
-                     klen, ks, m > n ? 1 : 0, part);
+  hipLaunchKernelGGL(k_gemm_h3r, dim3((nblk + 7) / 8 * 8), dim3(256), 0, ctx->stream, m, n, k,
+                     alpha, (const _Float16*)A.buf.p, A.ld, A.plane, A.ea,
+                     (const _Float16*)im->h3_bws.p, A.ld, bplane, d_eb, beta, C, ldc, klen, ks,
+                     m > n ? 1 : 0, part);
   CHS_LAUNCH_CHECK(ctx);
   if (ks > 1) {
     hipLaunchKernelGGL(k_h3_reduce, dim3(ceil_div((size_t)m * n, 256)), dim3(256), 0, ctx->stream,

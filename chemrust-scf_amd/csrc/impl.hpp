@@ -127,7 +127,7 @@ struct chs_context::Impl {
   chs::Hamiltonian ham;
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
-  chs::DevBuf gemm_ws, gemv_ws, lin_ws, scf_ws, proj_ws, chol_ws, eig_ws, mlp_ws;
+  chs::DevBuf gemm_ws, gemv_ws, h3_bws, lin_ws, scf_ws, proj_ws, chol_ws, eig_ws, mlp_ws;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Replay launch-bound loops (Jacobi sweeps) as captured HIP graphs. Off
   // when CHS_NO_GRAPHS is set to a non-zero value: some kernel tracers
@@ -137,7 +137,8 @@ struct chs_context::Impl {
   // vector-FMA kernel of the same tiling instead.
   bool use_mfma = true;
   // complex64 filter projector products on the fp16 matrix cores (fp16x3);
-  // opt-in: the present kernel is latency-bound and slower than the FP32 form
+  // opt-in: its split operands carry 1.5x the bytes of FP32 and the kernel is
+  // slower than the FP32 form
   bool use_h3 = false;
   // per-shape GEMM kernel variant, timed on first use (CHS_GEMM_TUNE=0: off)
   bool gemm_autotune = true;
