@@ -14,8 +14,9 @@
 //   p = tau A22 v, s = v^H A22 v, w = p - (|tau|^2 s / 2) v,
 //   A22 <- H^H A22 H = A22 - v w^H - w v^H.
 // Then A = Q T Q^H with Q = H_0 H_1 ... H_{n-2}, and A's eigenvectors are
-// Q Z for T's eigenvectors Z. Each step is two kernels (partial products of
-// p and s; rank-2 update), replayed from a captured graph.
+// Q Z for T's eigenvectors Z. Step k is one kernel (k_hh_step: the rank-2
+// update of step k - 1 fused with the partial products of step k), the n - 1
+// launches replayed from a captured graph.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -26,8 +27,7 @@
 
 namespace chs {
 
-constexpr int kHC = 64;   // row/column chunk of the symmetric product
-constexpr int kHT = 32;   // tile of the rank-2 update
+constexpr int kHC = 64;   // row/column chunk of the symmetric product and update tiles
 constexpr int kWY = 32;   // reflectors per compact-WY block
 
 // Block sum visible to every thread (sh: at least 9 doubles).
@@ -127,60 +127,133 @@ __global__ void __launch_bounds__(256) k_hh_symv(const z64* __restrict__ A, int 
   }
 }
 
-// Rank-2 update of step k on 32 x 32 tiles of A22.
-__global__ void __launch_bounds__(256) k_hh_update(z64* __restrict__ A, int n, int k,
-                                                   const z64* __restrict__ Vst,
-                                                   const z64* __restrict__ ppart,
-                                                   const z64* __restrict__ spart,
-                                                   const z64* __restrict__ tau) {
-  __shared__ double sh[9];
-  __shared__ z64 vr[kHT], wr[kHT], vc[kHT], wc[kHT];
-  const int m = n - k - 1;
-  const int nch = (m + kHC - 1) / kHC;
-  const z64 t_k = tau[k];
-  if (t_k.x == 0.0 && t_k.y == 0.0) return;  // H = I
+// w of step k - 1 for every index of its trailing matrix (t in [0, mp),
+// global index k + t) into LDS, with v of that step: w = tau p - c v,
+// p = sum of the step's partial products, c = |tau|^2 (v^H A22 v) / 2.
+__device__ void hh_prev_vw(const z64* __restrict__ Vst, const z64* __restrict__ ppart,
+                           const z64* __restrict__ spart, const z64* __restrict__ tau, int n,
+                           int k, z64* __restrict__ vp, z64* __restrict__ wp, double* sh) {
+  const int mp = n - k;  // trailing size of step k - 1
+  const int nchp = (mp + kHC - 1) / kHC;
+  const z64 t_p = tau[k - 1];
   double s = 0.0;
-  for (int t = threadIdx.x; t < nch * nch; t += blockDim.x) s += spart[t].x;
-  const double sv = block_sum_all(s, sh);  // v^H A22 v (real)
-  const double c = 0.5 * norm2(t_k) * sv;
-  const int r0 = blockIdx.x * kHT, c0 = blockIdx.y * kHT;
-  const z64* v = Vst + (size_t)k * n + (k + 1);
-  if (threadIdx.x < 2 * kHT) {
-    const int side = threadIdx.x / kHT, l = threadIdx.x % kHT;
-    const int i = (side ? c0 : r0) + l;
-    z64 vi = mk(0.0, 0.0), wi = mk(0.0, 0.0);
-    if (i < m) {
-      z64 p = mk(0.0, 0.0);
-      for (int b = 0; b < nch; ++b) p = p + ppart[(size_t)b * n + i];
-      p = t_k * p;
-      vi = v[i];
-      wi = mk(p.x - c * vi.x, p.y - c * vi.y);
-    }
-    if (side) {
-      vc[l] = vi;
-      wc[l] = wi;
-    } else {
-      vr[l] = vi;
-      wr[l] = wi;
-    }
+  for (int t = threadIdx.x; t < nchp * nchp; t += blockDim.x) s += spart[t].x;
+  const double c = 0.5 * norm2(t_p) * block_sum_all(s, sh);
+  const z64* v = Vst + (size_t)(k - 1) * n + k;
+  for (int t = threadIdx.x; t < mp; t += blockDim.x) {
+    z64 p = mk(0.0, 0.0);
+    for (int b = 0; b < nchp; ++b) p = p + ppart[(size_t)b * n + t];
+    p = t_p * p;
+    const z64 vt = v[t];
+    vp[t] = vt;
+    wp[t] = mk(p.x - c * vt.x, p.y - c * vt.y);
   }
   __syncthreads();
-  const int r = threadIdx.x % kHT, cq = threadIdx.x / kHT;
-  const int i = r0 + r;
-  if (i >= m) return;
-#pragma unroll
-  for (int q = 0; q < kHT / 8; ++q) {
-    const int cl = cq * (kHT / 8) + q, j = c0 + cl;
-    if (j < m) {
-      z64* a = A + (size_t)(k + 1 + i) + (size_t)(k + 1 + j) * n;
-      const z64 u = vr[r] * conj(wc[cl]) + wr[r] * conj(vc[cl]);
-      *a = *a - u;
+}
+
+// Step k >= 1 of the reduction fused with the rank-2 update of step k - 1.
+// The update A22 -= v w^H + w v^H of step k - 1 is applied in memory to
+// [k+1, n)^2 only; column k and A[k][k] of that trailing matrix are formed
+// on the fly by every block (read, never written: nothing reads them after
+// this step), giving x = A[k+1:, k], the reflector of step k and d_k. Each
+// block (bi, bj) then multiplies its freshly updated 64 x 64 tile with v_k:
+// ppart[bj][i] = sum_j A22[i][j] v_j, spart[bi][bj] = sum_i conj(v_i) p_i.
+// Block (0, 0) stores v_k (column k of Vst), tau_k, e_k = beta, d_k.
+__global__ void __launch_bounds__(256) k_hh_step(z64* __restrict__ A, int n, int k,
+                                                 z64* __restrict__ Vst,
+                                                 const z64* __restrict__ ppart_prev,
+                                                 const z64* __restrict__ spart_prev,
+                                                 z64* __restrict__ ppart, z64* __restrict__ spart,
+                                                 z64* __restrict__ tau, double* __restrict__ dd,
+                                                 double* __restrict__ ee) {
+  __shared__ double sh[9];
+  __shared__ z64 vp[1024], wp[1024], xk[1024];
+  __shared__ z64 vj[kHC];
+  __shared__ z64 acc_sh[4][kHC];
+  const int m = n - k - 1;
+  const int nch = (m + kHC - 1) / kHC;
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  hh_prev_vw(Vst, ppart_prev, spart_prev, tau, n, k, vp, wp, sh);
+  // column k of the step-(k-1) trailing matrix below the diagonal (rows k+1..)
+  const z64* colk = A + (size_t)(k + 1) + (size_t)k * n;
+  const z64 cw0 = conj(wp[0]), cv0 = conj(vp[0]);
+  for (int r = threadIdx.x; r < m; r += blockDim.x)
+    xk[r] = colk[r] - vp[r + 1] * cw0 - wp[r + 1] * cv0;
+  __syncthreads();
+  z64 t_k, scale;
+  double beta;
+  hh_reflector(xk, m, sh, &t_k, &beta, &scale);
+  if (bi == 0 && bj == 0) {
+    z64* vcol = Vst + (size_t)k * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const int j = r - k - 1;
+      vcol[r] = j < 0 ? mk(0.0, 0.0) : (j == 0 ? mk(1.0, 0.0) : xk[j] * scale);
     }
+    if (threadIdx.x == 0) {
+      tau[k] = t_k;
+      ee[k] = beta;
+      const z64 vw = vp[0] * cw0;
+      dd[k] = A[(size_t)k + (size_t)k * n].x - 2.0 * vw.x;
+    }
+  }
+  for (int t = threadIdx.x; t < kHC; t += blockDim.x) {
+    const int j = bj * kHC + t;
+    vj[t] = j < m ? (j == 0 ? mk(1.0, 0.0) : xk[j] * scale) : mk(0.0, 0.0);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = bi * kHC + lane;
+  z64 acc = mk(0.0, 0.0);
+  if (i < m) {
+    z64* arow = A + (size_t)(k + 1 + i);
+    const z64 vr = vp[i + 1], wr = wp[i + 1];
+#pragma unroll 4
+    for (int jj = 0; jj < kHC / 4; ++jj) {
+      const int jl = w * (kHC / 4) + jj, j = bj * kHC + jl;
+      if (j < m) {
+        z64* a = arow + (size_t)(k + 1 + j) * n;
+        const z64 u = vr * conj(wp[j + 1]) + wr * conj(vp[j + 1]);
+        const z64 upd = *a - u;
+        *a = upd;
+        acc = acc + upd * vj[jl];
+      }
+    }
+  }
+  acc_sh[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) {
+    z64 p = acc_sh[0][lane] + acc_sh[1][lane] + acc_sh[2][lane] + acc_sh[3][lane];
+    z64 sv = mk(0.0, 0.0);
+    if (i < m) {
+      ppart[(size_t)bj * n + i] = p;
+      sv = cmulc(i == 0 ? mk(1.0, 0.0) : xk[i] * scale, p);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      sv.x += __shfl_down(sv.x, off);
+      sv.y += __shfl_down(sv.y, off);
+    }
+    if (lane == 0) spart[(size_t)bi * nch + bj] = sv;
   }
 }
 
-__global__ void k_hh_last(const z64* __restrict__ A, int n, double* __restrict__ dd) {
-  if (threadIdx.x == 0) dd[n - 1] = A[(size_t)(n - 1) + (size_t)(n - 1) * n].x;
+// d_{n-1}: A[n-1][n-1] after the update of step n - 2 (its last element).
+__global__ void __launch_bounds__(256) k_hh_last(const z64* __restrict__ A, int n,
+                                                 const z64* __restrict__ Vst,
+                                                 const z64* __restrict__ ppart,
+                                                 const z64* __restrict__ spart,
+                                                 const z64* __restrict__ tau,
+                                                 double* __restrict__ dd) {
+  __shared__ double sh[9];
+  __shared__ z64 vp[2], wp[2];
+  if (n < 2) {
+    if (threadIdx.x == 0) dd[0] = A[0].x;
+    return;
+  }
+  hh_prev_vw(Vst, ppart, spart, tau, n, n - 1, vp, wp, sh);
+  if (threadIdx.x == 0) {
+    const z64 vw = vp[0] * conj(wp[0]);
+    dd[n - 1] = A[(size_t)(n - 1) + (size_t)(n - 1) * n].x - 2.0 * vw.x;
+  }
 }
 
 // Number of eigenvalues of the tridiagonal (d, e) below x (Sturm count).
@@ -392,8 +465,8 @@ struct EighWs {
 static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
   const size_t nn = (size_t)n * n;
   const int nch = (n + kHC - 1) / kHC;
-  const size_t zb = (nn + (size_t)nch * n + (size_t)nch * nch + n + kWY * kWY + 2 * (size_t)kWY * n) *
-                    sizeof(z64);
+  const size_t zb = (nn + 2 * (size_t)nch * n + 2 * (size_t)nch * nch + n + kWY * kWY +
+                     2 * (size_t)kWY * n) * sizeof(z64);
   const size_t db = (3 * (size_t)n + 6 * nn) * sizeof(double);
   CHS_TRY(ensure(ctx, ctx->impl->eig_ws, zb + db + 1024));
   char* p = (char*)ctx->impl->eig_ws.p;
@@ -408,8 +481,8 @@ static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
     return r;
   };
   ws->vst = take_z(nn);
-  ws->ppart = take_z((size_t)nch * n);
-  ws->spart = take_z((size_t)nch * nch);
+  ws->ppart = take_z(2 * (size_t)nch * n);   // two halves: steps alternate
+  ws->spart = take_z(2 * (size_t)nch * nch);
   ws->tau = take_z(n);
   ws->tmat = take_z(kWY * kWY);
   ws->w1 = take_z((size_t)kWY * n);
@@ -423,15 +496,23 @@ static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
 }
 
 static void tridiagonalise_launches(chs_context* ctx, z64* A, int n, const EighWs& ws) {
+  // partial products alternate between two halves of ppart / spart
+  const int nch0 = (n + kHC - 1) / kHC;
+  z64* pp[2] = {ws.ppart, ws.ppart + (size_t)nch0 * n};
+  z64* sp[2] = {ws.spart, ws.spart + (size_t)nch0 * nch0};
   for (int k = 0; k + 1 < n; ++k) {
     const int m = n - k - 1;
-    const unsigned nc = ceil_div(m, kHC), nt = ceil_div(m, kHT);
-    hipLaunchKernelGGL(k_hh_symv, dim3(nc, nc), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
-                       ws.ppart, ws.spart, ws.tau, ws.dd, ws.ee);
-    hipLaunchKernelGGL(k_hh_update, dim3(nt, nt), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
-                       ws.ppart, ws.spart, ws.tau);
+    const unsigned nc = ceil_div(m, kHC);
+    if (k == 0)
+      hipLaunchKernelGGL(k_hh_symv, dim3(nc, nc), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
+                         pp[0], sp[0], ws.tau, ws.dd, ws.ee);
+    else
+      hipLaunchKernelGGL(k_hh_step, dim3(nc, nc), dim3(256), 0, ctx->stream, A, n, k, ws.vst,
+                         pp[(k - 1) & 1], sp[(k - 1) & 1], pp[k & 1], sp[k & 1], ws.tau, ws.dd,
+                         ws.ee);
   }
-  hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(64), 0, ctx->stream, A, n, ws.dd);
+  hipLaunchKernelGGL(k_hh_last, dim3(1), dim3(256), 0, ctx->stream, A, n, ws.vst,
+                     pp[n >= 2 ? (n - 2) & 1 : 0], sp[n >= 2 ? (n - 2) & 1 : 0], ws.tau, ws.dd);
 }
 
 // Hermitian eigensolve: A (n x n, destroyed) -> eigenvalues ascending in

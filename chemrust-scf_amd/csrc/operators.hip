@@ -116,15 +116,17 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
   CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
                         d_coef, 0.0, 0.0, nullptr, nullptr));
   CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, d_coef));
-  CHS_TRY(ensure(ctx, im->part, sizeof(double) * kRedBlocks + 64));
+  // |Y_k|^2 over the whole block: enough workgroups to stream it at HBM rate
+  constexpr int kNormBlocks = 2048;
+  CHS_TRY(ensure(ctx, im->part, sizeof(double) * kNormBlocks + 64));
   if (precision == CHS_C128)
-    hipLaunchKernelGGL((k_sqnorm_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+    hipLaunchKernelGGL((k_sqnorm_part<double>), dim3(kNormBlocks), dim3(kRedThreads), 0,
                        ctx->stream, (const z64*)yout, ne, (double*)im->part.p);
   else
-    hipLaunchKernelGGL((k_sqnorm_part<float>), dim3(kRedBlocks), dim3(kRedThreads), 0,
+    hipLaunchKernelGGL((k_sqnorm_part<float>), dim3(kNormBlocks), dim3(kRedThreads), 0,
                        ctx->stream, (const c32*)yout, ne, (double*)im->part.p);
   hipLaunchKernelGGL(k_norm_guard_roll, dim3(1), dim3(256), 0, ctx->stream,
-                     (const double*)im->part.p, kRedBlocks, k, first ? 1 : 0, flag);
+                     (const double*)im->part.p, kNormBlocks, k, first ? 1 : 0, flag);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
 }

@@ -160,11 +160,11 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   int small = 0, it = 0, converged = 0;
   bool fp64_phase = !mixed;
   int fp64_iters = 0, fp32_iters = 0;
-  // complex64 filter until |dE|/atom < fp32_until (default 3 etol, but not
+  // complex64 filter until |dE|/atom < fp32_until (default 2 etol, but not
   // below 3e-7 Ha, near the energy noise the complex64 iterates leave), or
   // until |dE| stalls (> 0.9 of the previous one) within 10 fp32_until;
   // convergence is then decided on FP64 energies only
-  const double fp32_until = p->fp32_until > 0 ? p->fp32_until : std::max(3.0 * p->etol, 3e-7);
+  const double fp32_until = p->fp32_until > 0 ? p->fp32_until : std::max(2.0 * p->etol, 3e-7);
   double de_prev = INFINITY;
   const double t0 = now();
   double t_filter = 0, t_rr = 0, t_den = 0, t_veff = 0, t_lan = 0;

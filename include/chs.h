@@ -245,7 +245,7 @@ typedef struct chs_scf_params {
   int32_t precision;       /* CHS_C128, or CHS_C64 filter with FP64 RR */
   int32_t fp64_final;      /* with CHS_C64: at least this many FP64 SCF steps at the end */
   double fp32_until;       /* with CHS_C64: |dE|/atom (Ha) below which the filter goes back
-                              to FP64 (<= 0: max(3 etol, 3e-7)); it also goes back when |dE|
+                              to FP64 (<= 0: max(2 etol, 3e-7)); it also goes back when |dE|
                               stalls (> 0.9 x the previous) below 10 fp32_until. The
                               convergence test (|dE|/atom < etol twice in a row) then runs
                               on FP64 steps only */
