@@ -17,6 +17,12 @@ namespace chs {
 template <typename T> struct cplx {
   T x, y;
 };
+// complex64 on its natural 8-byte alignment, so that an element moves with
+// one 64-bit LDS access (ds_read_b64 / ds_write_b64) rather than a pair of
+// 32-bit ones
+template <> struct alignas(8) cplx<float> {
+  float x, y;
+};
 using z64 = cplx<double>;
 using c32 = cplx<float>;
 

@@ -546,8 +546,13 @@ int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U) {
   // 2. eigenvalues (multisection) and tridiagonal eigenvectors
   hipLaunchKernelGGL(k_tri_multisect, dim3(n), dim3(64), 0, ctx->stream, ws.dd, ws.ee, n, ws.e2, w);
   CHS_LAUNCH_CHECK(ctx);
+  // clusters: gaps below 1e-6 ||T||. A vector whose neighbours lie further
+  // away carries an angle error of about eps ||T|| / gap < 3e-10 without
+  // reorthogonalisation, while a looser cut lets ill-conditioned generalised
+  // problems (cond S ~ 1e5) form clusters of hundreds of eigenvalues, each
+  // of which one thread works through at O(n c^2)
   hipLaunchKernelGGL(k_tri_invit, dim3(ceil_div(n, 64)), dim3(64), 0, ctx->stream, ws.dd, ws.ee, w,
-                     n, 1e-5, ws.scr, ws.zt);
+                     n, 1e-6, ws.scr, ws.zt);
   hipLaunchKernelGGL(k_tri_to_complex, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0,
                      ctx->stream, ws.zt, n, U);
   CHS_LAUNCH_CHECK(ctx);

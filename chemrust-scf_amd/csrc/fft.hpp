@@ -135,6 +135,40 @@ CHS_SPLIT(128, 8, 16)
 CHS_SPLIT(144, 12, 12)
 #undef CHS_SPLIT
 
+// Zero window of a plane line. The wave-function sphere on a grid at least
+// twice its diameter leaves every line index in [LO, HI) = [Z0 N2, Z1 N2)
+// empty: the first-stage inputs n1 in [Z0, Z1) are zeros, and the outputs
+// there are not needed. Host code checks the stick table against the window
+// before it launches a kernel that relies on it. K2A..K2B: the second-stage
+// output blocks k2 whose indices k1 + N1 k2 (any k1 < N1) all fall inside.
+// RS: compact-plane row stride for such a table (every column outside the
+// window plus one spare, = 1 mod 8 to spread column accesses over the banks).
+template <int N> struct Window {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static constexpr int Z0 = (N1 + 3) / 4, Z1 = N1 - Z0;
+  static constexpr int LO = Z0 * N2, HI = Z1 * N2;
+  static constexpr int K2A = (LO + N1 - 1) / N1, K2B = HI >= N1 ? (HI - N1) / N1 + 1 : 0;
+  static constexpr int RS0 = N - (HI > LO ? HI - LO : 0) + 1;
+  static constexpr int RS = RS0 + ((9 - RS0 % 8) % 8);
+  static constexpr bool zero_in(int n1) { return Z0 <= n1 && n1 < Z1; }
+  static constexpr bool skip_out(int k2) { return K2A <= k2 && k2 < K2B; }
+  // Row order of the plane pass: slot t takes row 32 (t / 32) + (U t mod 32)
+  // (t below the last whole block of 32; the rest in order), U = N2 / RS
+  // mod 32, so the groups of a wavefront (consecutive t) start their compact
+  // rows N2 words apart modulo the 32 eight-byte bank pairs and a row load
+  // or store by a wavefront touches every bank pair once.
+  static constexpr int inv32(int a) {
+    int x = 1;
+    for (int i = 0; i < 5; ++i) x = (x * (2 - a * x)) & 31;
+    return x;
+  }
+  static constexpr int U = (inv32(RS) * N2) & 31;
+  static constexpr int N32 = N & ~31;
+  static __device__ __forceinline__ int row(int t) {
+    return (U & 1) && t < N32 ? (t & ~31) | ((U * t) & 31) : t;
+  }
+};
+
 // Twiddle w_N^t = e^{S 2 pi i t / N}; tw holds the S = +1 table (t < N).
 template <int S, typename T>
 __device__ __forceinline__ cplx<T> twiddle(const cplx<T>* tw, int t) {
@@ -321,17 +355,64 @@ template <int N, typename T> struct RowScratch {
   static constexpr int STRIDE = SPAN + (sizeof(T) == 4 ? 17 : 20);
 };
 
-template <int N, typename T, typename Plan>
+// One line of a plane (element j at line[j ES], ES compile-time) by the N2
+// lanes of a group, exchanging through the group's scratch tile sc (the row
+// scratch of row_apply_v, [k1][n2] with pitch RowScratch::PA) instead of
+// through the line itself: read along the line, the second pass would hit an
+// LDS bank pair up to four times per 32 lanes. ZIN: the zero window's inputs
+// are not read; ZOUT: its outputs are not stored (Window<N>).
+template <int N, int S, int ES, bool ZIN, bool ZOUT, typename T>
+__device__ __forceinline__ void group_fft_line_x(cplx<T>* line, cplx<T>* sc, int lane,
+                                                 const cplx<T>* __restrict__ tw) {
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int R = (N1 + N2 - 1) / N2, PA = RowScratch<N, T>::PA;
+  using W = Window<N>;
+  {
+    cplx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1)
+      a[n1] = ZIN && W::zero_in(n1) ? mk((T)0, (T)0) : line[(n1 * N2 + lane) * ES];
+    Dft<N1, S, T>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * twiddle<S>(tw, lane * k1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) sc[k1 * PA + lane] = a[k1];
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+      cplx<T> b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[n2] = sc[k1 * PA + n2];
+      Dft<N2, S, T>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2)
+        if (!(ZOUT && W::skip_out(k2))) line[(k1 + N1 * k2) * ES] = b[k2];
+    }
+  }
+  wave_sync();  // the tile is reused by the group's next line
+}
+
+// PR: the stick table fits the zero window (Window<N>): the window's inputs
+// are not read and its outputs not stored.
+template <int N, typename T, bool PR, typename Plan>
 __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Plan& rp,
                                             const T* __restrict__ vrow, int lane) {
   constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
   constexpr int R = (N1 + N2 - 1) / N2;
+  using W = Window<N>;
   {
     cplx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) {
-      const int ai = rp.in(n1);
-      a[n1] = ai >= 0 ? crow[ai] : mk((T)0, (T)0);
+      if (PR && W::zero_in(n1)) {
+        a[n1] = mk((T)0, (T)0);
+      } else {
+        const int ai = rp.in(n1);
+        a[n1] = ai >= 0 ? crow[ai] : mk((T)0, (T)0);
+      }
     }
     Dft<N1, 1, T>::run(a);
 #pragma unroll
@@ -375,6 +456,7 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     Dft<N1, -1, T>::run(c);
 #pragma unroll
     for (int m1 = 0; m1 < N1; ++m1) {
+      if (PR && W::zero_in(m1)) continue;
       const int ai = rp.out(m1);
       if (ai >= 0) crow[ai] = c[m1];
     }

@@ -37,6 +37,7 @@ namespace chs {
 struct StickArgs {
   int n_st, n_act, n_pw, rs;  // rs: row stride of the compact plane (= 1 mod 8)
   int ld;                     // stick-buffer row stride: n_st rounded up to 32 (256-B rows)
+  int win;                    // 1: no stick or active column in the zero window (rs = Window RS)
   const int32_t* iy;
   const int32_t* act;
   const int32_t* pw0;
@@ -108,16 +109,24 @@ __device__ __forceinline__ void expand_row(const cplx<T>* cmp_row, cplx<T>* rb, 
 // L = (p % 8) (P / 8) + p / 8 gives every XCD a contiguous range of
 // (z-plane, band) pairs, band fastest, so each XCD's L2 holds only its
 // N / 8 planes of V_eff while all bands stream through them.
-template <int N, typename T, int TPB>
+// PR: the stick table leaves the zero window of every line empty (st.win):
+// the compact plane has the compile-time row stride Window<N>::RS, the
+// y-lines exchange through the groups' row scratch (group_fft_line_x, by the
+// first nrb groups) without the window's inputs (inverse) or outputs
+// (forward), and the rows skip them too and run in the bank-spreading order
+// Window<N>::row.
+template <int N, typename T, int TPB, bool PR>
 __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, StickArgs st,
                                                const T* __restrict__ veff, int nrb, int n_b,
                                                const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
+  using W = Window<N>;
+  constexpr int RSW = W::RS;
   const int P = gridDim.x, p = blockIdx.x;
   const int L = (p & 7) * (P >> 3) + (p >> 3);
   if (L >= N * n_b) return;  // grid padding (whole block)
   const int z = L / n_b, b = L - z * n_b;
-  const int RS = st.rs;
+  const int RS = PR ? RSW : st.rs;
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
   int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
@@ -129,23 +138,41 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   __syncthreads();
   int lane;
   const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
-  if (g >= 0)
-    for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
+  cplx<T>* sc = rowbuf + (g >= 0 ? g : 0) * RowScratch<N, T>::STRIDE;
+  if constexpr (PR) {
+    if (g >= 0 && g < nrb)
+      for (int a = g; a < st.n_act; a += nrb)
+        group_fft_line_x<N, 1, RSW, true, false>(cmp + a, sc, lane, tw);
+  } else {
+    if (g >= 0)
+      for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
+  }
   __syncthreads();
   if (g >= 0 && g < nrb) {
-    cplx<T>* sc = rowbuf + g * RowScratch<N, T>::STRIDE;
     const T* vz = veff + (size_t)z * N * N;
     if (TPB <= 512) {
       const RowPlanRegs<N, T> rp(act_of, tw, lane);
-      for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+      for (int t = g; t < N; t += nrb) {
+        const int iy = PR ? W::row(t) : t;
+        row_apply_v<N, T, PR>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+      }
     } else {
       const RowPlanLoad<N, T> rp(act_of, tw, lane);
-      for (int iy = g; iy < N; iy += nrb) row_apply_v<N, T>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+      for (int t = g; t < N; t += nrb) {
+        const int iy = PR ? W::row(t) : t;
+        row_apply_v<N, T, PR>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+      }
     }
   }
   __syncthreads();
-  if (g >= 0)
-    for (int a = g; a < st.n_act; a += ng) group_fft_line<N, -1>(cmp + a, RS, lane, tw, PostId());
+  if constexpr (PR) {
+    if (g >= 0 && g < nrb)
+      for (int a = g; a < st.n_act; a += nrb)
+        group_fft_line_x<N, -1, RSW, false, true>(cmp + a, sc, lane, tw);
+  } else {
+    if (g >= 0)
+      for (int a = g; a < st.n_act; a += ng) group_fft_line<N, -1>(cmp + a, RS, lane, tw, PostId());
+  }
   __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) plane[s] = cmp[st.iy[s] * RS + st.act[s]];
 }
@@ -355,6 +382,21 @@ int build_sticks(chs_context* ctx) {
   st.act_of = d + 5 * (size_t)n_st + n_act;
   st.pw_siz = d + 5 * (size_t)n_st + n_act + n;
   st.h_act_ix = act_ix;
+  // the plane kernels' zero window (fft.hpp Window<N>): usable when no
+  // active column and no stick row falls in it
+  st.rs_win = 0;
+  if (ctx->impl->plane_window && fft_length_supported(n)) {
+    int lo = 0, hi = 0, rsw = 0;
+    CHS_DISPATCH_N(n, {
+      lo = Window<NN>::LO;
+      hi = Window<NN>::HI;
+      rsw = Window<NN>::RS;
+    });
+    bool fits = hi > lo && n_act < rsw;
+    for (int x : act_ix) fits = fits && (x < lo || x >= hi);
+    for (int y : iy) fits = fits && (y < lo || y >= hi);
+    if (fits) st.rs_win = rsw;
+  }
   return CHS_OK;
 }
 
@@ -368,8 +410,10 @@ static StickArgs stick_args(const StickTable& st) {
   a.ld = stick_ld(st.n_st);
   a.n_act = st.n_act;
   a.n_pw = st.n_pw;
+  a.win = st.rs_win > 0 ? 1 : 0;
   a.rs = st.n_act + 1;
   while (a.rs % 8 != 1) ++a.rs;  // spreads column accesses over LDS banks
+  if (a.win) a.rs = st.rs_win;
   a.iy = st.iy;
   a.act = st.act;
   a.pw0 = st.pw0;
@@ -439,22 +483,35 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
   // 1024-thread blocks for complex64 only (vloc_pipeline_t never asks for
   // them with complex128, whose codelets do not fit the 128 registers per
   // lane that 16 wavefronts per CU leave)
+  // the window variant needs the compact plane laid out with its stride
+  if (sa.win && sa.rs != Window<NN>::RS)
+    return set_error(ctx, CHS_ERR_FFT, "stick table row stride %d != window stride %d", sa.rs,
+                     Window<NN>::RS);
   bool launched = false;
+#define CHS_PLANE(TPB, PR)                                                                        hipLaunchKernelGGL((k_plane<NN, T, TPB, PR>), dim3(pgrid), dim3(TPB), pl.lds, ctx->stream, sbuf,                      sa, veff, pl.nrb, n_b, tw)
   if constexpr (sizeof(T) == 4) {
     if (pl.threads == 1024) {
-      hipLaunchKernelGGL((k_plane<NN, T, 1024>), dim3(pgrid), dim3(1024), pl.lds, ctx->stream,
-                         sbuf, sa, veff, pl.nrb, n_b, tw);
+      if (sa.win)
+        CHS_PLANE(1024, true);
+      else
+        CHS_PLANE(1024, false);
       launched = true;
     }
   }
   if (!launched) {
-    if (pl.threads == 512)
-      hipLaunchKernelGGL((k_plane<NN, T, 512>), dim3(pgrid), dim3(512), pl.lds, ctx->stream, sbuf,
-                         sa, veff, pl.nrb, n_b, tw);
-    else
-      hipLaunchKernelGGL((k_plane<NN, T, 256>), dim3(pgrid), dim3(256), pl.lds, ctx->stream, sbuf,
-                         sa, veff, pl.nrb, n_b, tw);
+    if (pl.threads == 512) {
+      if (sa.win)
+        CHS_PLANE(512, true);
+      else
+        CHS_PLANE(512, false);
+    } else {
+      if (sa.win)
+        CHS_PLANE(256, true);
+      else
+        CHS_PLANE(256, false);
+    }
   }
+#undef CHS_PLANE
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
 #define CHS_FWD(M)                                                                            \
   hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \

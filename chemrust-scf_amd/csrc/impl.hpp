@@ -27,6 +27,9 @@ struct StickTable {
   int32_t* act_of = nullptr; // n_grid: active index of each x column, or -1
   int32_t* pw_siz = nullptr; // n_pw: (stick << 8) | FFT z index of each plane wave
   std::vector<int32_t> h_act_ix;
+  // compact-plane row stride of the plane kernels that skip the zero window
+  // (Window<N> in fft.hpp), or 0 when some stick or active column lies in it
+  int rs_win = 0;
 };
 
 // Six fp16 planes [m][k] of a fixed complex operand for the fp16x3 GEMM
@@ -124,6 +127,9 @@ struct chs_context::Impl {
   std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
   std::map<int, int> density_tpb;                          // N -> density-plane workgroup size
   bool hpsi_autotune = true;  // chs_scf_run tunes new grids (CHS_HPSI_TUNE=0: off)
+  // plane kernels that skip the zero window of each line when the stick
+  // table allows (CHS_PLANE_WINDOW=0: the general kernels throughout)
+  bool plane_window = true;
   chs::Hamiltonian ham;
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;

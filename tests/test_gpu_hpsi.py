@@ -81,6 +81,28 @@ def test_apply_h_fp32(ctx):
     assert err < 2e-5, err
 
 
+@pytest.mark.parametrize("n_rep,n_grid,kf", [(1, 36, (0, 0, 0)), (2, 72, (0, 0, 0)),
+                                             (1, 36, (0.25, 0.25, 0.25))])
+@pytest.mark.parametrize("window", ["1", "0"])
+def test_apply_h_plane_window(monkeypatch, n_rep, n_grid, kf, window):
+    """The plane kernels that skip each line's zero window (the default on
+    these grids) and the general ones (CHS_PLANE_WINDOW=0) both match the
+    oracle, in both precisions."""
+    monkeypatch.setenv("CHS_PLANE_WINDOW", window)
+    c = chs.Context(0)
+    nb = 9  # these grids have an odd number of active x columns: the paired
+    # y-lines of the window kernels take the zero spare column
+    m = model(n_rep, n_grid, nb, kfrac=kf)
+    H = bind(c, m)
+    psi = m.initial_psi(nb)
+    ref = m.apply_h(psi)
+    for dt, tol in ((np.complex128, 1e-12), (np.complex64, 2e-5)):
+        got = chs.apply_full_hamiltonian(c, chs.Gpu.from_host(c, psi.astype(dt))).sync_to_host()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < tol, (dt, err)
+    del H
+
+
 def test_apply_h_closed_forms(ctx):
     """V = 0 => H psi = 0.5|G|^2 psi; V = const => shift."""
     m = model(1, 36, 4)
