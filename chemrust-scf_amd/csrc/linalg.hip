@@ -535,8 +535,13 @@ int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w) {
 }
 
 // Rayleigh-Ritz on a c128 block Y (n_b x n_pw band-major): see file header.
+__global__ void k_widen(const c32* __restrict__ a, z64* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = mk((double)a[i].x, (double)a[i].y);
+}
+
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
-                       double* eps_dev_out, z64* w_out) {
+                       double* eps_dev_out, z64* w_out, const c32* Y32) {
   CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
@@ -550,7 +555,16 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   // with projectors and no H|Y> wanted: the local H|Y> only, the projector
   // parts of H_sub and S_sub in coefficient space (subspace_nl)
   const bool nl = H.n_proj > 0 && !HY_out;
-  if (nl)
+  const bool lo = nl && Y32;
+  if (lo) {
+    CHS_TRY(ensure_precision_ready(ctx, CHS_C64));
+    CHS_TRY(ensure(ctx, im->rr32, ne * sizeof(c32)));
+    CHS_TRY(vloc_pipeline(ctx, CHS_C64, Y32, n_b, im->rr32.p, EPI_HPSI, nullptr, nullptr, 0.0, 0.0,
+                          nullptr, nullptr));
+    hipLaunchKernelGGL(k_widen, dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                       (const c32*)im->rr32.p, HY, ne);
+    CHS_LAUNCH_CHECK(ctx);
+  } else if (nl)
     CHS_TRY(vloc_pipeline(ctx, CHS_C128, Y, n_b, HY, EPI_HPSI, nullptr, nullptr, 0.0, 0.0, nullptr,
                           nullptr));
   else
@@ -564,7 +578,7 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, HY, n_pw, zero, Hs, n_b));
   CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen && !nl ? SY : Y, n_pw, zero, Ss,
                 n_b));
-  if (nl) CHS_TRY(subspace_nl(ctx, Y, n_b, Hs, gen ? Ss : nullptr));
+  if (nl) CHS_TRY(subspace_nl(ctx, Y, n_b, Hs, gen ? Ss : nullptr, lo ? Y32 : nullptr));
   CHS_TRY(hermitise(ctx, Hs, n_b));
   CHS_TRY(hermitise(ctx, Ss, n_b));
   CHS_TRY(zhegv(ctx, Hs, Ss, n_b, w));

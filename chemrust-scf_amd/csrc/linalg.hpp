@@ -18,8 +18,11 @@ int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U);
 int zhegv(chs_context* ctx, z64* A, z64* B, int n, double* w);
 // w_out (optional, with projectors and no HY_out): beta^H Y of the rotated
 // block (P x n_b), from the pre-rotation coefficients times the Ritz vectors.
+// Y32 (optional, same case): Y held exactly in complex64 (the complex64
+// filter's output); H|Y> and beta^H Y are then formed in complex64 and
+// widened, the Gram matrices, the eigensolve and the rotation stay FP64.
 int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64* HY_out,
-                       double* eps_dev_out, z64* w_out = nullptr);
+                       double* eps_dev_out, z64* w_out = nullptr, const c32* Y32 = nullptr);
 int cholqr_impl(chs_context* ctx, z64* Y, int n_b);
 // fp16x3 complex GEMM (gemm_h3.hip): split planes of op(A) for a c128
 // projector block (P x n_pw, lda; cn: op(A) = A^H, else A), per-column

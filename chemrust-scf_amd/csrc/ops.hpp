@@ -16,8 +16,10 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
 // when spsi != nullptr, spsi = psi + beta Q beta^H psi.
 int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,
                    void* spsi);
-// Hs += W^H D W, Ss += W^H Q W (Ss may be null) with W = beta^H Y (c128).
-int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss);
+// Hs += W^H D W, Ss += W^H Q W (Ss may be null) with W = beta^H Y (c128);
+// with Y32 (Y held exactly in complex64) W is formed in complex64 and widened.
+int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss,
+                const c32* Y32 = nullptr);
 // x <- S^{-1} x = x - beta M beta^H x (Woodbury).
 int sinv_apply(chs_context* ctx, int precision, void* x, int n_b);
 // Chebyshev filter with projectors, in projector-coefficient space:

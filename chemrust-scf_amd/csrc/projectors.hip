@@ -268,7 +268,7 @@ int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
 // with W = beta^H Y (P x n_b), Hs += W^H D W and, when Ss is given,
 // Ss += W^H Q W. One n_pw-long GEMM instead of the two (beta D W, beta Q W)
 // a full H|Y>, S|Y> would need.
-int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss) {
+int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss, const c32* Y32) {
   Hamiltonian& H = ctx->impl->ham;
   const int P = H.n_proj, n_pw = H.n_pw;
   const size_t pb = (size_t)P * n_b;
@@ -277,7 +277,18 @@ int subspace_nl(chs_context* ctx, const z64* Y, int n_b, z64* Hs, z64* Ss) {
   z64* DW = W + pb;
   z64* QW = DW + pb;
   const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, one, (const z64*)H.beta, n_pw, Y, n_pw, zero, W, P));
+  if (Y32) {
+    CHS_TRY(ensure_c64_projectors(ctx));
+    CHS_TRY(ensure(ctx, ctx->impl->rr32, pb * sizeof(c32)));
+    c32* W32 = (c32*)ctx->impl->rr32.p;
+    CHS_TRY(cgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0f, 0.0f), (const c32*)H.beta_f.p, n_pw, Y32,
+                  n_pw, mk(0.0f, 0.0f), W32, P));
+    hipLaunchKernelGGL((k_convert<c32, z64>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream,
+                       W32, W, pb);
+    CHS_LAUNCH_CHECK(ctx);
+  } else {
+    CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, one, (const z64*)H.beta, n_pw, Y, n_pw, zero, W, P));
+  }
   hipLaunchKernelGGL((k_row_scale2<double>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream, W,
                      P, n_b, H.dion, H.qion, DW, Ss ? QW : nullptr);
   CHS_LAUNCH_CHECK(ctx);

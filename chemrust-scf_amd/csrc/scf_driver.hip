@@ -212,8 +212,11 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
       eps.resize(n_b);
       // the last pass also hands over beta^H psi of the rotated block for
       // the augmentation charge
+      // in the complex64 phase H|Y> and beta^H Y come from the filter's
+      // complex64 output (the Gram matrices and the eigensolve stay FP64)
       CHS_TRY(rayleigh_ritz_impl(ctx, (z64*)S.psi.p, n_b, eps.data(), nullptr, nullptr,
-                                 P > 0 && ps + 1 == passes ? (z64*)S.wproj.p : nullptr));
+                                 P > 0 && ps + 1 == passes ? (z64*)S.wproj.p : nullptr,
+                                 use32 ? (const c32*)S.psi32.p : nullptr));
       t_rr += now() - tf1;
     }
     tb = now();
