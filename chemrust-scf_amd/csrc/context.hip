@@ -90,6 +90,7 @@ int chs_context_create(int device, chs_context** out) {
   if (const char* ng = std::getenv("CHS_NO_GRAPHS")) ctx->impl->use_graphs = !(ng[0] && ng[0] != '0');
   if (const char* gs = std::getenv("CHS_GEMM_SIMT")) ctx->impl->use_mfma = !(gs[0] && gs[0] != '0');
   if (const char* gt = std::getenv("CHS_GEMM_TUNE")) ctx->impl->gemm_autotune = !(gt[0] == '0');
+  if (const char* gc = std::getenv("CHS_GEMM_CHOICE")) ctx->impl->gemm_choice = std::atoi(gc);
   if (const char* ht = std::getenv("CHS_HPSI_TUNE")) ctx->impl->hpsi_autotune = !(ht[0] == '0');
   if (const char* pw = std::getenv("CHS_PLANE_WINDOW")) ctx->impl->plane_window = !(pw[0] == '0');
   if (const char* g3 = std::getenv("CHS_GEMM_FP16X3")) ctx->impl->use_h3 = g3[0] && g3[0] != '0';

@@ -162,7 +162,11 @@ __device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int nfast, in
   return true;
 }
 
-template <typename T, int OPA, int OPB, int V>
+// CL: operand loads at clamped addresses with a select for the out-of-range
+// elements (no branch per element, so a stage's loads issue back to back);
+// otherwise each element's load sits behind its bounds test. Which is faster
+// depends on the shape (the autotuner decides).
+template <typename T, int OPA, int OPB, int V, bool CL>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
                                                    const cplx<T>* __restrict__ A, int lda,
@@ -212,14 +216,24 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
       int r, kk;
       a_pos(threadIdx.x + 256 * j, r, kk);
       const int gr = m0 + r, gk = k0 + kk;
-      ra[j] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
+      if (CL) {
+        const cplx<T> v = load_op<T, OPA>(A, lda, min(gr, m - 1), min(gk, ke - 1));
+        ra[j] = (gr < m && gk < ke) ? v : mk((T)0, (T)0);
+      } else {
+        ra[j] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
+      }
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       int c, kk;
       b_pos(threadIdx.x + 256 * j, c, kk);
       const int gc = n0 + c, gk = k0 + kk;
-      rb[j] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+      if (CL) {
+        const cplx<T> v = load_op<T, OPB>(B, ldb, min(gk, ke - 1), min(gc, n - 1));
+        rb[j] = (gc < n && gk < ke) ? v : mk((T)0, (T)0);
+      } else {
+        rb[j] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
+      }
     }
   };
   if (kb < ke) load(kb);
@@ -460,9 +474,9 @@ static constexpr int kSplitTargets[4] = {512, 1024, 2048, 4096};
 template <typename T>
 static void tile_dims(int variant, int* bm, int* bn, int* bk) {
   *bm = GBM; *bn = GBN; *bk = GBK;
-  if (variant == 1) {
+  if (variant == 1 || variant == 3) {
     *bm = MfmaTile<T, 1>::BM; *bn = MfmaTile<T, 1>::BN; *bk = MfmaTile<T, 1>::BK;
-  } else if (variant == 2) {
+  } else if (variant == 2 || variant == 4) {
     *bm = MfmaTile<T, 2>::BM; *bn = MfmaTile<T, 2>::BN; *bk = MfmaTile<T, 2>::BK;
   }
 }
@@ -503,11 +517,17 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
 #define CHS_GEMM_CASE(OA, OB)                                                                    \
   if (oa == OA && ob == OB) {                                                                    \
     if (variant == 1)                                                                            \
-      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
-                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);        \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1, false>), g1, dim3(256), 0, ctx->stream, m,   \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
     else if (variant == 2)                                                                       \
-      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2>), g1, dim3(256), 0, ctx->stream, m, n, k,    \
-                         alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);        \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2, false>), g1, dim3(256), 0, ctx->stream, m,   \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+    else if (variant == 3)                                                                       \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1, true>), g1, dim3(256), 0, ctx->stream, m,    \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+    else if (variant == 4)                                                                       \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2, true>), g1, dim3(256), 0, ctx->stream, m,    \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
     else                                                                                         \
       hipLaunchKernelGGL((k_gemm<T, OA, OB>), g3, dim3(256), 0, ctx->stream, m, n, k, alpha, A,  \
                          lda, B, ldb, beta, C, ldc, klen, part);                                 \
@@ -526,7 +546,8 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
 }
 
 // Kernel choice for a GEMM shape: variant (low 4 bits: 0 vector-FMA 64 x 64,
-// 1 MFMA 64 x 64, 2 MFMA 128-wide) and split-K target (bits 4-5, index into
+// 1 MFMA 64 x 64, 2 MFMA 128-wide, 3 and 4 the same with clamped loads) and
+// split-K target (bits 4-5, index into
 // kSplitTargets). Chosen once per (precision, ops, m, n, k) by timing every
 // distinct launch into a scratch output (HIP events, context stream);
 // outside a stream capture only, otherwise the MFMA 64 x 64 default.
@@ -536,6 +557,10 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   auto* im = ctx->impl;
   if (!im->use_mfma) {
     *choice = 0;
+    return CHS_OK;
+  }
+  if (im->gemm_choice >= 0) {
+    *choice = im->gemm_choice;
     return CHS_OK;
   }
   const GemmKey key{(int)sizeof(T), oa, ob, m, n, k};
@@ -552,7 +577,7 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   cplx<T>* Ct = (cplx<T>*)im->gemm_tune_c.p;
   const cplx<T> one = mk((T)1, (T)0), zero = mk((T)0, (T)0);
   double best = 1e300;
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < 5; ++v) {
     int prev_split = -1;
     for (int ti = 0; ti < 4; ++ti) {
       int bm, bn, bk, ks, kl;
@@ -563,13 +588,19 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
       const int c = v | (ti << 4);
       CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
                              im->gemm_tune_ws));  // warm-up, workspace growth
-      CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
-      CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
-                             im->gemm_tune_ws));
-      CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
-      CHS_HIP(ctx, hipEventSynchronize(im->ev1));
-      float t = 0;
-      CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
+      // best of three launches: candidates a few percent apart must not be
+      // ranked by one launch's noise
+      float t = 1e30f;
+      for (int rep = 0; rep < 3; ++rep) {
+        CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
+        CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
+                               im->gemm_tune_ws));
+        CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
+        CHS_HIP(ctx, hipEventSynchronize(im->ev1));
+        float tr = 0;
+        CHS_HIP(ctx, hipEventElapsedTime(&tr, im->ev0, im->ev1));
+        t = std::min(t, tr);
+      }
       if (t < best) {
         best = t;
         *choice = c;

@@ -149,6 +149,7 @@ struct chs_context::Impl {
   bool use_h3 = false;
   // per-shape GEMM kernel variant, timed on first use (CHS_GEMM_TUNE=0: off)
   bool gemm_autotune = true;
+  int gemm_choice = -1;  // CHS_GEMM_CHOICE: one fixed kernel choice for every shape (>= 0)
   std::map<chs::GemmKey, int> gemm_tune;
   chs::DevBuf gemm_tune_ws, gemm_tune_c;
   // counts workspace (re)allocations: a captured graph is valid only while
