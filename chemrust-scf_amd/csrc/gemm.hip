@@ -133,6 +133,12 @@ template <> struct MfmaTile<float, 2> {
   static constexpr int BM = 128, BN = 128, BK = 16, WTM = 4, WTN = 4;
   static constexpr int WAVES = 1;
 };
+// 64 x 64 with a 32-deep K stage: half the barriers per matrix-core product
+// and twice the loads in flight per stage, for the long-K products
+template <typename T> struct MfmaTile<T, 3> {
+  static constexpr int BM = 64, BN = 64, BK = 32, WTM = 2, WTN = 2;
+  static constexpr int WAVES = sizeof(T) == 8 ? 2 : 4;
+};
 template <> struct MfmaTile<double, 2> {
   static constexpr int BM = 128, BN = 64, BK = 8, WTM = 4, WTN = 2;
   static constexpr int WAVES = 1;
@@ -476,6 +482,8 @@ static void tile_dims(int variant, int* bm, int* bn, int* bk) {
   *bm = GBM; *bn = GBN; *bk = GBK;
   if (variant == 1 || variant == 3) {
     *bm = MfmaTile<T, 1>::BM; *bn = MfmaTile<T, 1>::BN; *bk = MfmaTile<T, 1>::BK;
+  } else if (variant == 5) {
+    *bm = MfmaTile<T, 3>::BM; *bn = MfmaTile<T, 3>::BN; *bk = MfmaTile<T, 3>::BK;
   } else if (variant == 2 || variant == 4) {
     *bm = MfmaTile<T, 2>::BM; *bn = MfmaTile<T, 2>::BN; *bk = MfmaTile<T, 2>::BK;
   }
@@ -528,6 +536,9 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
     else if (variant == 4)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2, true>), g1, dim3(256), 0, ctx->stream, m,    \
                          n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+    else if (variant == 5)                                                                       \
+      hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 3, true>), g1, dim3(256), 0, ctx->stream, m,    \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
     else                                                                                         \
       hipLaunchKernelGGL((k_gemm<T, OA, OB>), g3, dim3(256), 0, ctx->stream, m, n, k, alpha, A,  \
                          lda, B, ldb, beta, C, ldc, klen, part);                                 \
@@ -546,7 +557,8 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
 }
 
 // Kernel choice for a GEMM shape: variant (low 4 bits: 0 vector-FMA 64 x 64,
-// 1 MFMA 64 x 64, 2 MFMA 128-wide, 3 and 4 the same with clamped loads) and
+// 1 MFMA 64 x 64, 2 MFMA 128-wide, 3 and 4 the same with clamped loads, 5
+// MFMA 64 x 64 x 32 with clamped loads) and
 // split-K target (bits 4-5, index into
 // kSplitTargets). Chosen once per (precision, ops, m, n, k) by timing every
 // distinct launch into a scratch output (HIP events, context stream);
@@ -577,7 +589,7 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   cplx<T>* Ct = (cplx<T>*)im->gemm_tune_c.p;
   const cplx<T> one = mk((T)1, (T)0), zero = mk((T)0, (T)0);
   double best = 1e300;
-  for (int v = 0; v < 5; ++v) {
+  for (int v = 0; v < 6; ++v) {
     int prev_split = -1;
     for (int ti = 0; ti < 4; ++ti) {
       int bm, bn, bk, ks, kl;
