@@ -178,7 +178,7 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
                                                    const cplx<T>* __restrict__ A, int lda,
                                                    const cplx<T>* __restrict__ B, int ldb,
                                                    cplx<T> beta, cplx<T>* __restrict__ C, int ldc,
-                                                   int klen, int ksplit, int nfast,
+                                                   int klen, int ksplit, int nfast, int tri,
                                                    cplx<T>* __restrict__ part) {
   using M = Mfma<T>;
   using acc_t = typename M::acc_t;
@@ -187,6 +187,7 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
   constexpr int WTM = TL::WTM, WTN = TL::WTN;
   int bm, bn, bz;
   if (!gemm_block((m + BM - 1) / BM, (n + BN - 1) / BN, ksplit, nfast, &bm, &bn, &bz)) return;
+  if (tri && bm * BM >= (bn + 1) * BN) return;  // Hermitian product: tiles below the diagonal
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // staged elements per thread
   // one padding element per row: the K-major (transposing) stores of T / C
   // operands then hit distinct banks across each phase of lanes, while the
@@ -489,9 +490,15 @@ static void tile_dims(int variant, int* bm, int* bn, int* bk) {
   }
 }
 
+// tri: only the tiles that reach the diagonal or above are computed
 static void split_plan(int m, int n, int k, int bm, int bn, int bk, int target, int* ksplit,
-                       int* klen) {
-  const int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
+                       int* klen, bool tri = false) {
+  int tiles = (int)(ceil_div(m, bm) * ceil_div(n, bn));
+  if (tri) {
+    tiles = 0;
+    for (int i = 0; i < (int)ceil_div(m, bm); ++i)
+      for (int j = 0; j < (int)ceil_div(n, bn); ++j) tiles += i * bm < (j + 1) * bn;
+  }
   int ks = 1;
   // at most `target` workgroups: whole rounds of resident workgroups, no
   // nearly empty last round
@@ -507,12 +514,12 @@ static void split_plan(int m, int n, int k, int bm, int bn, int bk, int target, 
 template <typename T>
 static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int n, int k,
                        cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
-                       cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws) {
+                       cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws, int tri = 0) {
   const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % 4];
   int bm, bn, bk;
   tile_dims<T>(variant, &bm, &bn, &bk);
   int ksplit, klen;
-  split_plan(m, n, k, bm, bn, bk, target, &ksplit, &klen);
+  split_plan(m, n, k, bm, bn, bk, target, &ksplit, &klen, tri && variant != 0);
   cplx<T>* part = nullptr;
   if (ksplit > 1) {
     CHS_TRY(ensure(ctx, ws, (size_t)ksplit * m * n * sizeof(cplx<T>)));
@@ -526,19 +533,19 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
   if (oa == OA && ob == OB) {                                                                    \
     if (variant == 1)                                                                            \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1, false>), g1, dim3(256), 0, ctx->stream, m,   \
-                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, tri, part);  \
     else if (variant == 2)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2, false>), g1, dim3(256), 0, ctx->stream, m,   \
-                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, tri, part);  \
     else if (variant == 3)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 1, true>), g1, dim3(256), 0, ctx->stream, m,    \
-                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, tri, part);  \
     else if (variant == 4)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 2, true>), g1, dim3(256), 0, ctx->stream, m,    \
-                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, tri, part);  \
     else if (variant == 5)                                                                       \
       hipLaunchKernelGGL((k_gemm_mfma<T, OA, OB, 3, true>), g1, dim3(256), 0, ctx->stream, m,    \
-                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, part);  \
+                         n, k, alpha, A, lda, B, ldb, beta, C, ldc, klen, ksplit, nfast, tri, part);  \
     else                                                                                         \
       hipLaunchKernelGGL((k_gemm<T, OA, OB>), g3, dim3(256), 0, ctx->stream, m, n, k, alpha, A,  \
                          lda, B, ldb, beta, C, ldc, klen, part);                                 \
@@ -565,7 +572,7 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
 // outside a stream capture only, otherwise the MFMA 64 x 64 default.
 template <typename T>
 static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, const cplx<T>* A,
-                        int lda, const cplx<T>* B, int ldb, int* choice) {
+                        int lda, const cplx<T>* B, int ldb, int* choice, int tri = 0) {
   auto* im = ctx->impl;
   if (!im->use_mfma) {
     *choice = 0;
@@ -575,7 +582,7 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
     *choice = im->gemm_choice;
     return CHS_OK;
   }
-  const GemmKey key{(int)sizeof(T), oa, ob, m, n, k};
+  const GemmKey key{(int)sizeof(T), oa, ob + 16 * tri, m, n, k};
   auto it = im->gemm_tune.find(key);
   if (it != im->gemm_tune.end()) {
     *choice = it->second;
@@ -594,19 +601,19 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
     for (int ti = 0; ti < 4; ++ti) {
       int bm, bn, bk, ks, kl;
       tile_dims<T>(v, &bm, &bn, &bk);
-      split_plan(m, n, k, bm, bn, bk, kSplitTargets[ti], &ks, &kl);
+      split_plan(m, n, k, bm, bn, bk, kSplitTargets[ti], &ks, &kl, tri && v != 0);
       if (ks == prev_split) continue;  // same launch as a smaller target
       prev_split = ks;
       const int c = v | (ti << 4);
       CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
-                             im->gemm_tune_ws));  // warm-up, workspace growth
+                             im->gemm_tune_ws, tri));  // warm-up, workspace growth
       // best of three launches: candidates a few percent apart must not be
       // ranked by one launch's noise
       float t = 1e30f;
       for (int rep = 0; rep < 3; ++rep) {
         CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
         CHS_TRY(gemm_launch<T>(ctx, c, oa, ob, m, n, k, one, A, lda, B, ldb, zero, Ct, m,
-                               im->gemm_tune_ws));
+                               im->gemm_tune_ws, tri));
         CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
         CHS_HIP(ctx, hipEventSynchronize(im->ev1));
         float tr = 0;
@@ -641,6 +648,27 @@ static int gemm_t(chs_context* ctx, char opa, char opb, int m, int n, int k, cpl
 int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
           int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc) {
   return gemm_t<double>(ctx, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+// C[i][j] = conj(C[j][i]) for i > j
+__global__ void k_mirror_lower(z64* __restrict__ C, int n, int ldc) {
+  const int i = blockIdx.x * 16 + (threadIdx.x & 15), j = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (i < n && j < i) C[(size_t)i + (size_t)j * ldc] = conj(C[(size_t)j + (size_t)i * ldc]);
+}
+
+int zgemm_herm(chs_context* ctx, int n, int k, const z64* A, int lda, const z64* B, int ldb, z64* C,
+               int ldc) {
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+  if (n < 128 || k <= 0 || !ctx->impl->use_mfma)
+    return gemm_t<double>(ctx, 'C', 'N', n, n, k, one, A, lda, B, ldb, zero, C, ldc);
+  int choice = 1;
+  CHS_TRY(gemm_variant<double>(ctx, 2, 0, n, n, k, A, lda, B, ldb, &choice, 1));
+  CHS_TRY(gemm_launch<double>(ctx, choice, 2, 0, n, n, k, one, A, lda, B, ldb, zero, C, ldc,
+                              ctx->impl->gemm_ws, 1));
+  hipLaunchKernelGGL(k_mirror_lower, dim3(ceil_div(n, 16), ceil_div(n, 16)), dim3(256), 0,
+                     ctx->stream, C, n, ldc);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
 }
 
 int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,

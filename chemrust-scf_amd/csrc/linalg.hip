@@ -575,9 +575,9 @@ int rayleigh_ritz_impl(chs_context* ctx, z64* Y, int n_b, double* eps_host, z64*
   z64* Ss = Hs + nn;
   double* w = (double*)(Ss + nn);
   const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, HY, n_pw, zero, Hs, n_b));
-  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen && !nl ? SY : Y, n_pw, zero, Ss,
-                n_b));
+  // both Gram products are Hermitian: upper tiles only, lower triangle mirrored
+  CHS_TRY(zgemm_herm(ctx, n_b, n_pw, Y, n_pw, HY, n_pw, Hs, n_b));
+  CHS_TRY(zgemm_herm(ctx, n_b, n_pw, Y, n_pw, gen && !nl ? SY : Y, n_pw, Ss, n_b));
   if (nl) CHS_TRY(subspace_nl(ctx, Y, n_b, Hs, gen ? Ss : nullptr, lo ? Y32 : nullptr));
   CHS_TRY(hermitise(ctx, Hs, n_b));
   CHS_TRY(hermitise(ctx, Ss, n_b));

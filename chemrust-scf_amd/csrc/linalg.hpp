@@ -6,6 +6,10 @@ namespace chs {
 
 int zgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, z64 alpha, const z64* A,
           int lda, const z64* B, int ldb, z64 beta, z64* C, int ldc);
+// C = A^H B (n x n) for a product the caller knows to be Hermitian: only the
+// tiles on or above the diagonal are computed, the lower triangle mirrored.
+int zgemm_herm(chs_context* ctx, int n, int k, const z64* A, int lda, const z64* B, int ldb, z64* C,
+               int ldc);
 int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,
           int lda, const c32* B, int ldb, c32 beta, c32* C, int ldc);
 int gemm_opcode(char op);
