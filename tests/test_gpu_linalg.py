@@ -48,6 +48,39 @@ def test_zgemm_random(ctx, opa, opb, m, n, k):
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
 
 
+@pytest.mark.parametrize("choice", [0, 1, 2, 3, 4, 5, 0x31, 0x35])
+def test_zgemm_every_kernel_choice(monkeypatch, choice):
+    """Each GEMM kernel the autotuner can pick (variant in the low bits, split-K
+    target index in bits 4-5), pinned with CHS_GEMM_CHOICE, on ragged shapes
+    (partial tiles in m, n and the last K slice), both precisions."""
+    monkeypatch.setenv("CHS_GEMM_CHOICE", str(choice))
+    c = chs.Context(0)
+    rng = np.random.default_rng(choice)
+    for opa, opb, m, n, k in (("C", "N", 150, 77, 1303), ("N", "N", 1303, 77, 150),
+                              ("N", "C", 97, 130, 211)):
+        sa = (m, k) if opa == "N" else (k, m)
+        sb = (k, n) if opb == "N" else (n, k)
+        A = rng.standard_normal(sa) + 1j * rng.standard_normal(sa)
+        B = rng.standard_normal(sb) + 1j * rng.standard_normal(sb)
+        C0 = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        ref = (0.7 - 0.2j) * (A.conj().T if opa == "C" else A) @ \
+            (B.conj().T if opb == "C" else B) + (0.3 + 0.1j) * C0
+        for dt, tol in ((np.complex128, 1e-13), (np.complex64, 3e-6)):
+            a, b = chs.Gpu.from_host(c, fmat(A).astype(dt)), chs.Gpu.from_host(c, fmat(B).astype(dt))
+            cc = chs.Gpu.from_host(c, fmat(C0).astype(dt))
+            if dt == np.complex128:
+                chs.zgemm(c, opa, opb, m, n, k, 0.7 - 0.2j, a, sa[0], b, sb[0], 0.3 + 0.1j, cc, m)
+            else:
+                al = np.array([0.7, -0.2], np.float32)
+                be = np.array([0.3, 0.1], np.float32)
+                c.check(chs.lib().chs_cgemm(c.h, opa.encode(), opb.encode(), m, n, k,
+                                            al.ctypes.data_as(chs.C.POINTER(chs.C.c_float)), a.ptr,
+                                            sa[0], b.ptr, sb[0],
+                                            be.ctypes.data_as(chs.C.POINTER(chs.C.c_float)), cc.ptr, m))
+            got = cc.sync_to_host().T
+            assert np.abs(got - ref).max() / np.abs(ref).max() < tol, (choice, opa, opb, dt)
+
+
 def test_zhegvd_kat_identity(ctx):
     # src/device/solver.rs:113-163: A = diag(1,2,3,4), B = I
     a = chs.Gpu.from_host(ctx, fmat(np.diag([1.0, 2, 3, 4]).astype(complex)))
