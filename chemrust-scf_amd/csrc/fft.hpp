@@ -464,6 +464,52 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
   wave_sync();
 }
 
+// Inverse x-transform of one compact row for the density: the first half of
+// row_apply_v (zero-window inputs not read), each output y[x] then adding
+// f |y[x]|^2 to orow[x] in FP64. A row belongs to one group, an x to one
+// lane of it, so the update needs no atomics.
+template <int N, typename T, typename Plan>
+__device__ __forceinline__ void row_density(const cplx<T>* crow, cplx<T>* sc, const Plan& rp,
+                                            double* __restrict__ orow, double f, int lane) {
+  constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  constexpr int R = (N1 + N2 - 1) / N2, PA = RowScratch<N, T>::PA;
+  using W = Window<N>;
+  {
+    cplx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      if (W::zero_in(n1)) {
+        a[n1] = mk((T)0, (T)0);
+      } else {
+        const int ai = rp.in(n1);
+        a[n1] = ai >= 0 ? crow[ai] : mk((T)0, (T)0);
+      }
+    }
+    Dft<N1, 1, T>::run(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = a[k1] * rp.w1(k1);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) sc[k1 * PA + lane] = a[k1];
+  }
+  wave_sync();
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k1 = lane + r * N2;
+    if (k1 < N1) {
+      cplx<T> b[N2];
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[n2] = sc[k1 * PA + n2];
+      Dft<N2, 1, T>::run(b);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const double re = (double)b[k2].x, im = (double)b[k2].y;
+        orow[k1 + N1 * k2] += f * (re * re + im * im);
+      }
+    }
+  }
+  wave_sync();
+}
+
 // Group geometry of a block: groups of N2 lanes packed into each wavefront.
 template <int N> struct Groups {
   static constexpr int N2 = Split<N>::N2;

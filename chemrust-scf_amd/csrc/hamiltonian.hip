@@ -297,6 +297,57 @@ __global__ void __launch_bounds__(TPB) k_density_plane(const z64* __restrict__ s
   }
 }
 
+// Density planes for a stick table inside the zero window (st.win), either
+// precision with FP64 accumulation: per (z-plane, chunk of bands) and
+// occupied band, sticks -> compact plane (stride Window<N>::RS), inverse
+// y-lines through the groups' tiles (first nrb groups), then the rows in the
+// bank-spreading order, each an inverse x-transform read straight from the
+// compact row (row_density).
+template <int N, typename T, int TPB>
+__global__ void __launch_bounds__(TPB) k_density_plane_w(const cplx<T>* __restrict__ sbuf,
+                                                         StickArgs st,
+                                                         const double* __restrict__ occ, int n_b,
+                                                         int chunk, int nrb,
+                                                         double* __restrict__ part,
+                                                         const cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  using W = Window<N>;
+  constexpr int RS = W::RS;
+  cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* rowbuf = cmp + N * RS;
+  int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
+  const int z = blockIdx.x, c = blockIdx.y;
+  const int b0 = c * chunk, b1 = min(n_b, b0 + chunk);
+  double* out = part + ((size_t)c * N + z) * N * N;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) out[i] = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
+  __syncthreads();
+  int lane;
+  const int g = Groups<N>::id(&lane);
+  const bool grp = g >= 0 && g < nrb;
+  cplx<T>* sc = rowbuf + (grp ? g : 0) * RowScratch<N, T>::STRIDE;
+  const RowPlanRegs<N, T> rp(act_of, tw, grp ? lane : 0);
+  for (int b = b0; b < b1; ++b) {
+    const double f = occ[b];
+    if (f == 0.0) continue;
+    const cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.ld;
+    __syncthreads();
+    for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
+    __syncthreads();
+    for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
+    __syncthreads();
+    if (grp)
+      for (int a = g; a < st.n_act; a += nrb)
+        group_fft_line_x<N, 1, RS, true, false>(cmp + a, sc, lane, tw);
+    __syncthreads();
+    if (grp)
+      for (int t = g; t < N; t += nrb) {
+        const int iy = W::row(t);
+        row_density<N, T>(cmp + iy * RS, sc, rp, out + iy * N, f, lane);
+      }
+  }
+}
+
 __global__ void k_density_reduce(const double* __restrict__ part, int n_chunks, size_t n3,
                                  double inv_omega, double* __restrict__ rho) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -649,14 +700,21 @@ int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, voi
   return CHS_OK;
 }
 
-int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
-                 double* d_rho) {
+int ensure_precision_ready(chs_context* ctx, int precision);
+
+// rho = sum_b occ_b |psi_b(r)|^2 / omega from a band-major block in either
+// precision (FP64 accumulation).
+template <typename T>
+static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double* occ_host,
+                     double omega, double* d_rho) {
   Hamiltonian& H = ctx->impl->ham;
   if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "no Hamiltonian bound");
   auto* im = ctx->impl;
   const StickTable& st = H.st;
   const int n = H.n_grid;
   const size_t n3 = (size_t)n * n * n;
+  const int precision = sizeof(T) == 8 ? CHS_C128 : CHS_C64;
+  CHS_TRY(ensure_precision_ready(ctx, precision));
   int nb = 0;  // bands up to the last occupied one
   for (int b = 0; b < n_b; ++b)
     if (occ_host[b] != 0.0) nb = b + 1;
@@ -672,31 +730,49 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
     CHS_HIP(ctx, hipMemsetAsync(d_rho, 0, n3 * sizeof(double), ctx->stream));
     return CHS_OK;
   }
-  CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * stick_ld(st.n_st) * sizeof(z64)));
-  z64* sbuf = (z64*)im->stick_buf.p;
+  CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * stick_ld(st.n_st) * sizeof(cplx<T>)));
+  cplx<T>* sbuf = (cplx<T>*)im->stick_buf.p;
   const void* twv = nullptr;
-  CHS_TRY(twiddles(ctx, n, CHS_C128, &twv));
-  const z64* tw = (const z64*)twv;
+  CHS_TRY(twiddles(ctx, n, precision, &twv));
+  const cplx<T>* tw = (const cplx<T>*)twv;
   StickArgs sa = stick_args(st);
   constexpr int SB = 16;
   const int gz = ceil_div(st.n_st, SB);
   // workgroup size of the density planes (256 or 512 threads), timed once per
-  // grid on the first call (both give bit-identical partial planes)
-  auto tpb_it = im->density_tpb.find(n);
+  // (grid, precision) on the first call (both give bit-identical partial planes)
+  const int key = 2 * n + (sizeof(T) == 4 ? 1 : 0);
+  auto tpb_it = im->density_tpb.find(key);
   const bool tune = tpb_it == im->density_tpb.end();
   CHS_DISPATCH_N(n, {
-    hipLaunchKernelGGL((k_stick_inv<NN, double, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
+    if (sa.win && sa.rs != Window<NN>::RS)
+      return set_error(ctx, CHS_ERR_FFT, "stick table row stride %d != window stride %d", sa.rs,
+                       Window<NN>::RS);
+    hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
     auto planes = [&](int tpb) -> int {
       PlaneLaunch pl;
-      CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, tpb, NN + 1, &pl));
-      if (tpb == 512)
-        hipLaunchKernelGGL((k_density_plane<NN, 512>), dim3(NN, n_chunks), dim3(512), pl.lds,
-                           ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
-      else
-        hipLaunchKernelGGL((k_density_plane<NN, 256>), dim3(NN, n_chunks), dim3(256), pl.lds,
-                           ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+      if (sa.win) {
+        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(cplx<T>), Split<NN>::N2, 1, tpb,
+                             RowScratch<NN, T>::STRIDE, &pl));
+        if (tpb == 512)
+          hipLaunchKernelGGL((k_density_plane_w<NN, T, 512>), dim3(NN, n_chunks), dim3(512),
+                             pl.lds, ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+        else
+          hipLaunchKernelGGL((k_density_plane_w<NN, T, 256>), dim3(NN, n_chunks), dim3(256),
+                             pl.lds, ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+      } else if constexpr (sizeof(T) == 8) {
+        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, tpb, NN + 1, &pl));
+        if (tpb == 512)
+          hipLaunchKernelGGL((k_density_plane<NN, 512>), dim3(NN, n_chunks), dim3(512), pl.lds,
+                             ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+        else
+          hipLaunchKernelGGL((k_density_plane<NN, 256>), dim3(NN, n_chunks), dim3(256), pl.lds,
+                             ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
+      } else {
+        return set_error(ctx, CHS_ERR_INVALID_ARG,
+                         "complex64 density needs a stick table inside the zero window");
+      }
       CHS_LAUNCH_CHECK(ctx);
       return CHS_OK;
     };
@@ -715,7 +791,7 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
           best = tpb;
         }
       }
-      im->density_tpb[n] = best;
+      im->density_tpb[key] = best;
     } else {
       CHS_TRY(planes(tpb_it->second));
     }
@@ -724,6 +800,16 @@ int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_ho
                      n_chunks, n3, 1.0 / omega, d_rho);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
+}
+
+int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
+                 double* d_rho) {
+  return density_t<double>(ctx, psi, n_b, occ_host, omega, d_rho);
+}
+
+int density_c64(chs_context* ctx, const c32* psi, int n_b, const double* occ_host, double omega,
+                double* d_rho) {
+  return density_t<float>(ctx, psi, n_b, occ_host, omega, d_rho);
 }
 
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {

@@ -47,5 +47,8 @@ double fermi_occupations(const double* eps, int n, double n_e, double kt, double
 // rho(r) = sum_b occ_b |psi_b(r)|^2 / omega for a c128 band-major block
 int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
                  double* d_rho);
+// the same from a complex64 block (FP64 accumulation; zero-window stick tables)
+int density_c64(chs_context* ctx, const c32* psi, int n_b, const double* occ_host, double omega,
+                double* d_rho);
 
 }  // namespace chs

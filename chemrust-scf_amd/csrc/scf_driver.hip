@@ -222,7 +222,18 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     tb = now();
     double ts = 0;
     mu = fermi_occupations(eps.data(), n_b, n_e, p->smear_kt, occ.data(), &ts);
-    CHS_TRY(density_impl(ctx, (const z64*)S.psi.p, n_b, occ.data(), omega, (double*)S.rho_out.p));
+    if (use32 && S.psi32.p && ctx->impl->ham.st.rs_win > 0) {
+      // complex64 phase: the density from the rotated block rounded to
+      // complex64 (FP64 accumulation), like the filter that produced it
+      hipLaunchKernelGGL(k_z2c, dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                         (const z64*)S.psi.p, (c32*)S.psi32.p, ne);
+      CHS_LAUNCH_CHECK(ctx);
+      CHS_TRY(density_c64(ctx, (const c32*)S.psi32.p, n_b, occ.data(), omega,
+                          (double*)S.rho_out.p));
+    } else {
+      CHS_TRY(density_impl(ctx, (const z64*)S.psi.p, n_b, occ.data(), omega,
+                           (double*)S.rho_out.p));
+    }
     if (P > 0 && p->q0 != 0.0) {
       CHS_HIP(ctx, hipMemcpyAsync(S.occ.p, occ.data(), n_b * sizeof(double),
                                   hipMemcpyHostToDevice, ctx->stream));

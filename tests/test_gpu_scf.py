@@ -27,6 +27,28 @@ def test_density_vs_oracle(ctx):
     assert abs(got.sum() * omega / 36 ** 3 - occ.sum()) < 1e-10
 
 
+@pytest.mark.parametrize("n_rep,n_grid,nb", [(1, 36, 20), (2, 72, 40)])
+@pytest.mark.parametrize("window", ["1", "0"])
+def test_density_plane_window(monkeypatch, n_rep, n_grid, nb, window):
+    """The zero-window density planes (the default on these grids) and the
+    general ones (CHS_PLANE_WINDOW=0) both match the oracle; chunks of 8
+    bands with unoccupied bands at the end."""
+    monkeypatch.setenv("CHS_PLANE_WINDOW", window)
+    c = chs.Context(0)
+    p = O.default_params(n_rep=n_rep, n_grid=n_grid, n_bands=nb)
+    m = O.Model(p, O.positions(n_rep))
+    mil, idx, kin = m.pw()
+    H = chs.Hamiltonian(c, n_grid, mil, kin, m.vion())
+    psi = m.initial_psi(nb)
+    occ = np.linspace(2.0, 0.1, nb)
+    occ[-5:] = 0.0
+    ref = m.density(psi, occ)
+    omega = (10.26 * n_rep) ** 3
+    got = chs.density(c, chs.Gpu.from_host(c, psi), occ, omega, n_grid).sync_to_host()
+    assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max() + 1e-16
+    del H
+
+
 def test_density_single_plane_wave(ctx):
     """One plane wave => |psi(r)|^2 is constant = 1/Omega."""
     p = O.default_params(n_rep=1, n_grid=36, n_bands=1)
