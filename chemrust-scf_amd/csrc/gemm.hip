@@ -596,7 +596,9 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   cplx<T>* Ct = (cplx<T>*)im->gemm_tune_c.p;
   const cplx<T> one = mk((T)1, (T)0), zero = mk((T)0, (T)0);
   double best = 1e300;
-  for (int v = 0; v < 6; ++v) {
+  // the vector-FMA kernel never wins a large product: not timed there
+  const bool big = (double)m * n * k > 1e9;
+  for (int v = big ? 1 : 0; v < 6; ++v) {
     int prev_split = -1;
     for (int ti = 0; ti < 4; ++ti) {
       int bm, bn, bk, ks, kl;
