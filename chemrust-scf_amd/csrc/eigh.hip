@@ -338,11 +338,11 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
   if (i > 0 && w[i] - w[i - 1] <= ortol) return;  // not a cluster leader
   int end = i + 1;
   while (end < n && w[end] - w[end - 1] <= ortol) ++end;
-  double* mult = scr;
-  double* u0 = scr + (size_t)n * n;
-  double* u1 = scr + 2 * (size_t)n * n;
-  double* u2 = scr + 3 * (size_t)n * n;
-  double* swp = scr + 4 * (size_t)n * n;
+  double* __restrict__ mult = scr;
+  double* __restrict__ u0 = scr + (size_t)n * n;
+  double* __restrict__ u1 = scr + 2 * (size_t)n * n;
+  double* __restrict__ u2 = scr + 3 * (size_t)n * n;
+  double* __restrict__ swp = scr + 4 * (size_t)n * n;
   auto at = [n](int j, int t) { return (size_t)j * n + t; };
   const double eps_t = DBL_EPSILON * fmax(tnorm, DBL_MIN);
   const double pert = 10.0 * eps_t;
@@ -353,6 +353,7 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
     lam_prev = lam;
     // LU of T - lam I with row interchanges; U has up to two superdiagonals
     double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0, cc = 0.0;
+#pragma unroll 4
     for (int r = 0; r < n - 1; ++r) {
       const double er = e[r], an = d[r + 1] - lam, en = r + 1 < n - 1 ? e[r + 1] : 0.0;
       if (fabs(ca) >= fabs(er)) {
@@ -386,25 +387,36 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
       Z[at(j, t)] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
     }
     for (int iter = 0; iter < 3; ++iter) {
-      // forward: row operations of the factorisation
+      // forward: row operations of the factorisation; the running entry
+      // stays in a register (a store and reload per row would put a memory
+      // round trip on the recurrence)
+      double yc = Z[at(0, t)];
+#pragma unroll 8
       for (int r = 0; r < n - 1; ++r) {
-        double yr = Z[at(r, t)], yn = Z[at(r + 1, t)];
+        double yr = yc, yn = Z[at(r + 1, t)];
         if (swp[at(r, i)] != 0.0) {
           const double tmp = yr;
           yr = yn;
           yn = tmp;
         }
         Z[at(r, t)] = yr;
-        Z[at(r + 1, t)] = yn - mult[at(r, i)] * yr;
+        yc = yn - mult[at(r, i)] * yr;
       }
-      // back substitution with U (tiny pivots replaced by eps ||T||)
+      Z[at(n - 1, t)] = yc;
+      // back substitution with U (tiny pivots replaced by eps ||T||), the two
+      // previous solution entries in registers
+      double x1 = 0.0, x2 = 0.0;
+#pragma unroll 8
       for (int r = n - 1; r >= 0; --r) {
         double s = Z[at(r, t)];
-        if (r + 1 < n) s -= u1[at(r, i)] * Z[at(r + 1, t)];
-        if (r + 2 < n) s -= u2[at(r, i)] * Z[at(r + 2, t)];
+        if (r + 1 < n) s -= u1[at(r, i)] * x1;
+        if (r + 2 < n) s -= u2[at(r, i)] * x2;
         double piv = u0[at(r, i)];
         if (fabs(piv) < eps_t) piv = piv < 0.0 ? -eps_t : eps_t;
-        Z[at(r, t)] = s / piv;
+        const double x = s / piv;
+        Z[at(r, t)] = x;
+        x2 = x1;
+        x1 = x;
       }
       // orthogonalise against the cluster's earlier vectors, normalise
       for (int q = i; q < t; ++q) {
@@ -413,8 +425,10 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
         for (int j = 0; j < n; ++j) Z[at(j, t)] -= dot * Z[at(j, q)];
       }
       double nrm = 0.0;
+#pragma unroll 8
       for (int j = 0; j < n; ++j) nrm += Z[at(j, t)] * Z[at(j, t)];
       const double inv = 1.0 / sqrt(nrm);
+#pragma unroll 8
       for (int j = 0; j < n; ++j) Z[at(j, t)] *= inv;
     }
   }
