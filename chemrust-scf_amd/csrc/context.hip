@@ -112,7 +112,7 @@ int chs_context_destroy(chs_context* ctx) {
   auto* im = ctx->impl;
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
-                    &im->ws4, &im->ws5, &im->ws6, &im->rr32, &im->gemm_ws, &im->gemv_ws, &im->h3_bws, &im->lin_ws, &im->scf_ws,
+                    &im->ws4, &im->ws5, &im->ws6, &im->rr32, &im->lanczos_v0, &im->gemm_ws, &im->gemv_ws, &im->h3_bws, &im->lin_ws, &im->scf_ws,
                     &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->kset_ws,
                     &im->gemm_tune_ws, &im->gemm_tune_c};
   for (DevBuf* b : bufs) release(*b);

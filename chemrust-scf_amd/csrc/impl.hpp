@@ -134,6 +134,10 @@ struct chs_context::Impl {
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
   chs::DevBuf rr32;  // complex64 H|Y> and beta^H Y of Rayleigh-Ritz in the complex64 phase
+  // Lanczos start vector of the last (seed, n_pw): generated on the host once
+  chs::DevBuf lanczos_v0;
+  uint64_t lanczos_seed = 0;
+  int lanczos_n = 0;
   chs::DevBuf gemm_ws, gemv_ws, h3_bws, lin_ws, scf_ws, proj_ws, chol_ws, eig_ws, mlp_ws;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Replay launch-bound loops (Jacobi sweeps) as captured HIP graphs. Off

@@ -360,9 +360,11 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
   const int P = H.n_proj;
   const bool gen = H.has_s;
   if (k < 1 || k > 16) return set_error(ctx, CHS_ERR_INVALID_ARG, "Lanczos steps must be 1..16");
-  // start vector from the LCG (host, once per call; n_pw complex)
-  std::vector<double> h0(2 * (size_t)n);
-  {
+  const size_t vb = (size_t)n * sizeof(z64);
+  // start vector from the LCG (n_pw complex), generated on the host once per
+  // (seed, n_pw) and kept on the device: the SCF asks for it every iteration
+  if (!im->lanczos_v0.p || im->lanczos_seed != seed || im->lanczos_n != n) {
+    std::vector<double> h0(2 * (size_t)n);
     uint64_t s = seed;
     auto uni = [&s]() {
       s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
@@ -377,8 +379,16 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
       h0[2 * g] = nrm();
       h0[2 * g + 1] = nrm();
     }
+    im->lanczos_n = 0;  // invalid until the upload has landed
+    CHS_TRY(ensure(ctx, im->lanczos_v0, vb));
+    // stream-ordered after any earlier reader of the buffer; h0 must outlive it
+    CHS_HIP(ctx, hipMemcpyAsync(im->lanczos_v0.p, h0.data(), vb, hipMemcpyHostToDevice,
+                                ctx->stream));
+    CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->h2d += vb;
+    im->lanczos_seed = seed;
+    im->lanczos_n = n;
   }
-  const size_t vb = (size_t)n * sizeof(z64);
   CHS_TRY(ensure(ctx, im->lin_ws, 4 * vb + 7 * (size_t)std::max(P, 1) * sizeof(z64) +
                                       64 * sizeof(double)));
   z64* base = (z64*)im->lin_ws.p;
@@ -398,8 +408,8 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
   const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   const unsigned nblk = (unsigned)ceil_div((size_t)n + P, 256);
   int cur = 0, prv = 1, nxt = 2;
-  CHS_HIP(ctx, hipMemcpyAsync(vec[cur], h0.data(), vb, hipMemcpyHostToDevice, ctx->stream));
-  ctx->h2d += vb;
+  CHS_HIP(ctx, hipMemcpyAsync(vec[cur], im->lanczos_v0.p, vb, hipMemcpyDeviceToDevice,
+                              ctx->stream));
   CHS_HIP(ctx, hipMemsetAsync(ab, 0, 40 * sizeof(double), ctx->stream));
   // normalise: v /= sqrt(<v|v> + sum Q |w_v|^2)
   if (P > 0)

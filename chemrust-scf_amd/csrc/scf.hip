@@ -502,7 +502,12 @@ double fermi_occupations(const double* eps, int n, double n_e, double kt, double
     const double mid = 0.5 * (lo + hi);
     double s = 0;
     for (int i = 0; i < n; ++i) s += fd(eps[i], mid);
-    if (s > n_e) hi = mid; else lo = mid;
+    const double nlo = s > n_e ? lo : mid, nhi = s > n_e ? mid : hi;
+    // a step that changes neither end is a fixed point: the remaining
+    // iterations would repeat it, so stopping gives the same mu
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
   }
   const double mu = 0.5 * (lo + hi);
   double t = 0;
