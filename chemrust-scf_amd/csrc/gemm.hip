@@ -476,7 +476,9 @@ int gemm_opcode(char op) {
 // device, K is cut into slices of whole K tiles until about `target`
 // workgroups are in flight (more resident waves per CU hide the operand
 // latency of the long-K products better; the autotuner picks the target).
-static constexpr int kSplitTargets[4] = {512, 1024, 2048, 4096};
+// (8192 and 16384 were measured too: never faster on the 216-atom shapes)
+static constexpr int kNumSplitTargets = 4;
+static constexpr int kSplitTargets[kNumSplitTargets] = {512, 1024, 2048, 4096};
 
 template <typename T>
 static void tile_dims(int variant, int* bm, int* bn, int* bk) {
@@ -515,7 +517,7 @@ template <typename T>
 static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int n, int k,
                        cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
                        cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws, int tri = 0) {
-  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % 4];
+  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % kNumSplitTargets];
   int bm, bn, bk;
   tile_dims<T>(variant, &bm, &bn, &bk);
   int ksplit, klen;
@@ -565,9 +567,8 @@ static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int 
 
 // Kernel choice for a GEMM shape: variant (low 4 bits: 0 vector-FMA 64 x 64,
 // 1 MFMA 64 x 64, 2 MFMA 128-wide, 3 and 4 the same with clamped loads, 5
-// MFMA 64 x 64 x 32 with clamped loads) and
-// split-K target (bits 4-5, index into
-// kSplitTargets). Chosen once per (precision, ops, m, n, k) by timing every
+// MFMA 64 x 64 x 32 with clamped loads) and split-K target (bits 4-5, index
+// into kSplitTargets). Chosen once per (precision, ops, m, n, k) by timing every
 // distinct launch into a scratch output (HIP events, context stream);
 // outside a stream capture only, otherwise the MFMA 64 x 64 default.
 template <typename T>
@@ -600,7 +601,7 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
   const bool big = (double)m * n * k > 1e9;
   for (int v = big ? 1 : 0; v < 6; ++v) {
     int prev_split = -1;
-    for (int ti = 0; ti < 4; ++ti) {
+    for (int ti = 0; ti < kNumSplitTargets; ++ti) {
       int bm, bn, bk, ks, kl;
       tile_dims<T>(v, &bm, &bn, &bk);
       split_plan(m, n, k, bm, bn, bk, kSplitTargets[ti], &ks, &kl, tri && v != 0);

@@ -48,7 +48,7 @@ def test_zgemm_random(ctx, opa, opb, m, n, k):
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-13
 
 
-@pytest.mark.parametrize("choice", [0, 1, 2, 3, 4, 5, 0x31, 0x35])
+@pytest.mark.parametrize("choice", [0, 1, 2, 3, 4, 5, 0x31, 0x35, 0x23])
 def test_zgemm_every_kernel_choice(monkeypatch, choice):
     """Each GEMM kernel the autotuner can pick (variant in the low bits, split-K
     target index in bits 4-5), pinned with CHS_GEMM_CHOICE, on ragged shapes
