@@ -56,6 +56,7 @@ WORKLOAD = {
          "Chebyshev degree 12, FP64 Cholesky-QR/Rayleigh-Ritz",
 }
 PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second (MI355X)
 
 
 def roofline(chs, ctx, cfg, precision, tune=False):
@@ -97,12 +98,20 @@ def roofline(chs, ctx, cfg, precision, tune=False):
         achieved, peak, unit = dom["work"] / (dom["ms"] * 1e-3) / 1e12, PEAK_TFLOPS[precision], \
             "TFLOP/s"
     traffic = None
+    valu = {}
     tpath = os.path.join(ROOT, "profiles", f"{dom['kernel']}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
         if t.get("n_grid") == n and t.get("n_bands") == nb and t.get("precision") == precision:
             traffic = t.get("bytes_per_launch")
+            if t.get("valu_wave_insts_per_launch"):
+                # the plane kernel's own limit: VALU issue, one wave64 instruction
+                # per 4 cycles per SIMD (16 lanes, no packed FP32), 1,024 SIMDs
+                # at 2.4 GHz
+                rate = t["valu_wave_insts_per_launch"] / (dom["ms"] * 1e-3)
+                valu = {"valu_wave_insts_per_launch": t["valu_wave_insts_per_launch"],
+                        "valu_issue_frac": round(rate / VALU_ISSUE_PEAK, 4)}
     s_b = 16 if precision == "c128" else 8
     canonical = nb * (4 * s_b * n ** 3 + 3 * s_b * npw) + 8 * n ** 3
     hpsi_ms = float(prof["ms"].sum())
@@ -113,7 +122,7 @@ def roofline(chs, ctx, cfg, precision, tune=False):
         "step_kernels_ms": {it["kernel"]: round(it["ms"], 4) for it in items},
         "hpsi_canonical_bytes": float(canonical),
         "hpsi_canonical_gbs": round(canonical / (hpsi_ms * 1e-3) / 1e9, 1),
-        "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision,
+        "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision, **valu,
         **({"hpsi_launch": dict(zip(("sticks_per_block", "row_div", "plane_threads"), tuned))}
            if tuned else {}),
     }
