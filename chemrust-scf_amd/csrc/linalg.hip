@@ -620,7 +620,7 @@ int cholqr_impl(chs_context* ctx, z64* Y, int n_b) {
   z64* Ss = (z64*)im->scf_ws.p;
   z64* Li = Ss + nn;
   const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-  CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, Y, n_pw, gen ? SY : Y, n_pw, zero, Ss, n_b));
+  CHS_TRY(zgemm_herm(ctx, n_b, n_pw, Y, n_pw, gen ? SY : Y, n_pw, Ss, n_b));
   CHS_TRY(hermitise(ctx, Ss, n_b));
   int info = 0;
   CHS_TRY(cholesky(ctx, Ss, n_b, &info));
