@@ -130,23 +130,37 @@ __global__ void __launch_bounds__(256) k_hh_symv(const z64* __restrict__ A, int 
 // w of step k - 1 for every index of its trailing matrix (t in [0, mp),
 // global index k + t) into LDS, with v of that step: w = tau p - c v,
 // p = sum of the step's partial products, c = |tau|^2 (v^H A22 v) / 2.
+// The partial products are summed before the block reduction of c so that
+// their loads are in flight together (blockDim 256, mp <= 1024).
 __device__ void hh_prev_vw(const z64* __restrict__ Vst, const z64* __restrict__ ppart,
                            const z64* __restrict__ spart, const z64* __restrict__ tau, int n,
                            int k, z64* __restrict__ vp, z64* __restrict__ wp, double* sh) {
   const int mp = n - k;  // trailing size of step k - 1
   const int nchp = (mp + kHC - 1) / kHC;
   const z64 t_p = tau[k - 1];
+  const z64* v = Vst + (size_t)(k - 1) * n + k;
+  z64 preg[4], vreg[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = threadIdx.x + 256 * q;
+    z64 p = mk(0.0, 0.0), vt = mk(0.0, 0.0);
+    if (t < mp) {
+      for (int b = 0; b < nchp; ++b) p = p + ppart[(size_t)b * n + t];
+      vt = v[t];
+    }
+    preg[q] = t_p * p;
+    vreg[q] = vt;
+  }
   double s = 0.0;
   for (int t = threadIdx.x; t < nchp * nchp; t += blockDim.x) s += spart[t].x;
   const double c = 0.5 * norm2(t_p) * block_sum_all(s, sh);
-  const z64* v = Vst + (size_t)(k - 1) * n + k;
-  for (int t = threadIdx.x; t < mp; t += blockDim.x) {
-    z64 p = mk(0.0, 0.0);
-    for (int b = 0; b < nchp; ++b) p = p + ppart[(size_t)b * n + t];
-    p = t_p * p;
-    const z64 vt = v[t];
-    vp[t] = vt;
-    wp[t] = mk(p.x - c * vt.x, p.y - c * vt.y);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int t = threadIdx.x + 256 * q;
+    if (t < mp) {
+      vp[t] = vreg[q];
+      wp[t] = mk(preg[q].x - c * vreg[q].x, preg[q].y - c * vreg[q].y);
+    }
   }
   __syncthreads();
 }
@@ -173,12 +187,36 @@ __global__ void __launch_bounds__(256) k_hh_step(z64* __restrict__ A, int n, int
   const int m = n - k - 1;
   const int nch = (m + kHC - 1) / kHC;
   const int bi = blockIdx.x, bj = blockIdx.y;
-  hh_prev_vw(Vst, ppart_prev, spart_prev, tau, n, k, vp, wp, sh);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = bi * kHC + lane;
+  // Loads that depend on nothing computed here go first, so that their
+  // latency overlaps the reductions below: the thread's 16 elements of its
+  // tile (addresses clamped into the trailing matrix, m >= 1; nothing in
+  // this launch writes them before the update) and its entries of column k.
+  z64 areg[kHC / 4];
+  {
+    const size_t ro = (size_t)(k + 1 + min(i, m - 1));
+#pragma unroll
+    for (int jj = 0; jj < kHC / 4; ++jj) {
+      const int j = bj * kHC + w * (kHC / 4) + jj;
+      areg[jj] = A[ro + (size_t)(k + 1 + min(j, m - 1)) * n];
+    }
+  }
   // column k of the step-(k-1) trailing matrix below the diagonal (rows k+1..)
   const z64* colk = A + (size_t)(k + 1) + (size_t)k * n;
+  z64 creg[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = threadIdx.x + 256 * q;
+    creg[q] = colk[min(r, m - 1)];
+  }
+  hh_prev_vw(Vst, ppart_prev, spart_prev, tau, n, k, vp, wp, sh);
   const z64 cw0 = conj(wp[0]), cv0 = conj(vp[0]);
-  for (int r = threadIdx.x; r < m; r += blockDim.x)
-    xk[r] = colk[r] - vp[r + 1] * cw0 - wp[r + 1] * cv0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = threadIdx.x + 256 * q;
+    if (r < m) xk[r] = creg[q] - vp[r + 1] * cw0 - wp[r + 1] * cv0;
+  }
   __syncthreads();
   z64 t_k, scale;
   double beta;
@@ -201,20 +239,17 @@ __global__ void __launch_bounds__(256) k_hh_step(z64* __restrict__ A, int n, int
     vj[t] = j < m ? (j == 0 ? mk(1.0, 0.0) : xk[j] * scale) : mk(0.0, 0.0);
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = bi * kHC + lane;
   z64 acc = mk(0.0, 0.0);
   if (i < m) {
     z64* arow = A + (size_t)(k + 1 + i);
     const z64 vr = vp[i + 1], wr = wp[i + 1];
-#pragma unroll 4
+#pragma unroll
     for (int jj = 0; jj < kHC / 4; ++jj) {
       const int jl = w * (kHC / 4) + jj, j = bj * kHC + jl;
       if (j < m) {
-        z64* a = arow + (size_t)(k + 1 + j) * n;
         const z64 u = vr * conj(wp[j + 1]) + wr * conj(vp[j + 1]);
-        const z64 upd = *a - u;
-        *a = upd;
+        const z64 upd = areg[jj] - u;
+        arow[(size_t)(k + 1 + j) * n] = upd;
         acc = acc + upd * vj[jl];
       }
     }

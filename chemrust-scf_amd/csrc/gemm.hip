@@ -337,9 +337,16 @@ __device__ __forceinline__ void gemv_out(cplx<T> acc, cplx<T> alpha, cplx<T> bet
   }
 }
 
-template <typename T>
+// TA: the matrix's element type (complex64 under complex128 vectors and
+// accumulation is the mixed form: half the matrix bytes).
+template <typename T, typename TA = T>
+__device__ __forceinline__ cplx<T> widen_el(cplx<TA> a) {
+  return mk((T)a.x, (T)a.y);
+}
+
+template <typename T, typename TA = T>
 __global__ void __launch_bounds__(256) k_gemv_n(int m, int k, int klen, cplx<T> alpha,
-                                                const cplx<T>* __restrict__ A, int lda,
+                                                const cplx<TA>* __restrict__ A, int lda,
                                                 const cplx<T>* __restrict__ x, cplx<T> beta,
                                                 cplx<T>* __restrict__ y,
                                                 cplx<T>* __restrict__ part) {
@@ -351,32 +358,32 @@ __global__ void __launch_bounds__(256) k_gemv_n(int m, int k, int klen, cplx<T> 
   for (; kk + 4 <= ke; kk += 4) {
     cplx<T> a[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = A[(size_t)i + (size_t)(kk + u) * lda];
+    for (int u = 0; u < 4; ++u) a[u] = widen_el<T>(A[(size_t)i + (size_t)(kk + u) * lda]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) cmac(a[u], x[kk + u], re[u], im[u]);
   }
-  for (; kk < ke; ++kk) cmac(A[(size_t)i + (size_t)kk * lda], x[kk], re[0], im[0]);
+  for (; kk < ke; ++kk) cmac(widen_el<T>(A[(size_t)i + (size_t)kk * lda]), x[kk], re[0], im[0]);
   gemv_out(mk((re[0] + re[1]) + (re[2] + re[3]), (im[0] + im[1]) + (im[2] + im[3])), alpha, beta,
            y, part, m, i);
 }
 
-template <typename T, int OPA>
+template <typename T, int OPA, typename TA = T>
 __global__ void __launch_bounds__(256) k_gemv_t(int m, int k, int klen, cplx<T> alpha,
-                                                const cplx<T>* __restrict__ A, int lda,
+                                                const cplx<TA>* __restrict__ A, int lda,
                                                 const cplx<T>* __restrict__ x, cplx<T> beta,
                                                 cplx<T>* __restrict__ y,
                                                 cplx<T>* __restrict__ part) {
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (i >= m) return;
   const int kb = blockIdx.y * klen, ke = min(k, kb + klen);
-  const cplx<T>* col = A + (size_t)i * lda;
+  const cplx<TA>* col = A + (size_t)i * lda;
   T re[4] = {0, 0, 0, 0}, im[4] = {0, 0, 0, 0};
   int kk = kb + lane;
   for (; kk + 192 < ke; kk += 256) {
     cplx<T> a[4], b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[u] = col[kk + 64 * u];
+      a[u] = widen_el<T>(col[kk + 64 * u]);
       b[u] = x[kk + 64 * u];
     }
 #pragma unroll
@@ -386,7 +393,7 @@ __global__ void __launch_bounds__(256) k_gemv_t(int m, int k, int klen, cplx<T> 
     }
   }
   for (; kk < ke; kk += 64) {
-    cplx<T> a = col[kk];
+    cplx<T> a = widen_el<T>(col[kk]);
     if (OPA == 2) a.y = -a.y;
     cmac(a, x[kk], re[0], im[0]);
   }
@@ -419,8 +426,8 @@ static void gemv_plan(int k, int waves, int min_len, int* ns, int* klen) {
   *ns = (k + *klen - 1) / *klen;
 }
 
-template <typename T>
-static int gemv_launch(chs_context* ctx, int oa, int m, int k, cplx<T> alpha, const cplx<T>* A,
+template <typename T, typename TA = T>
+static int gemv_launch(chs_context* ctx, int oa, int m, int k, cplx<T> alpha, const cplx<TA>* A,
                        int lda, const cplx<T>* x, cplx<T> beta, cplx<T>* y) {
   int ns, klen;
   if (oa == 0)
@@ -433,13 +440,13 @@ static int gemv_launch(chs_context* ctx, int oa, int m, int k, cplx<T> alpha, co
     part = (cplx<T>*)ctx->impl->gemv_ws.p;
   }
   if (oa == 0)
-    hipLaunchKernelGGL((k_gemv_n<T>), dim3(ceil_div(m, 256), ns), dim3(256), 0, ctx->stream, m,
+    hipLaunchKernelGGL((k_gemv_n<T, TA>), dim3(ceil_div(m, 256), ns), dim3(256), 0, ctx->stream, m,
                        k, klen, alpha, A, lda, x, beta, y, part);
   else if (oa == 1)
-    hipLaunchKernelGGL((k_gemv_t<T, 1>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
+    hipLaunchKernelGGL((k_gemv_t<T, 1, TA>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
                        k, klen, alpha, A, lda, x, beta, y, part);
   else
-    hipLaunchKernelGGL((k_gemv_t<T, 2>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
+    hipLaunchKernelGGL((k_gemv_t<T, 2, TA>), dim3(ceil_div(m, 4), ns), dim3(256), 0, ctx->stream, m,
                        k, klen, alpha, A, lda, x, beta, y, part);
   CHS_LAUNCH_CHECK(ctx);
   if (ns > 1) {
@@ -677,6 +684,14 @@ int zgemm_herm(chs_context* ctx, int n, int k, const z64* A, int lda, const z64*
 int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,
           int lda, const c32* B, int ldb, c32 beta, c32* C, int ldc) {
   return gemm_t<float>(ctx, opa, opb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+}
+
+int zgemv_c32(chs_context* ctx, char opa, int m, int k, z64 alpha, const c32* A, int lda,
+              const z64* x, z64 beta, z64* y) {
+  const int oa = gemm_opcode(opa);
+  if (oa < 0 || m < 0 || k < 1) return CHS_ERR_INVALID_ARG;
+  if (m == 0) return CHS_OK;
+  return gemv_launch<double, float>(ctx, oa, m, k, alpha, A, lda, x, beta, y);
 }
 
 }  // namespace chs

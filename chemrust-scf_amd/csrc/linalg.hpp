@@ -12,6 +12,10 @@ int zgemm_herm(chs_context* ctx, int n, int k, const z64* A, int lda, const z64*
                int ldc);
 int cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, c32 alpha, const c32* A,
           int lda, const c32* B, int ldb, c32 beta, c32* C, int ldc);
+// y = alpha op(A) x + beta y (op N, T or C) with A held in complex64 and x,
+// y and the accumulation in complex128.
+int zgemv_c32(chs_context* ctx, char opa, int m, int k, z64 alpha, const c32* A, int lda,
+              const z64* x, z64 beta, z64* y);
 int gemm_opcode(char op);
 int hermitise(chs_context* ctx, z64* M, int n);
 int cholesky(chs_context* ctx, z64* A, int n, int* info_host);

@@ -351,13 +351,18 @@ static int dot_re_into(chs_context* ctx, const z64* a, const z64* b, size_t n, d
 }
 
 // Lanczos bound in the S inner product (S = I without projectors); start
-// vector complex N(0,1) per plane wave from the LCG with `seed`.
-int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min) {
+// vector complex N(0,1) per plane wave from the LCG with `seed`. beta_c64:
+// the projector products read the complex64 copy of beta (vectors and sums
+// stay complex128), for the complex64 filter phase, whose own products use
+// that copy: half the bytes of the two matrix-vector passes per step.
+int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min,
+                 bool beta_c64) {
   CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
   const int n = H.n_pw;
   const int P = H.n_proj;
+  if (beta_c64 && P > 0) CHS_TRY(ensure_precision_ready(ctx, CHS_C64));
   const bool gen = H.has_s;
   if (k < 1 || k > 16) return set_error(ctx, CHS_ERR_INVALID_ARG, "Lanczos steps must be 1..16");
   const size_t vb = (size_t)n * sizeof(z64);
@@ -401,19 +406,24 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
   z64 *g = pb + 3 * P1, *y = pb + 4 * P1, *c = pb + 5 * P1, *wu = pb + 6 * P1;
   double* ab = (double*)(pb + 7 * P1);  // alpha/beta pairs + scratch
   double* red = ab + 40;
+  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   const z64* beta = (const z64*)H.beta;
+  const c32* beta32 = beta_c64 ? (const c32*)H.beta_f.p : nullptr;
+  // op(beta) x: op C gives the coefficients beta^H x, op N adds beta c to u
+  auto bprod = [&](char op, int m, int kk, const z64* x, z64 bt, z64* yv) -> int {
+    if (beta32) return zgemv_c32(ctx, op, m, kk, one, beta32, n, x, bt, yv);
+    return zgemm(ctx, op, 'N', m, 1, kk, one, beta, n, x, kk, bt, yv, m);
+  };
   const z64* B = (const z64*)H.bmat.p;
   const z64* M = gen ? (const z64*)H.sinv_m.p : nullptr;
   const double* Q = gen ? H.qion : nullptr;
-  const z64 one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
   const unsigned nblk = (unsigned)ceil_div((size_t)n + P, 256);
   int cur = 0, prv = 1, nxt = 2;
   CHS_HIP(ctx, hipMemcpyAsync(vec[cur], im->lanczos_v0.p, vb, hipMemcpyDeviceToDevice,
                               ctx->stream));
   CHS_HIP(ctx, hipMemsetAsync(ab, 0, 40 * sizeof(double), ctx->stream));
   // normalise: v /= sqrt(<v|v> + sum Q |w_v|^2)
-  if (P > 0)
-    CHS_TRY(zgemm(ctx, 'C', 'N', P, 1, n, one, beta, n, vec[cur], n, zero, wvec[cur], P));
+  if (P > 0) CHS_TRY(bprod('C', P, n, vec[cur], zero, wvec[cur]));
   CHS_TRY(dot_re_into(ctx, vec[cur], vec[cur], n, red));
   hipLaunchKernelGGL(k_lanczos_scalar, dim3(1), dim3(256), 0, ctx->stream, P, Q, wvec[cur],
                      wvec[cur], red, ab, 16, 1);
@@ -430,7 +440,7 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
     CHS_LAUNCH_CHECK(ctx);
     if (P > 0) {
       const unsigned pblk = (unsigned)ceil_div(P, 4);
-      CHS_TRY(zgemm(ctx, 'C', 'N', P, 1, n, one, beta, n, u, n, zero, g, P));
+      CHS_TRY(bprod('C', P, n, u, zero, g));
       hipLaunchKernelGGL(k_herm_mv, dim3(pblk), dim3(256), 0, ctx->stream, P, B, wvec[cur],
                          H.dion, g, nullptr, 1.0, y);
       CHS_LAUNCH_CHECK(ctx);
@@ -440,7 +450,7 @@ int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* 
       hipLaunchKernelGGL(k_herm_mv, dim3(pblk), dim3(256), 0, ctx->stream, P, B, c, nullptr, g,
                          nullptr, 1.0, wu);
       CHS_LAUNCH_CHECK(ctx);
-      CHS_TRY(zgemm(ctx, 'N', 'N', n, 1, P, one, beta, n, c, P, one, u, n));
+      CHS_TRY(bprod('N', n, P, c, one, u));
     }
     hipLaunchKernelGGL(k_lanczos_f, dim3(nblk), dim3(256), 0, ctx->stream, u, vec[cur], vec[prv],
                        wu, wvec[cur], wvec[prv], ab, j, (size_t)n, P, vec[nxt], wvec[nxt]);
@@ -577,7 +587,7 @@ int chs_chebyshev_filter(chs_context* ctx, int precision, const void* d_psi, int
 
 int chs_lanczos_bound(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min) {
   if (!ctx || !upper) return CHS_ERR_INVALID_ARG;
-  return lanczos_impl(ctx, k, seed, upper, ritz_min);
+  return lanczos_impl(ctx, k, seed, upper, ritz_min, false);
 }
 
 int chs_grid_lambda_max(chs_context* ctx, double L, double* out) {

@@ -11,7 +11,8 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
 int filter_steps(chs_context* ctx, int precision, void* a, void* b, int n_b, int degree,
                  const double* d_coef, double* flag, void** res);
 int filter_check(chs_context* ctx, const double* flag);
-int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min);
+int lanczos_impl(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min,
+                 bool beta_c64 = false);
 // Nonlocal projector terms (projectors.hip): hpsi += beta D beta^H psi and,
 // when spsi != nullptr, spsi = psi + beta Q beta^H psi.
 int nonlocal_apply(chs_context* ctx, int precision, const void* psi, int n_b, void* hpsi,

@@ -183,7 +183,10 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
     t_veff += tb - ta;
     double lmax;
     if (p->lanczos_k > 0) {
-      CHS_TRY(lanczos_impl(ctx, p->lanczos_k, p->seed_lanczos, &lmax, nullptr));
+      // complex64 filter phase: the bound's projector products on the
+      // complex64 beta the filter itself uses
+      CHS_TRY(lanczos_impl(ctx, p->lanczos_k, p->seed_lanczos, &lmax, nullptr,
+                           mixed && !fp64_phase));
     } else {
       CHS_TRY(chs_grid_lambda_max(ctx, L, &lmax));
     }
