@@ -492,14 +492,21 @@ __global__ void __launch_bounds__(256) k_wy_tmat(const z64* __restrict__ Gm, int
     Ts[a][c] = mk(0.0, 0.0);
   }
   __syncthreads();
+  // row a of column c by the 8 lanes a * 8 + z (z: every eighth q), summed
+  // with shuffles inside the lane group (256 threads = kWY rows x 8)
+  static_assert(kWY * 8 == 256, "one lane group of 8 per row");
+  const int a = threadIdx.x >> 3, z = threadIdx.x & 7;
   for (int c = 0; c < b; ++c) {
     const z64 tc = tau[k0 + c];
-    if (threadIdx.x < c) {
-      const int a = threadIdx.x;
-      z64 s = mk(0.0, 0.0);
-      for (int q = a; q < c; ++q) s = s + Ts[a][q] * G[q][c];
-      Ts[a][c] = mk(-(tc.x * s.x - tc.y * s.y), -(tc.x * s.y + tc.y * s.x));
+    z64 s = mk(0.0, 0.0);
+    if (a < c)
+      for (int q = a + z; q < c; q += 8) s = s + Ts[a][q] * G[q][c];
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+      s.x += __shfl_xor(s.x, off, 8);
+      s.y += __shfl_xor(s.y, off, 8);
     }
+    if (a < c && z == 0) Ts[a][c] = mk(-(tc.x * s.x - tc.y * s.y), -(tc.x * s.y + tc.y * s.x));
     if (threadIdx.x == 0) Ts[c][c] = tc;
     __syncthreads();
   }

@@ -408,27 +408,69 @@ __global__ void k_add_real(const z64* __restrict__ g, double* __restrict__ r, si
   if (i < n) r[i] += g[i].x;
 }
 
-// part[a * nchunk + c] = sum over the chunk of Re(V(G) e^{+iG.R_a}) exp(-G^2 rq^2/4)
+// part[a * nchunk + c] = sum over chunk c of Re(V(G) e^{+iG.R_a}) exp(-G^2 rq^2/4)
+// with the Nyquist planes of an even grid left out. Both factors separate
+// over the axes (G^2 = b^2 (h^2 + k^2 + l^2), e^{iG.R} = px py pz), so a
+// block tables conj(px) gx, py and pz for kScreenAtoms atoms in LDS (gx with
+// its Nyquist entry zero drops those planes) and reads each V(G) once for
+// all of them. A chunk is the (iz, iy) rows c, c + nchunk, ... of n points.
+constexpr int kScreenAtoms = 8;
+
 __global__ void __launch_bounds__(256) k_screen(const z64* __restrict__ vg, int n, int n_atoms,
                                                 const z64* __restrict__ ph, double b, double rq,
                                                 double* __restrict__ part) {
+  extern __shared__ z64 tab[];  // cx, py, pz: [kScreenAtoms][n] each, then gauss[n]
   __shared__ double sh[8];
-  const int a = blockIdx.y, c = blockIdx.x, nchunk = gridDim.x;
-  const size_t n3 = (size_t)n * n * n;
-  const z64* pa = ph + (size_t)a * 3 * n;
-  double s = 0;
-  for (size_t id = (size_t)c * blockDim.x + threadIdx.x; id < n3; id += (size_t)nchunk * blockDim.x) {
-    const int ix = (int)(id % n), iy = (int)((id / n) % n), iz = (int)(id / ((size_t)n * n));
-    if ((n % 2 == 0) && (ix == n / 2 || iy == n / 2 || iz == n / 2)) continue;
-    const int h = ix <= n / 2 ? ix : ix - n, k = iy <= n / 2 ? iy : iy - n,
-              l = iz <= n / 2 ? iz : iz - n;
-    const double g2 = b * b * (double)(h * h + k * k + l * l);
-    const z64 e = conj(pa[ix] * pa[n + iy] * pa[2 * n + iz]);
-    const z64 v = vg[id];
-    s += (v.x * e.x - v.y * e.y) * exp(-g2 * rq * rq / 4.0);
+  z64* cx = tab;
+  z64* py = tab + kScreenAtoms * n;
+  z64* pz = tab + 2 * kScreenAtoms * n;
+  double* gs = (double*)(tab + 3 * kScreenAtoms * n);
+  const int c = blockIdx.x, nchunk = gridDim.x, a0 = blockIdx.y * kScreenAtoms;
+  const int na = min(kScreenAtoms, n_atoms - a0);
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int h = t <= n / 2 ? t : t - n;
+    gs[t] = (n % 2 == 0 && t == n / 2) ? 0.0 : exp(-b * b * (double)(h * h) * rq * rq / 4.0);
   }
-  const double r = block_sum(s, sh);
-  if (threadIdx.x == 0) part[(size_t)a * nchunk + c] = r;
+  __syncthreads();
+  for (int t = threadIdx.x; t < kScreenAtoms * n; t += blockDim.x) {
+    const int q = t / n, i = t % n;
+    z64 x = mk(0.0, 0.0), y = mk(0.0, 0.0), z = mk(0.0, 0.0);
+    if (q < na) {
+      const z64* pa = ph + (size_t)(a0 + q) * 3 * n;
+      x = conj(pa[i]);
+      x = mk(x.x * gs[i], x.y * gs[i]);
+      y = pa[n + i];
+      z = pa[2 * n + i];
+    }
+    cx[t] = x;
+    py[t] = y;
+    pz[t] = z;
+  }
+  __syncthreads();
+  const int rpp = blockDim.x / n;  // rows per pass (n <= blockDim.x)
+  const int ix = threadIdx.x % n, roff = threadIdx.x / n;
+  double s[kScreenAtoms];
+#pragma unroll
+  for (int q = 0; q < kScreenAtoms; ++q) s[q] = 0.0;
+  if (roff < rpp) {
+    for (int r = c * rpp + roff; r < n * n; r += nchunk * rpp) {
+      const int iz = r / n, iy = r % n;
+      const double gyz = gs[iy] * gs[iz];
+      if (gyz == 0.0) continue;
+      const z64 v = vg[(size_t)r * n + ix];
+#pragma unroll
+      for (int q = 0; q < kScreenAtoms; ++q) {
+        // Re(v conj(px py pz)) gx gy gz
+        const z64 yz = conj(py[q * n + iy] * pz[q * n + iz]);
+        const z64 t = cx[q * n + ix] * yz;
+        s[q] += gyz * (v.x * t.x - v.y * t.y);
+      }
+    }
+  }
+  for (int q = 0; q < na; ++q) {
+    const double r = block_sum(s[q], sh);
+    if (threadIdx.x == 0) part[(size_t)(a0 + q) * nchunk + c] = r;
+  }
 }
 
 __global__ void k_screen_finish(const double* __restrict__ part, int nchunk, int n_atoms, int np,
@@ -495,13 +537,16 @@ int screen_dion(chs_context* ctx, const double* veff, int n, double L, int n_ato
                 double* d_out) {
   const size_t n3 = (size_t)n * n * n;
   const int nchunk = 64;
+  if (n > 256) return set_error(ctx, CHS_ERR_INVALID_ARG, "augmentation screening limited to n <= 256");
   CHS_TRY(ensure(ctx, ctx->impl->part, sizeof(double) * ((size_t)n_atoms * nchunk + 2 * kRedBlocks)));
   hipLaunchKernelGGL(k_real_to_complex_proj, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream,
                      veff, cgrid, n3);
   CHS_LAUNCH_CHECK(ctx);
   CHS_TRY(fft3d(ctx, CHS_C128, cgrid, n, 1, -1));
-  hipLaunchKernelGGL(k_screen, dim3(nchunk, n_atoms), dim3(256), 0, ctx->stream, cgrid, n,
-                     n_atoms, d_ph, 2.0 * M_PI / L, rq, (double*)ctx->impl->part.p);
+  const size_t tab = (3 * (size_t)kScreenAtoms * n) * sizeof(z64) + n * sizeof(double);
+  hipLaunchKernelGGL(k_screen, dim3(nchunk, ceil_div(n_atoms, kScreenAtoms)), dim3(256), tab,
+                     ctx->stream, cgrid, n, n_atoms, d_ph, 2.0 * M_PI / L, rq,
+                     (double*)ctx->impl->part.p);
   CHS_LAUNCH_CHECK(ctx);
   hipLaunchKernelGGL(k_screen_finish, dim3(ceil_div(n_atoms, 256)), dim3(256), 0, ctx->stream,
                      (const double*)ctx->impl->part.p, nchunk, n_atoms, np, d_d0, q0,
