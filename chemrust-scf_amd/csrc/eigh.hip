@@ -352,13 +352,145 @@ __global__ void __launch_bounds__(64) k_tri_multisect(const double* __restrict__
   if (lane == 0) w[i] = 0.5 * (lo + hi);
 }
 
+// One pass of inverse iteration over a vector stored interleaved (element j
+// of column t at j * n + t), from one buffer into another. The passes work
+// in blocks of kIB elements: the block's operands are loaded into registers
+// first, then its results stored. (Vector stores count in the same memory
+// counter as loads here, so a store between two loads would make the second
+// load's wait cover the store too: one memory latency per element.)
+constexpr int kIB = 16;
+
+// Forward: the row operations of the factorisation of column i; the running
+// entry stays in a register.
+__device__ __forceinline__ void invit_forward(const double* __restrict__ src,
+                                              double* __restrict__ dst,
+                                              const double* __restrict__ mult,
+                                              const double* __restrict__ swp, int n, int t, int i) {
+  auto at = [n](int j, int c) { return (size_t)j * n + c; };
+  double yc = src[at(0, t)];
+  for (int r0 = 0; r0 < n - 1; r0 += kIB) {
+    double sv[kIB], mv[kIB], wv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int r = min(r0 + u, n - 2);
+      sv[u] = src[at(r + 1, t)];
+      mv[u] = mult[at(r, i)];
+      wv[u] = swp[at(r, i)];
+    }
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int r = r0 + u;
+      if (r < n - 1) {
+        double yr = yc, yn = sv[u];
+        if (wv[u] != 0.0) {
+          const double tmp = yr;
+          yr = yn;
+          yn = tmp;
+        }
+        dst[at(r, t)] = yr;
+        yc = yn - mv[u] * yr;
+      }
+    }
+  }
+  dst[at(n - 1, t)] = yc;
+}
+
+// Back substitution with U given as reciprocal pivots rinv and the two
+// superdiagonals (u1 and u2 are zero where U has no entry, so every row
+// reads the same operands), the two previous solution entries in registers.
+__device__ __forceinline__ void invit_back(const double* __restrict__ src, double* __restrict__ dst,
+                                           const double* __restrict__ rinv,
+                                           const double* __restrict__ u1,
+                                           const double* __restrict__ u2, int n, int t, int i) {
+  auto at = [n](int j, int c) { return (size_t)j * n + c; };
+  double x1 = 0.0, x2 = 0.0;
+  for (int r0 = n - 1; r0 >= 0; r0 -= kIB) {
+    double sv[kIB], av[kIB], bv[kIB], pv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int r = max(r0 - u, 0);
+      sv[u] = src[at(r, t)];
+      av[u] = u1[at(r, i)];
+      bv[u] = u2[at(r, i)];
+      pv[u] = rinv[at(r, i)];
+    }
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int r = r0 - u;
+      if (r >= 0) {
+        const double x = (sv[u] - av[u] * x1 - bv[u] * x2) * pv[u];
+        dst[at(r, t)] = x;
+        x2 = x1;
+        x1 = x;
+      }
+    }
+  }
+}
+
+// dst_t = src_t - (zq . src_t) zq (one modified Gram-Schmidt step)
+__device__ __forceinline__ void invit_mgs(const double* __restrict__ src, double* __restrict__ dst,
+                                          const double* __restrict__ Z, int n, int t, int q) {
+  auto at = [n](int j, int c) { return (size_t)j * n + c; };
+  double dot = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kIB) {
+    double sv[kIB], zv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int j = min(j0 + u, n - 1);
+      sv[u] = src[at(j, t)];
+      zv[u] = Z[at(j, q)];
+    }
+#pragma unroll
+    for (int u = 0; u < kIB; ++u)
+      if (j0 + u < n) dot += zv[u] * sv[u];
+  }
+  for (int j0 = 0; j0 < n; j0 += kIB) {
+    double sv[kIB], zv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) {
+      const int j = min(j0 + u, n - 1);
+      sv[u] = src[at(j, t)];
+      zv[u] = Z[at(j, q)];
+    }
+#pragma unroll
+    for (int u = 0; u < kIB; ++u)
+      if (j0 + u < n) dst[at(j0 + u, t)] = sv[u] - dot * zv[u];
+  }
+}
+
+// dst_t = src_t / ||src_t||
+__device__ __forceinline__ void invit_normalise(const double* __restrict__ src,
+                                                double* __restrict__ dst, int n, int t) {
+  auto at = [n](int j, int c) { return (size_t)j * n + c; };
+  double nrm = 0.0;
+  for (int j0 = 0; j0 < n; j0 += kIB) {
+    double sv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) sv[u] = src[at(min(j0 + u, n - 1), t)];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u)
+      if (j0 + u < n) nrm += sv[u] * sv[u];
+  }
+  const double inv = 1.0 / sqrt(nrm);
+  for (int j0 = 0; j0 < n; j0 += kIB) {
+    double sv[kIB];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u) sv[u] = src[at(min(j0 + u, n - 1), t)];
+#pragma unroll
+    for (int u = 0; u < kIB; ++u)
+      if (j0 + u < n) dst[at(j0 + u, t)] = sv[u] * inv;
+  }
+}
+
 // Eigenvectors of the tridiagonal by inverse iteration with partial
 // pivoting LU of (T - lambda I). Eigenvalues closer than ortol form a
 // cluster; one thread handles a whole cluster and orthogonalises each new
 // vector against the cluster's earlier ones (modified Gram-Schmidt), so no
 // thread reads another thread's vectors. Scratch and output are interleaved
 // by eigenvalue index (element j of column t at j * n + t) so that
-// neighbouring threads touch neighbouring addresses.
+// neighbouring threads touch neighbouring addresses. Iterates move between
+// the two scratch vectors xa and xb (each pass reads one and writes the
+// other); the last normalisation writes Z.
 __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
                                                   const double* __restrict__ e,
                                                   const double* __restrict__ w, int n,
@@ -373,14 +505,18 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
   if (i > 0 && w[i] - w[i - 1] <= ortol) return;  // not a cluster leader
   int end = i + 1;
   while (end < n && w[end] - w[end - 1] <= ortol) ++end;
+  const size_t nn = (size_t)n * n;
   double* __restrict__ mult = scr;
-  double* __restrict__ u0 = scr + (size_t)n * n;
-  double* __restrict__ u1 = scr + 2 * (size_t)n * n;
-  double* __restrict__ u2 = scr + 3 * (size_t)n * n;
-  double* __restrict__ swp = scr + 4 * (size_t)n * n;
+  double* __restrict__ rinv = scr + nn;  // 1 / pivot (tiny pivots replaced by eps ||T||)
+  double* __restrict__ u1 = scr + 2 * nn;
+  double* __restrict__ u2 = scr + 3 * nn;
+  double* __restrict__ swp = scr + 4 * nn;
+  double* xa = scr + 5 * nn;
+  double* xb = scr + 6 * nn;
   auto at = [n](int j, int t) { return (size_t)j * n + t; };
   const double eps_t = DBL_EPSILON * fmax(tnorm, DBL_MIN);
   const double pert = 10.0 * eps_t;
+  auto pivot = [eps_t](double p) { return fabs(p) < eps_t ? (p < 0.0 ? -eps_t : eps_t) : p; };
   double lam_prev = -INFINITY;
   for (int t = i; t < end; ++t) {
     double lam = w[t];
@@ -388,83 +524,77 @@ __global__ void __launch_bounds__(64) k_tri_invit(const double* __restrict__ d,
     lam_prev = lam;
     // LU of T - lam I with row interchanges; U has up to two superdiagonals
     double ca = d[0] - lam, cb = n > 1 ? e[0] : 0.0, cc = 0.0;
-#pragma unroll 4
-    for (int r = 0; r < n - 1; ++r) {
-      const double er = e[r], an = d[r + 1] - lam, en = r + 1 < n - 1 ? e[r + 1] : 0.0;
-      if (fabs(ca) >= fabs(er)) {
-        const double mu = ca != 0.0 ? er / ca : 0.0;
-        mult[at(r, i)] = mu;
-        swp[at(r, i)] = 0.0;
-        u0[at(r, i)] = ca;
-        u1[at(r, i)] = cb;
-        u2[at(r, i)] = cc;
-        ca = an - mu * cb;
-        cb = en - mu * cc;
-        cc = 0.0;
-      } else {
-        const double mu = ca / er;
-        mult[at(r, i)] = mu;
-        swp[at(r, i)] = 1.0;
-        u0[at(r, i)] = er;
-        u1[at(r, i)] = an;
-        u2[at(r, i)] = en;
-        const double na = cb - mu * an, nb = cc - mu * en;
-        ca = na;
-        cb = nb;
-        cc = 0.0;
+    for (int r0 = 0; r0 < n - 1; r0 += kIB) {
+      // the block's rows of T first (see kIB)
+      double ev[kIB], dv[kIB], fv[kIB];
+#pragma unroll
+      for (int u = 0; u < kIB; ++u) {
+        const int r = min(r0 + u, n - 2);
+        ev[u] = e[r];
+        dv[u] = d[r + 1];
+        fv[u] = r + 1 < n - 1 ? e[r + 1] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kIB; ++u) {
+        const int r = r0 + u;
+        if (r < n - 1) {
+          const double er = ev[u], an = dv[u] - lam, en = fv[u];
+          if (fabs(ca) >= fabs(er)) {
+            const double mu = ca != 0.0 ? er / ca : 0.0;
+            mult[at(r, i)] = mu;
+            swp[at(r, i)] = 0.0;
+            rinv[at(r, i)] = 1.0 / pivot(ca);
+            u1[at(r, i)] = cb;
+            u2[at(r, i)] = cc;
+            ca = an - mu * cb;
+            cb = en - mu * cc;
+            cc = 0.0;
+          } else {
+            const double mu = ca / er;
+            mult[at(r, i)] = mu;
+            swp[at(r, i)] = 1.0;
+            rinv[at(r, i)] = 1.0 / pivot(er);
+            u1[at(r, i)] = an;
+            u2[at(r, i)] = en;
+            const double na = cb - mu * an, nb = cc - mu * en;
+            ca = na;
+            cb = nb;
+            cc = 0.0;
+          }
+        }
       }
     }
-    u0[at(n - 1, i)] = ca;
+    rinv[at(n - 1, i)] = 1.0 / pivot(ca);
+    // entries beyond U's last columns: zero, read against zero solution entries
+    u1[at(n - 1, i)] = 0.0;
+    u2[at(n - 1, i)] = 0.0;
+    if (n > 1) u2[at(n - 2, i)] = 0.0;
     // start vector: deterministic pseudo-random entries in [-1, 1)
     unsigned long long st = 0x9E3779B97F4A7C15ull ^ (unsigned long long)(t + 1) * 0xBF58476D1CE4E5B9ull;
     for (int j = 0; j < n; ++j) {
       st = st * 6364136223846793005ull + 1442695040888963407ull;
-      Z[at(j, t)] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+      xa[at(j, t)] = (double)(st >> 11) * (2.0 / 9007199254740992.0) - 1.0;
     }
+    double* cur = xa;  // the iterate
+    double* oth = xb;
     for (int iter = 0; iter < 3; ++iter) {
-      // forward: row operations of the factorisation; the running entry
-      // stays in a register (a store and reload per row would put a memory
-      // round trip on the recurrence)
-      double yc = Z[at(0, t)];
-#pragma unroll 8
-      for (int r = 0; r < n - 1; ++r) {
-        double yr = yc, yn = Z[at(r + 1, t)];
-        if (swp[at(r, i)] != 0.0) {
-          const double tmp = yr;
-          yr = yn;
-          yn = tmp;
-        }
-        Z[at(r, t)] = yr;
-        yc = yn - mult[at(r, i)] * yr;
-      }
-      Z[at(n - 1, t)] = yc;
-      // back substitution with U (tiny pivots replaced by eps ||T||), the two
-      // previous solution entries in registers
-      double x1 = 0.0, x2 = 0.0;
-#pragma unroll 8
-      for (int r = n - 1; r >= 0; --r) {
-        double s = Z[at(r, t)];
-        if (r + 1 < n) s -= u1[at(r, i)] * x1;
-        if (r + 2 < n) s -= u2[at(r, i)] * x2;
-        double piv = u0[at(r, i)];
-        if (fabs(piv) < eps_t) piv = piv < 0.0 ? -eps_t : eps_t;
-        const double x = s / piv;
-        Z[at(r, t)] = x;
-        x2 = x1;
-        x1 = x;
-      }
+      invit_forward(cur, oth, mult, swp, n, t, i);
+      invit_back(oth, cur, rinv, u1, u2, n, t, i);
       // orthogonalise against the cluster's earlier vectors, normalise
       for (int q = i; q < t; ++q) {
-        double dot = 0.0;
-        for (int j = 0; j < n; ++j) dot += Z[at(j, q)] * Z[at(j, t)];
-        for (int j = 0; j < n; ++j) Z[at(j, t)] -= dot * Z[at(j, q)];
+        invit_mgs(cur, oth, Z, n, t, q);
+        double* tmp = cur;
+        cur = oth;
+        oth = tmp;
       }
-      double nrm = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < n; ++j) nrm += Z[at(j, t)] * Z[at(j, t)];
-      const double inv = 1.0 / sqrt(nrm);
-#pragma unroll 8
-      for (int j = 0; j < n; ++j) Z[at(j, t)] *= inv;
+      if (iter == 2) {
+        invit_normalise(cur, Z, n, t);
+      } else {
+        invit_normalise(cur, oth, n, t);
+        double* tmp = cur;
+        cur = oth;
+        oth = tmp;
+      }
     }
   }
 }
@@ -523,7 +653,7 @@ static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
   const int nch = (n + kHC - 1) / kHC;
   const size_t zb = (nn + 2 * (size_t)nch * n + 2 * (size_t)nch * nch + n + kWY * kWY +
                      2 * (size_t)kWY * n) * sizeof(z64);
-  const size_t db = (3 * (size_t)n + 6 * nn) * sizeof(double);
+  const size_t db = (3 * (size_t)n + 8 * nn) * sizeof(double);
   CHS_TRY(ensure(ctx, ctx->impl->eig_ws, zb + db + 1024));
   char* p = (char*)ctx->impl->eig_ws.p;
   auto take_z = [&p](size_t cnt) {
@@ -546,7 +676,7 @@ static int eigh_ws(chs_context* ctx, int n, EighWs* ws) {
   ws->dd = take_d(n);
   ws->ee = take_d(n);
   ws->e2 = take_d(n);
-  ws->scr = take_d(5 * nn);
+  ws->scr = take_d(7 * nn);  // LU factors (5 n x n) and two iterate vectors
   ws->zt = take_d(nn);
   return CHS_OK;
 }
