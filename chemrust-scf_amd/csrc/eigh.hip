@@ -271,6 +271,84 @@ __global__ void __launch_bounds__(256) k_hh_step(z64* __restrict__ A, int n, int
   }
 }
 
+// The last steps of the reduction in one workgroup (256 threads): from step
+// k0 on, the trailing matrix [k0, n)^2 (at most kHT x kHT) stays in LDS
+// (column-major, As[col][row] relative to k0), so each step's product,
+// reflector and rank-2 update are separated by barriers instead of kernel
+// boundaries. At entry the pending update of step k0 - 1 is applied while the
+// matrix is loaded; at exit d_{n-1} is written (the job of k_hh_last).
+// Same formulas as k_hh_step: p = A22 v, s = v^H p, w = tau p - (|tau|^2 s / 2) v,
+// A22 -= v w^H + w v^H.
+constexpr int kHT = 64;
+
+__global__ void __launch_bounds__(256) k_hh_tail(const z64* __restrict__ A, int n, int k0,
+                                                 z64* __restrict__ Vst,
+                                                 const z64* __restrict__ ppart_prev,
+                                                 const z64* __restrict__ spart_prev,
+                                                 z64* __restrict__ tau, double* __restrict__ dd,
+                                                 double* __restrict__ ee) {
+  __shared__ z64 As[kHT][kHT];
+  __shared__ z64 vp[kHT], wp[kHT], pp[4][kHT];
+  __shared__ double sh[9];
+  const int mp = n - k0;  // <= kHT
+  hh_prev_vw(Vst, ppart_prev, spart_prev, tau, n, k0, vp, wp, sh);
+  for (int e = threadIdx.x; e < kHT * kHT; e += blockDim.x) {
+    const int r = e % kHT, c = e / kHT;
+    if (r < mp && c < mp)
+      As[c][r] = A[(size_t)(k0 + r) + (size_t)(k0 + c) * n] - vp[r] * conj(wp[c]) -
+                 wp[r] * conj(vp[c]);
+  }
+  __syncthreads();
+  const int i4 = threadIdx.x & (kHT - 1), q4 = threadIdx.x >> 6;
+  for (int k = k0; k + 1 < n; ++k) {
+    const int l = k - k0, m = n - k - 1;
+    const z64* x = &As[l][l + 1];  // column k below the diagonal
+    z64 t_k, scale;
+    double beta;
+    hh_reflector(x, m, sh, &t_k, &beta, &scale);
+    if (threadIdx.x == 0) {
+      tau[k] = t_k;
+      ee[k] = beta;
+      dd[k] = As[l][l].x;
+    }
+    z64* vcol = Vst + (size_t)k * n;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+      const int j = r - k - 1;
+      vcol[r] = j < 0 ? mk(0.0, 0.0) : (j == 0 ? mk(1.0, 0.0) : x[j] * scale);
+    }
+    for (int j = threadIdx.x; j < m; j += blockDim.x) vp[j] = j == 0 ? mk(1.0, 0.0) : x[j] * scale;
+    __syncthreads();
+    // p = A22 v: row i4, every fourth column from q4, then the four partials
+    {
+      z64 a = mk(0.0, 0.0);
+      if (i4 < m)
+        for (int j = q4; j < m; j += 4) a = a + As[l + 1 + j][l + 1 + i4] * vp[j];
+      pp[q4][i4] = a;
+    }
+    __syncthreads();
+    double sv = 0.0;
+    z64 p = mk(0.0, 0.0);
+    if (threadIdx.x < m) {
+      p = (pp[0][threadIdx.x] + pp[1][threadIdx.x]) + (pp[2][threadIdx.x] + pp[3][threadIdx.x]);
+      sv = cmulc(vp[threadIdx.x], p).x;
+    }
+    const double c = 0.5 * norm2(t_k) * block_sum_all(sv, sh);
+    if (threadIdx.x < m) {
+      const z64 tp = t_k * p;
+      wp[threadIdx.x] = mk(tp.x - c * vp[threadIdx.x].x, tp.y - c * vp[threadIdx.x].y);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHT * kHT; e += blockDim.x) {
+      const int r = e % kHT, cc = e / kHT;
+      if (r < m && cc < m)
+        As[l + 1 + cc][l + 1 + r] =
+            As[l + 1 + cc][l + 1 + r] - vp[r] * conj(wp[cc]) - wp[r] * conj(vp[cc]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dd[n - 1] = As[mp - 1][mp - 1].x;
+}
+
 // d_{n-1}: A[n-1][n-1] after the update of step n - 2 (its last element).
 __global__ void __launch_bounds__(256) k_hh_last(const z64* __restrict__ A, int n,
                                                  const z64* __restrict__ Vst,
@@ -686,7 +764,15 @@ static void tridiagonalise_launches(chs_context* ctx, z64* A, int n, const EighW
   const int nch0 = (n + kHC - 1) / kHC;
   z64* pp[2] = {ws.ppart, ws.ppart + (size_t)nch0 * n};
   z64* sp[2] = {ws.spart, ws.spart + (size_t)nch0 * nch0};
+  // steps from k0 on (trailing matrix [k0, n)^2 of at most kHT^2) run in one
+  // workgroup with the matrix in LDS
+  const int k0 = n >= 3 ? std::max(1, n - kHT) : n;
   for (int k = 0; k + 1 < n; ++k) {
+    if (k == k0) {
+      hipLaunchKernelGGL(k_hh_tail, dim3(1), dim3(256), 0, ctx->stream, A, n, k0, ws.vst,
+                         pp[(k0 - 1) & 1], sp[(k0 - 1) & 1], ws.tau, ws.dd, ws.ee);
+      return;
+    }
     const int m = n - k - 1;
     const unsigned nc = ceil_div(m, kHC);
     if (k == 0)
