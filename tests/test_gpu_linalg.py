@@ -97,8 +97,10 @@ def test_zhegvd_kat_generalized(ctx):
     assert np.abs(w - [0.5, 4 / 3, 3.0, 5.0]).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [6, 33, 160, 512, 1000])
+@pytest.mark.parametrize("n", [3, 6, 33, 65, 66, 160, 512, 1000])
 def test_zhegvd_random_vs_scipy(ctx, n):
+    """Generalised problems against scipy; n = 3, 65 and 66 are the edges of
+    the reduction's single-workgroup tail (trailing matrices up to 64 x 64)."""
     import scipy.linalg as sl
     rng = np.random.default_rng(n)
     X = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
