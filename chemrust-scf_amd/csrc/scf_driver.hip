@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "blas1.hpp"
@@ -27,31 +29,61 @@ __global__ void k_fill_d(double* __restrict__ a, double v, size_t n) {
   if (i < n) a[i] = v;
 }
 
+// The reference's 48-bit LCG step s -> (a s + c) mod 2^48
+// (src/eigensolver/subspace_diag.rs:434-447) advanced by `steps` draws at
+// once: the affine map composed by repeated squaring (ring arithmetic mod 2^64
+// reduces exactly mod 2^48).
+static uint64_t lcg_jump(uint64_t s, uint64_t steps) {
+  uint64_t A = 1, C = 0;                      // accumulated map x -> A x + C
+  uint64_t a = 25214903917ull, c = 11ull;     // map of 2^j steps
+  while (steps) {
+    if (steps & 1) {
+      A = a * A;
+      C = a * C + c;
+    }
+    c = a * c + c;
+    a = a * a;
+    steps >>= 1;
+  }
+  return (A * s + C) & 0xFFFFFFFFFFFFull;
+}
+
 // Seeded start psi_{b,g} = (x + i y)/(1 + |k+G|^2), x, y ~ N(0,1) from the
 // reference's LCG (src/eigensolver/subspace_diag.rs:434-447) with
-// Box-Muller, drawn band-major then plane-wave order.
+// Box-Muller, drawn band-major then plane-wave order. Every element takes
+// four draws, so host threads start their ranges at draw 4 e (lcg_jump) and
+// produce the sequential stream exactly.
+// psi: 2 n_b n_pw doubles, left uninitialised by the caller, so that the
+// pages are first touched by the threads that fill them.
 static void initial_psi_host(const std::vector<double>& kin, int n_b, uint64_t seed,
-                             std::vector<double>& psi) {
+                             double* psi) {
   const size_t n_pw = kin.size();
-  psi.resize(2 * (size_t)n_b * n_pw);
-  uint64_t s = seed;
-  auto uni = [&s]() {
-    s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
-    return (double)s / 281474976710656.0;
-  };
-  auto nrm = [&]() {
-    const double u1 = 1.0 - uni();
-    const double u2 = uni();
-    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
-  };
-  for (int b = 0; b < n_b; ++b)
-    for (size_t g = 0; g < n_pw; ++g) {
+  const size_t ne = (size_t)n_b * n_pw;
+  auto fill = [&](size_t e0, size_t e1) {
+    uint64_t s = lcg_jump(seed, 4 * (uint64_t)e0);
+    auto uni = [&s]() {
+      s = (s * 25214903917ull + 11ull) & 0xFFFFFFFFFFFFull;
+      return (double)s / 281474976710656.0;
+    };
+    auto nrm = [&]() {
+      const double u1 = 1.0 - uni();
+      const double u2 = uni();
+      return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+    };
+    for (size_t e = e0; e < e1; ++e) {
       const double x = nrm();
       const double y = nrm();
-      const double d = 1.0 + 2.0 * kin[g];
-      psi[2 * ((size_t)b * n_pw + g)] = x / d;
-      psi[2 * ((size_t)b * n_pw + g) + 1] = y / d;
+      const double d = 1.0 + 2.0 * kin[e % n_pw];
+      psi[2 * e] = x / d;
+      psi[2 * e + 1] = y / d;
     }
+  };
+  const size_t nt = std::max<size_t>(1, std::min<size_t>({16, std::thread::hardware_concurrency(),
+                                                            ne / 65536 + 1}));
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; ++t) pool.emplace_back(fill, ne * t / nt, ne * (t + 1) / nt);
+  fill(0, ne / nt);
+  for (auto& th : pool) th.join();
 }
 
 struct ScfState {
@@ -107,13 +139,14 @@ int scf_run_impl(chs_context* ctx, const chs_scf_params* p, const double* pos, c
   CHS_TRY(G.set_atoms(pos, n_atoms));
   CHS_TRY(G.vion(p->zval, p->rc_loc, (double*)S.vion.p));
   // start wavefunctions (host generator, one upload)
-  std::vector<double> kin(n_pw), h_psi;
+  std::vector<double> kin(n_pw);
   CHS_HIP(ctx, hipMemcpy(kin.data(), S.kin.p, n_pw * sizeof(double), hipMemcpyDeviceToHost));
   ctx->d2h += n_pw * sizeof(double);
-  initial_psi_host(kin, n_b, p->seed_psi, h_psi);
   const size_t ne = (size_t)n_b * n_pw;
+  std::unique_ptr<double[]> h_psi(new double[2 * ne]);
+  initial_psi_host(kin, n_b, p->seed_psi, h_psi.get());
   CHS_TRY(ensure(ctx, S.psi, ne * sizeof(z64)));
-  CHS_HIP(ctx, hipMemcpy(S.psi.p, h_psi.data(), ne * sizeof(z64), hipMemcpyHostToDevice));
+  CHS_HIP(ctx, hipMemcpy(S.psi.p, h_psi.get(), ne * sizeof(z64), hipMemcpyHostToDevice));
   ctx->h2d += ne * sizeof(z64);
   const bool mixed = p->precision == CHS_C64;
   if (mixed) CHS_TRY(ensure(ctx, S.psi32, ne * sizeof(c32)));
