@@ -296,6 +296,74 @@ int orc_scf_run(const orc_params* p, const double* pos, int* n_iter, int* conver
   return 0;
 }
 
+int orc_scf_run_ex(const orc_params* p, const double* pos, int* n_iter, int* converged,
+                   double* energies, double* drho, double* eps, double* rho, double* psi,
+                   double* occ, double* mu) {
+  ScfParams s = to_params(p);
+  ScfResult r;
+  int rc = run_scf(s, pos, r, nullptr);
+  if (rc) return rc;
+  *n_iter = r.n_iter;
+  *converged = r.converged;
+  std::copy(r.energies.begin(), r.energies.end(), energies);
+  std::copy(r.drho.begin(), r.drho.end(), drho);
+  std::copy(r.eps.begin(), r.eps.end(), eps);
+  if (rho) std::copy(r.rho.begin(), r.rho.end(), rho);
+  if (psi) std::copy(r.psi.begin(), r.psi.end(), C(psi));
+  if (occ) std::copy(r.occ.begin(), r.occ.end(), occ);
+  if (mu) *mu = r.mu;
+  return 0;
+}
+
+int orc_scf_run_k(const orc_params* p, const double* pos, int n_k, const double* kfrac,
+                  const double* wk, int* n_iter, int* converged, double* energies, double* drho,
+                  double* eps, double* rho, int* n_pw, double* mu) {
+  ScfParams s = to_params(p);
+  ScfResultK r;
+  int rc = run_scf_k(s, pos, n_k, kfrac, wk, r);
+  if (rc) return rc;
+  *n_iter = r.n_iter;
+  *converged = r.converged;
+  std::copy(r.energies.begin(), r.energies.end(), energies);
+  std::copy(r.drho.begin(), r.drho.end(), drho);
+  std::copy(r.eps.begin(), r.eps.end(), eps);
+  if (rho) std::copy(r.rho.begin(), r.rho.end(), rho);
+  std::copy(r.n_pw.begin(), r.n_pw.end(), n_pw);
+  *mu = r.mu;
+  return 0;
+}
+
+double orc_fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                               double* occ, double* ts) {
+  return fermi_occupations_w(eps, w, n, n_e, kt, occ, ts);
+}
+
+struct OrcPulay {
+  System sys;
+  Pulay mixer;
+};
+
+void* orc_pulay_create(int n_grid, double L, int depth, double alpha, double kerker_q) {
+  auto* h = new OrcPulay;
+  h->sys.n_grid = n_grid;
+  h->sys.L = L;
+  h->sys.omega = L * L * L;
+  h->mixer.depth = depth;
+  h->mixer.alpha = alpha;
+  h->mixer.kerker_q = kerker_q;
+  return h;
+}
+
+void orc_pulay_mix(void* h, const double* rin, const double* rout, double* rnext) {
+  auto* p = static_cast<OrcPulay*>(h);
+  const size_t n3 = (size_t)p->sys.n_grid * p->sys.n_grid * p->sys.n_grid;
+  std::vector<double> a(rin, rin + n3), b(rout, rout + n3), c;
+  p->mixer.mix(p->sys, a, b, c);
+  std::copy(c.begin(), c.end(), rnext);
+}
+
+void orc_pulay_destroy(void* h) { delete static_cast<OrcPulay*>(h); }
+
 void orc_line_search(double a, double hd, double dd, double sd, double* out3, int* status) {
   LineSearchResult r = line_search_2d_quadratic(a, hd, dd, sd);
   out3[0] = r.step_size;

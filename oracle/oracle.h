@@ -75,6 +75,23 @@ int orc_gen_eigh(const double* h, const double* s, int n, double* eps, double* v
 int orc_scf_run(const orc_params* p, const double* pos, int* n_iter, int* converged,
                 double* energies, double* drho, double* eps, double* rho);
 
+/* the same with the final block (psi, n_bands x n_pw complex), occupations
+ * (n_bands) and chemical potential (each may be NULL) */
+int orc_scf_run_ex(const orc_params* p, const double* pos, int* n_iter, int* converged,
+                   double* energies, double* drho, double* eps, double* rho, double* psi,
+                   double* occ, double* mu);
+/* k-point SCF over n_k k-points (kfrac 3 n_k, weights wk summing to 1):
+ * eps[n_k * n_bands] k-major, n_pw[n_k] */
+int orc_scf_run_k(const orc_params* p, const double* pos, int n_k, const double* kfrac,
+                  const double* wk, int* n_iter, int* converged, double* energies, double* drho,
+                  double* eps, double* rho, int* n_pw, double* mu);
+double orc_fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                               double* occ, double* ts);
+/* Pulay/Kerker density mixer on a cubic n^3 grid of edge L */
+void* orc_pulay_create(int n_grid, double L, int depth, double alpha, double kerker_q);
+void orc_pulay_mix(void* h, const double* rin, const double* rout, double* rnext);
+void orc_pulay_destroy(void* h);
+
 /* band-CG stack (src/eigensolver/{line_search,cg_helpers,uspp_preconditioner,band_cg}.rs) */
 void orc_line_search(double a, double hd, double dd, double sd, double* out3, int* status);
 int orc_compute_residual(const double* psi, const double* hpsi, const double* spsi, int n,

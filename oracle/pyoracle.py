@@ -81,6 +81,17 @@ def lib():
             "orc_gen_eigh": (C.c_int, [c128p, c128p, C.c_int, f64p, c128p]),
             "orc_scf_run": (C.c_int, [C.POINTER(Params), f64p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                       f64p, f64p, f64p, vp]),
+            "orc_scf_run_ex": (C.c_int, [C.POINTER(Params), f64p, C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), f64p, f64p, f64p, vp, vp, vp,
+                                         C.POINTER(C.c_double)]),
+            "orc_scf_run_k": (C.c_int, [C.POINTER(Params), f64p, C.c_int, f64p, f64p,
+                                        C.POINTER(C.c_int), C.POINTER(C.c_int), f64p, f64p, f64p,
+                                        vp, i32p, C.POINTER(C.c_double)]),
+            "orc_fermi_occupations_w": (C.c_double, [f64p, f64p, C.c_int, C.c_double, C.c_double,
+                                                     f64p, C.POINTER(C.c_double)]),
+            "orc_pulay_create": (vp, [C.c_int, C.c_double, C.c_int, C.c_double, C.c_double]),
+            "orc_pulay_mix": (None, [vp, f64p, f64p, f64p]),
+            "orc_pulay_destroy": (None, [vp]),
             "orc_line_search": (None, [C.c_double] * 4 + [f64p, C.POINTER(C.c_int)]),
             "orc_compute_residual": (C.c_int, [c128p, c128p, c128p, C.c_int, c128p, C.POINTER(C.c_double)]),
             "orc_residual_bare_norm": (C.c_double, [c128p, C.c_int]),
@@ -277,3 +288,72 @@ def scf_run(params, pos, with_rho=False):
     n = n_iter.value
     return dict(n_iter=n, converged=bool(conv.value), energies=energies[:n], drho=drho[:n],
                 eps=eps, rho=rho)
+
+
+def scf_run_ex(params, pos):
+    """Full SCF with the final block, occupations and chemical potential."""
+    n_iter, conv, mu = C.c_int(), C.c_int(), C.c_double()
+    m = Model(params, pos)
+    energies = np.zeros(params.max_iter)
+    drho = np.zeros(params.max_iter)
+    eps = np.zeros(params.n_bands)
+    rho = np.zeros(params.n_grid ** 3)
+    psi = np.zeros((params.n_bands, m.n_pw), np.complex128)
+    occ = np.zeros(params.n_bands)
+    rc = lib().orc_scf_run_ex(C.byref(params), np.ascontiguousarray(pos, np.float64),
+                              C.byref(n_iter), C.byref(conv), energies, drho, eps, _ptr(rho),
+                              _ptr(psi), _ptr(occ), C.byref(mu))
+    if rc:
+        raise RuntimeError(f"oracle scf failed: {rc}")
+    n = n_iter.value
+    return dict(n_iter=n, converged=bool(conv.value), energies=energies[:n], drho=drho[:n],
+                eps=eps, rho=rho, psi=psi, occ=occ, mu=mu.value, model=m)
+
+
+def scf_run_k(params, pos, kpts, weights):
+    """k-point SCF: kpts (n_k x 3 fractional), weights (n_k, sum 1)."""
+    kpts = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+    n_k = len(kpts)
+    n_iter, conv, mu = C.c_int(), C.c_int(), C.c_double()
+    energies = np.zeros(params.max_iter)
+    drho = np.zeros(params.max_iter)
+    eps = np.zeros(n_k * params.n_bands)
+    rho = np.zeros(params.n_grid ** 3)
+    n_pw = np.zeros(n_k, np.int32)
+    rc = lib().orc_scf_run_k(C.byref(params), np.ascontiguousarray(pos, np.float64), n_k,
+                             kpts.reshape(-1), np.ascontiguousarray(weights, np.float64),
+                             C.byref(n_iter), C.byref(conv), energies, drho, eps, _ptr(rho), n_pw,
+                             C.byref(mu))
+    if rc:
+        raise RuntimeError(f"oracle k-point scf failed: {rc}")
+    n = n_iter.value
+    return dict(n_iter=n, converged=bool(conv.value), energies=energies[:n], drho=drho[:n],
+                eps=eps.reshape(n_k, params.n_bands), rho=rho, n_pw=n_pw, mu=mu.value)
+
+
+def fermi_occupations_w(eps, w, n_e, kt):
+    eps = np.ascontiguousarray(eps, np.float64)
+    w = np.ascontiguousarray(w, np.float64)
+    occ = np.zeros(len(eps))
+    ts = C.c_double()
+    mu = lib().orc_fermi_occupations_w(eps, w, len(eps), n_e, kt, occ, C.byref(ts))
+    return occ, mu, ts.value
+
+
+class Pulay:
+    """The oracle's Pulay/Kerker density mixer on an n^3 grid of edge L."""
+
+    def __init__(self, n_grid, L, depth=6, alpha=0.5, kerker_q=0.8):
+        self.n3 = n_grid ** 3
+        self.h = lib().orc_pulay_create(n_grid, L, depth, alpha, kerker_q)
+
+    def mix(self, rin, rout):
+        out = np.zeros(self.n3)
+        lib().orc_pulay_mix(self.h, np.ascontiguousarray(rin, np.float64),
+                            np.ascontiguousarray(rout, np.float64), out)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_pulay_destroy(self.h)
+            self.h = None

@@ -175,6 +175,55 @@ double fermi_occupations(const double* eps, int n, double n_e, double kt, double
   return mu;
 }
 
+// Fermi-Dirac occupations of states with weights w_i (the k-point weight of
+// each state, summing to one per band index over k): sum_i w_i f_i = n_e,
+// -T S = 2 kT sum_i w_i [f ln f + (1-f) ln(1-f)]. With every w_i = 1 this is
+// fermi_occupations above. kT = 0: states filled in ascending order, two
+// electrons of weight w_i each, until n_e is reached.
+double fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                           double* occ, double* ts) {
+  *ts = 0;
+  double emin = eps[0], emax = eps[0];
+  for (int i = 1; i < n; ++i) {
+    emin = std::min(emin, eps[i]);
+    emax = std::max(emax, eps[i]);
+  }
+  if (kt <= 0) {
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return eps[a] < eps[b]; });
+    double left = n_e, mu = emin;
+    for (int i : ord) {
+      const double take = std::min(2.0, std::max(0.0, left / w[i]));
+      occ[i] = take;
+      left -= take * w[i];
+      if (take > 0) mu = eps[i];
+    }
+    return mu;
+  }
+  auto fd = [&](double e, double mu) {
+    double x = (e - mu) / kt;
+    if (x > 200) return 0.0;
+    if (x < -200) return 2.0;
+    return 2.0 / (1.0 + std::exp(x));
+  };
+  double lo = emin - 50 * kt, hi = emax + 50 * kt;
+  for (int it = 0; it < 200; ++it) {
+    double mid = 0.5 * (lo + hi), s = 0;
+    for (int i = 0; i < n; ++i) s += w[i] * fd(eps[i], mid);
+    if (s > n_e) hi = mid; else lo = mid;
+  }
+  const double mu = 0.5 * (lo + hi);
+  double t = 0;
+  for (int i = 0; i < n; ++i) {
+    occ[i] = fd(eps[i], mu);
+    const double f = occ[i] / 2.0;
+    if (f > 1e-300 && f < 1.0 - 1e-16) t += w[i] * (f * std::log(f) + (1.0 - f) * std::log(1.0 - f));
+  }
+  *ts = 2.0 * kt * t;
+  return mu;
+}
+
 // LDA: Slater exchange + Perdew-Zunger 1981 correlation (unpolarised).
 void lda_pz(double rho, double* exc, double* vxc) {
   if (rho <= 1e-14) {
@@ -288,7 +337,7 @@ void Pulay::mix(const System& sys, const std::vector<double>& rin, const std::ve
     for (int i = 0; i < k; ++i)
       for (int j = 0; j <= i; ++j) {
         double s = 0;
-        for (size_t r = 0; r < n3; ++r) s += hres[off + i][r] * hres[off + j][r];
+        for (size_t r = 0; r < n3; ++r) s += hres[i][r] * hres[j][r];
         a[i * (k + 1) + j] = s;
         a[j * (k + 1) + i] = s;
       }
@@ -471,6 +520,123 @@ int run_scf(const ScfParams& p, const double* pos, ScfResult& res, Model* model_
   res.rho = rho_out;
   res.psi = psi;
   if (model_out) *model_out = m;
+  return 0;
+}
+
+// k-point SCF: the k loop of ScfIteration::diagonalize (src/scf.rs:169-265,
+// one filter + Rayleigh-Ritz per k-point) around the same glue as run_scf.
+// Every k-point has its own plane-wave sphere 0.5|k+G|^2 <= ecut, start
+// block (seed_psi) and Lanczos bound (seed_lanczos); V_eff and the screened
+// D are shared. One Fermi level over all (k, band) states with the k
+// weights wk (sum 1); rho = sum_k wk rho_k; E = sum_k wk sum_i f eps -
+// int V_Hxc[rho_in] rho_out + E_H + E_xc - TS. The batched intent is
+// PROJECT_ROOT_PLAN.md:568-611.
+int run_scf_k(const ScfParams& p, const double* pos, int n_k, const double* kfrac,
+              const double* wk, ScfResultK& res) {
+  if (n_k < 1) return 1;
+  std::vector<Model> ms(n_k);
+  for (int k = 0; k < n_k; ++k) {
+    ScfParams pk = p;
+    for (int c = 0; c < 3; ++c) pk.kfrac[c] = kfrac[3 * k + c];
+    setup_model(pk, pos, ms[k]);
+  }
+  const Model& m0 = ms[0];
+  const int n_b = p.n_bands;
+  const size_t n3 = (size_t)p.n_grid * p.n_grid * p.n_grid;
+  const double n_e = p.zval * m0.sys.n_atoms;
+  std::vector<double> vion;
+  vion_real(m0.sys, vion);
+  std::vector<std::vector<cd>> psi(n_k);
+  std::vector<std::vector<double>> eps(n_k);
+  res.n_pw.assign(n_k, 0);
+  for (int k = 0; k < n_k; ++k) {
+    initial_psi(ms[k].pw, n_b, p.seed_psi, psi[k]);
+    res.n_pw[k] = ms[k].pw.n_pw;
+  }
+  std::vector<double> rho_in(n3, n_e / m0.sys.omega), rho_out, v, dscr;
+  std::vector<double> w_state((size_t)n_k * n_b), e_all((size_t)n_k * n_b),
+      occ((size_t)n_k * n_b);
+  for (int k = 0; k < n_k; ++k)
+    for (int i = 0; i < n_b; ++i) w_state[(size_t)k * n_b + i] = wk[k];
+  Pulay mixer;
+  mixer.depth = p.pulay_depth;
+  mixer.alpha = p.mix_alpha;
+  mixer.kerker_q = p.kerker_q;
+  res.energies.clear();
+  res.drho.clear();
+  res.n_iter = 0;
+  res.converged = 0;
+  int small = 0;
+  double e_prev = 0;
+  for (int it = 1; it <= p.max_iter; ++it) {
+    double eh_in, exc_in;
+    vhxc(m0.sys, rho_in, v, &eh_in, &exc_in);
+    for (int k = 0; k < n_k; ++k) {
+      ms[k].veff.resize(n3);
+      for (size_t r = 0; r < n3; ++r) ms[k].veff[r] = vion[r] + v[r];
+    }
+    if (m0.pj.n_proj) {
+      screen_d(ms[0], dscr);
+      for (int k = 0; k < n_k; ++k) ms[k].pj.d = dscr;
+    }
+    for (int k = 0; k < n_k; ++k) {
+      const double lmax = p.lanczos_k > 0 ? lanczos_bound(ms[k], p.lanczos_k, p.seed_lanczos, nullptr)
+                                          : grid_lambda_max(ms[k]);
+      const int passes = it == 1 ? p.first_passes : 1;
+      for (int ps = 0; ps < passes; ++ps) {
+        Bounds bd = spectral_bounds(eps[k].data(), (int)eps[k].size(), lmax);
+        std::vector<cd> y;
+        double diag[4];
+        int rc = chebyshev_filter(ms[k], psi[k].data(), n_b, p.cheb_degree, bd, y, diag);
+        if (rc) return rc;
+        rc = rayleigh_ritz(ms[k], y, n_b, eps[k], nullptr);
+        if (rc) return 1000 + rc;
+        psi[k].swap(y);
+      }
+      std::copy(eps[k].begin(), eps[k].end(), e_all.begin() + (size_t)k * n_b);
+    }
+    double ts;
+    res.mu = fermi_occupations_w(e_all.data(), w_state.data(), n_k * n_b, n_e, p.smear_kt,
+                                 occ.data(), &ts);
+    rho_out.assign(n3, 0.0);
+    double eband = 0;
+    for (int k = 0; k < n_k; ++k) {
+      std::vector<double> ok(n_b), rk;
+      for (int i = 0; i < n_b; ++i) {
+        ok[i] = wk[k] * occ[(size_t)k * n_b + i];
+        eband += ok[i] * eps[k][i];
+      }
+      density(ms[k], psi[k].data(), n_b, ok.data(), rk);
+      for (size_t r = 0; r < n3; ++r) rho_out[r] += rk[r];
+    }
+    double eh, exc;
+    std::vector<double> vout;
+    vhxc(m0.sys, rho_out, vout, &eh, &exc);
+    double dc = 0, drho = 0;
+    for (size_t r = 0; r < n3; ++r) {
+      dc += v[r] * rho_out[r];
+      drho += std::abs(rho_out[r] - rho_in[r]);
+    }
+    const double dv = m0.sys.omega / (double)n3;
+    const double etot = eband - dc * dv + eh + exc + ts;
+    res.energies.push_back(etot);
+    res.drho.push_back(drho * dv);
+    res.n_iter = it;
+    if (it > 1) {
+      double de = std::abs(etot - e_prev) / m0.sys.n_atoms;
+      small = de < p.etol ? small + 1 : 0;
+    }
+    e_prev = etot;
+    if (small >= 2) {
+      res.converged = 1;
+      break;
+    }
+    std::vector<double> rnext;
+    mixer.mix(m0.sys, rho_in, rho_out, rnext);
+    rho_in.swap(rnext);
+  }
+  res.eps = e_all;
+  res.rho = rho_out;
   return 0;
 }
 

@@ -32,6 +32,14 @@ struct ScfResult {
   std::vector<cd> psi;
 };
 
+// k-point SCF result: eigenvalues k-major (eps[k * n_bands + i]).
+struct ScfResultK {
+  int n_iter = 0, converged = 0;
+  double mu = 0;
+  std::vector<int> n_pw;
+  std::vector<double> energies, drho, eps, rho;
+};
+
 struct Pulay {
   int depth = 6;
   double alpha = 0.5, kerker_q = 0.8;
@@ -47,9 +55,13 @@ void vhxc(const System& sys, const std::vector<double>& rho, std::vector<double>
           double* exc);
 void density(const Model& m, const cd* psi, int n_b, const double* occ, std::vector<double>& rho);
 double fermi_occupations(const double* eps, int n, double n_e, double kt, double* occ, double* ts);
+double fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                           double* occ, double* ts);
 void screen_d(const Model& m, std::vector<double>& d);
 void initial_psi(const PwSet& pw, int n_b, uint64_t seed, std::vector<cd>& psi);
 void setup_model(const ScfParams& p, const double* pos, Model& m);
 int run_scf(const ScfParams& p, const double* pos, ScfResult& res, Model* model_out);
+int run_scf_k(const ScfParams& p, const double* pos, int n_k, const double* kfrac,
+              const double* wk, ScfResultK& res);
 
 }  // namespace orc
