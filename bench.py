@@ -25,9 +25,15 @@ The JSON line also carries
                  the algorithmic bytes of the stick/plane design, or the
                  dense FP32 / FP64 matrix peak for the GEMMs);
   cpu_baseline - the FP64 CPU oracle (oracle/) on this host: one SCF
-                 iteration timed and extrapolated (configs A, B), or a
-                 4-band H|psi> sample scaled to the run's band applications
-                 (config C, a lower bound).
+                 iteration of the workload at one k-point (the filter and
+                 density on a band sample for the 216-atom cell, scaled by
+                 the band count; Rayleigh-Ritz at full size), extrapolated to
+                 the GPU run's iterations and k-points; CPU model, core count
+                 and a one-thread filter rate stated.
+
+Config D (--config D) is the 64-atom USPP cell on the {0, 1/2}^3 k-point
+set (8 k-points, 160 bands each), whose filters run as one graph launch per
+pass.
 """
 import argparse
 import json
@@ -45,7 +51,12 @@ CONFIGS = {
     "A": dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10),
     "B": dict(n_rep=2, n_grid=72, n_bands=160, cheb_degree=12),
     "C": dict(n_rep=3, n_grid=108, n_bands=512, cheb_degree=12, n_proj_per_atom=4, q0=0.25),
+    "D": dict(n_rep=2, n_grid=72, n_bands=160, cheb_degree=12, n_proj_per_atom=4, q0=0.25),
 }
+# k-point sets: config D samples the 64-atom cell on the Gamma-centred 2x2x2
+# set {0, 1/2}^3 (equal weights), whose spheres hold 23,847-23,984 plane waves
+# (the shifted (+-1/4)^3 set would give every k the same count)
+KPTS = {"D": [[i, j, k] for i in (0.0, 0.5) for j in (0.0, 0.5) for k in (0.0, 0.5)]}
 WORKLOAD = {
     "A": "8-atom diamond-cubic cell, Gamma, Ecut 15 Ha (2,945 PW), 36^3, 20 bands, "
          "Chebyshev degree 10",
@@ -54,6 +65,9 @@ WORKLOAD = {
     "C": "216-atom diamond-cubic 3x3x3 USPP cell (4 projectors/atom, 864 total, Q=0.25 "
          "Gaussian augmentation), Gamma, Ecut 15 Ha (80,725 PW), 108^3, 512 bands, "
          "Chebyshev degree 12, FP64 Cholesky-QR/Rayleigh-Ritz",
+    "D": "64-atom diamond-cubic 2x2x2 USPP cell (256 projectors), 8 k-points {0,1/2}^3 "
+         "(23,847-23,984 PW each), 72^3, 160 bands per k, Chebyshev degree 12, all k-points' "
+         "filters in one graph launch per pass, FP64 Rayleigh-Ritz per k, one Fermi level",
 }
 PEAK_TFLOPS = {"c128": 78.6, "c64": 157.3}  # MI355X dense FP64 / FP32 matrix peaks
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second (MI355X)
@@ -66,7 +80,10 @@ def roofline(chs, ctx, cfg, precision, tune=False):
     dominant (longest) one is reported against its MI355X peak: HBM 8 TB/s
     for the H|psi> kernels (algorithmic bytes of the stick/plane design),
     dense matrix-core TFLOP/s for the GEMMs."""
+    import hashlib
     import numpy as np
+    with open(chs.LIB_PATH, "rb") as f:
+        lib_sha = hashlib.sha256(f.read()).hexdigest()[:16]
     L = cfg["n_rep"] * 10.26
     n = cfg["n_grid"]
     prec = chs.C128 if precision == "c128" else chs.C64
@@ -104,8 +121,14 @@ def roofline(chs, ctx, cfg, precision, tune=False):
         with open(tpath) as f:
             t = json.load(f)
         if t.get("n_grid") == n and t.get("n_bands") == nb and t.get("precision") == precision:
-            traffic = t.get("bytes_per_launch")
-            if t.get("valu_wave_insts_per_launch"):
+            # counters come from a separate rocprofv3 run (profiles/): stamped with
+            # the library build they measured, so a stale file shows as such
+            valu["traffic_source"] = {"file": os.path.relpath(tpath, ROOT),
+                                      "library_sha256": t.get("library_sha256"),
+                                      "matches_this_build": t.get("library_sha256") == lib_sha}
+            traffic = t.get("bytes_per_launch") if valu["traffic_source"]["matches_this_build"] \
+                else None
+            if t.get("valu_wave_insts_per_launch") and traffic is not None:
                 # the plane kernel's own limit: VALU issue, one wave64 instruction
                 # per 4 cycles per SIMD (16 lanes, no packed FP32), 1,024 SIMDs
                 # at 2.4 GHz
@@ -122,66 +145,99 @@ def roofline(chs, ctx, cfg, precision, tune=False):
         "step_kernels_ms": {it["kernel"]: round(it["ms"], 4) for it in items},
         "hpsi_canonical_bytes": float(canonical),
         "hpsi_canonical_gbs": round(canonical / (hpsi_ms * 1e-3) / 1e9, 1),
-        "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision, **valu,
+        "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision,
+        "library_sha256": lib_sha, **valu,
         **({"hpsi_launch": dict(zip(("sticks_per_block", "row_div", "plane_threads"), tuned))}
            if tuned else {}),
     }
 
 
-def cpu_baseline(cfg, n_iter_gpu, n_pass_gpu):
-    """FP64 oracle on this host. Small cells: one SCF iteration (Lanczos k=10,
-    one filter + Rayleigh-Ritz pass, density, V_Hxc) timed and extrapolated to
-    the GPU run's iteration count (+3 first-iteration passes). Cells whose
-    iteration would take minutes: the oracle's H|psi> on a sample of 4 bands,
-    scaled to the H applications of the GPU run (n_bands x (degree + 1) per
-    filter + Rayleigh-Ritz pass), a lower bound on the CPU time."""
+def host_cpu():
+    """(model name, logical CPUs, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+def cpu_baseline(cfg, n_iter_gpu, kpts=None):
+    """The FP64 oracle (oracle/) on this host, one SCF iteration of the same
+    workload at one k-point: the 10-step Lanczos bound, the degree-m
+    Chebyshev filter of every band, Rayleigh-Ritz of the full block (H|Y> of
+    every band, Gram products, eigensolve, rotation), the density of the
+    occupied bands and V_Hxc. Cells whose filter would take minutes run the
+    filter and the density on a band sample and scale them by the band
+    count (bands are independent in both). The run's time is extrapolated:
+    iterations x (per-iteration time) + 3 first-iteration passes, times the
+    k-points. The filter is also timed on one thread (per band and step)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle as O
-    ocfg = {k: v for k, v in cfg.items()}
-    p = O.default_params(**ocfg)
-    pos = O.positions(cfg["n_rep"])
-    m = O.Model(p, pos)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    if cfg["n_grid"] > 72:
-        psi = m.initial_psi(4)
-        m.apply_h(psi)  # first touch
-        t0 = time.perf_counter()
-        m.apply_h(psi, want_s=cfg.get("n_proj_per_atom", 0) > 0)
-        per_band = (time.perf_counter() - t0) / 4
-        applies = n_pass_gpu * cfg["n_bands"] * (cfg["cheb_degree"] + 1)
-        return {
-            "value": round(per_band * applies, 1), "unit": "s", "cores": cores, "kind": "port",
-            "sample": f"oracle H|psi> (+S) on 4 bands of the same cell: {per_band * 1e3:.1f} ms "
-                      f"per band, x {applies} band applications of the GPU run "
-                      f"({n_pass_gpu} filter + Rayleigh-Ritz passes); dense algebra, density "
-                      f"and mixing not counted (lower bound)",
-        }
-    n3 = m.n3
+    model, ncpu, avail = host_cpu()
+    threads = max(1, min(16, avail))
+    O.set_num_threads(threads)
+    n_k = len(kpts) if kpts else 1
+    kf = kpts[0] if kpts else [0.0, 0.0, 0.0]
+    p = O.default_params(**cfg, kfrac=kf)
+    m = O.Model(p, O.positions(cfg["n_rep"]))
+    nb, deg = cfg["n_bands"], cfg["cheb_degree"]
+    n_e = 4.0 * 8 * cfg["n_rep"] ** 3
+    nocc = min(nb, int(n_e / 2) + 8)
     omega = (cfg["n_rep"] * 10.26) ** 3
-    rho = np.full(n3, 4.0 * 8 * cfg["n_rep"] ** 3 / omega)
-    v, _, _ = m.vhxc(rho)
+    v, _, _ = m.vhxc(np.full(m.n3, n_e / omega))
     m.set_veff(m.vion() + v)
-    psi = m.initial_psi(cfg["n_bands"])
+    if m.n_proj:
+        d = np.zeros(m.n_proj)
+        O.lib().orc_screen_d(m.h, d)
+        m.set_d(d)
+    psi = m.initial_psi(nb)
+    sample = nb if cfg["n_grid"] <= 72 else 16
     t0 = time.perf_counter()
     lmax, _ = m.lanczos_bound(10, 4)
     t1 = time.perf_counter()
-    _, y, _ = m.chebyshev_filter(psi, cfg["cheb_degree"], O.spectral_bounds(None, lmax))
-    y, eps = m.rayleigh_ritz(y)
+    b4 = O.spectral_bounds(None, lmax)
+    _, y, _ = m.chebyshev_filter(psi[:sample], deg, b4)
     t2 = time.perf_counter()
-    occ, _, _ = O.fermi_occupations(eps, 4.0 * 8 * cfg["n_rep"] ** 3, 0.01)
-    rho_out = m.density(y, occ)
-    m.vhxc(rho_out)
+    m.rayleigh_ritz(psi)
     t3 = time.perf_counter()
-    t_pass = t2 - t1
-    t_iter = (t1 - t0) + t_pass + (t3 - t2)
-    total = n_iter_gpu * t_iter + 3 * t_pass
+    occ = np.full(min(sample, nocc), 2.0)
+    m.density(y[: len(occ)], occ)
+    m.vhxc(np.full(m.n3, n_e / omega))
+    t4 = time.perf_counter()
+    filt = (t2 - t1) * nb / sample
+    rr = t3 - t2
+    den = (t4 - t3) * nocc / len(occ)
+    per_iter = (t1 - t0) + filt + rr + den
+    total = n_k * (n_iter_gpu * per_iter + 3 * (filt + rr))
+    # one thread: the filter on one band (deg H applications and updates)
+    O.set_num_threads(1)
+    s0 = time.perf_counter()
+    m.chebyshev_filter(psi[:1], deg, b4)
+    one = (time.perf_counter() - s0) / deg
+    O.set_num_threads(threads)
+    scaled = f"on {sample} of {nb} bands, scaled x{nb / sample:g}" if sample < nb else "all bands"
     return {
-        "value": round(total, 3), "unit": "s", "cores": cores, "kind": "port",
-        "sample": f"one SCF iteration of the same workload with the FP64 oracle "
-                  f"({t_iter:.2f} s: Lanczos {t1 - t0:.2f} s, filter+RR {t_pass:.2f} s, "
-                  f"density+V_Hxc {t3 - t2:.2f} s) x {n_iter_gpu} GPU iterations "
-                  f"+ 3 first-iteration passes (extrapolated)",
+        "value": round(total, 1), "unit": "s", "cores": threads, "kind": "port",
+        "host": {"cpu_model": model, "logical_cpus": ncpu, "cpus_available": avail},
+        "per_iteration_s": {"lanczos": round(t1 - t0, 3), "filter": round(filt, 2),
+                            "rayleigh_ritz": round(rr, 2), "density_vhxc": round(den, 2)},
+        "filter_band_step_ms": {f"{threads}_threads": round((t2 - t1) / (sample * deg) * 1e3, 2),
+                                "1_thread": round(one * 1e3, 2)},
+        "sample": f"one SCF iteration of the same workload at one k-point with the FP64 oracle: "
+                  f"Lanczos (10 steps), filter {scaled}, Rayleigh-Ritz of all {nb} bands, "
+                  f"density of the occupied bands{' (sampled)' if sample < nb else ''} + V_Hxc; "
+                  f"extrapolated to {n_iter_gpu} iterations + 3 first-iteration passes"
+                  f"{f' x {n_k} k-points' if n_k > 1 else ''}",
     }
 
 
@@ -239,18 +295,24 @@ def learned_report(chs, ctx):
     }
 
 
-def run_form(chs, ctx, cfg, precision, steps, warmup):
+def run_form(chs, ctx, cfg, precision, steps, warmup, kpts=None):
     """Warm-up and timed SCF runs of one precision form; returns the runs."""
     pos = chs.synth_positions(cfg["n_rep"])
     prec = chs.C128 if precision == "c128" else chs.C64
     params = chs.scf_params(**cfg, max_iter=100, precision=prec)
+
+    def run():
+        if kpts:
+            return chs.run_scf_k(ctx, params, pos, kpts, [1.0 / len(kpts)] * len(kpts))
+        return chs.run_scf(ctx, params, pos)
+
     for _ in range(warmup):
-        r = chs.run_scf(ctx, params, pos)
+        r = run()
         print(f"[{precision}] warmup: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
               f"converged={r['converged']}", file=sys.stderr, flush=True)
     runs = []
     for k in range(steps):
-        r = chs.run_scf(ctx, params, pos)
+        r = run()
         runs.append(r)
         print(f"[{precision}] step {k}: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
               f"E = {r['energy']:.10f}", file=sys.stderr, flush=True)
@@ -291,8 +353,9 @@ def main():
     import chs
     ctx = chs.Context(0)
     cfg = CONFIGS[args.config]
-    if args.config == "C" and not args.no_compare:
+    if args.config in ("C", "D") and not args.no_compare:
         args.compare = True
+    kpts = KPTS.get(args.config)
     precision = "c64" if args.compare else args.precision
     if args.hpsi_only:
         if args.copy_bytes:
@@ -306,7 +369,7 @@ def main():
         return
     ctx.synchronize()
     t0 = time.perf_counter()
-    runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup)
+    runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup, kpts)
     ctx.synchronize()
     t1 = time.perf_counter()
     value = sum(r["seconds"] for r in runs) / len(runs)
@@ -321,17 +384,27 @@ def main():
         "config": {"workload": WORKLOAD[args.config], "atoms": n_atoms,
                    "n_pw": last["n_pw"], "grid": cfg["n_grid"], "bands": cfg["n_bands"],
                    "projectors": last["n_proj"], "filter_precision": precision,
+                   "measured_form": (
+                       "complex64 Chebyshev filter (FP32 projector products on the matrix "
+                       "cores) with FP64 Rayleigh-Ritz, density and energies, the last SCF "
+                       "iterations with the complex128 filter; the learned preconditioner is "
+                       "not in the timed SCF (reported separately)") if precision == "c64"
+                   else "complex128 throughout",
+                   "learned_in_timed_run": False,
+                   **({"k_points": len(kpts), "filter_graph_launches_per_pass": 1}
+                      if kpts else {}),
                    "scf_iterations": last["n_iter"],
+                   "density_change_last": last["drho"],
                    "scf_iterations_fp32_filter": last["n_iter_fp32"], "energy_Ha": last["energy"],
                    "phase_s": {k: round(last[k], 4) for k in
                                ("t_filter", "t_rr", "t_density", "t_veff", "t_lanczos")}},
     }
     if args.compare:
-        ref = run_form(chs, ctx, cfg, "c128", args.steps, 0)
+        ref = run_form(chs, ctx, cfg, "c128", args.steps, 0, kpts)
         v64 = sum(r["seconds"] for r in ref) / len(ref)
         line["config"]["fp64_form"] = {
             "seconds": round(v64, 4), "scf_iterations": ref[-1]["n_iter"],
-            "energy_Ha": ref[-1]["energy"],
+            "energy_Ha": ref[-1]["energy"], "density_change_last": ref[-1]["drho"],
             "phase_s": {k: round(ref[-1][k], 4) for k in
                         ("t_filter", "t_rr", "t_density", "t_veff", "t_lanczos")}}
         line["config"]["speedup_vs_fp64"] = round(v64 / value, 3)
@@ -340,8 +413,7 @@ def main():
     if args.compare and not args.no_learned:
         line["learned_preconditioner"] = learned_report(chs, ctx)
     if not args.no_cpu:
-        n_pass = last["n_iter"] + 3
-        line["cpu_baseline"] = cpu_baseline(cfg, last["n_iter"], n_pass)
+        line["cpu_baseline"] = cpu_baseline(cfg, last["n_iter"], kpts)
     print(json.dumps(line))
 
 

@@ -76,7 +76,21 @@ class ScfResult(C.Structure):
                 ("n_proj", C.c_int32), ("energy", C.c_double), ("mu", C.c_double),
                 ("seconds", C.c_double), ("t_filter", C.c_double), ("t_rr", C.c_double),
                 ("t_density", C.c_double), ("t_veff", C.c_double), ("t_lanczos", C.c_double),
-                ("n_iter_fp32", C.c_int32)]
+                ("n_iter_fp32", C.c_int32), ("drho", C.c_double)]
+
+
+class ScfSystem(C.Structure):
+    """chs_scf_system: a caller-supplied cell (scf_context_create,
+    PROJECT_ROOT_PLAN.md:880-1029)."""
+    _fields_ = [
+        ("grid", C.c_int32 * 3), ("real_lattice", C.c_double * 9),
+        ("n_atoms", C.c_int32), ("positions", C.c_void_p), ("n_electrons", C.c_double),
+        ("v_ion", C.c_void_p), ("n_bands", C.c_int32), ("n_k", C.c_int32),
+        ("k_frac", C.c_void_p), ("k_weight", C.c_void_p), ("n_pw", C.c_void_p),
+        ("miller", C.c_void_p), ("proj_per_atom", C.c_int32), ("beta", C.c_void_p),
+        ("d0", C.c_void_p), ("q0", C.c_double), ("q_rc", C.c_double), ("rho0", C.c_void_p),
+        ("psi0", C.c_void_p),
+    ]
 
 
 _LIB = None
@@ -107,7 +121,8 @@ def lib():
             "chs_memset": [vp, vp, i, sz],
             "chs_pw_coords_to_fft_indices": [vp, vp, i, i, i, i, vp],
             "chs_gsphere": [vp, d, d, dp, i, i, C.POINTER(i), vp, vp, vp],
-            "chs_fft3d_c2c": [vp, i, vp, i, i, i],
+            "chs_fft3d_c2c": [vp, i, vp, i, i, i, i, i],
+            "chs_fft3d_z2d": [vp, vp, i, i, i, i, vp],
             "chs_bind_hamiltonian": [vp, i, i, vp, vp, vp],
             "chs_set_veff": [vp, vp],
             "chs_bind_projectors": [vp, i, vp, vp, vp],
@@ -126,6 +141,13 @@ def lib():
             "chs_zscal": [vp, i, dp, vp],
             "chs_density": [vp, vp, i, dp, d, vp],
             "chs_scf_run": [vp, C.POINTER(ScfParams), dp, C.POINTER(ScfResult), dp, dp, vp],
+            "chs_scf_run_k": [vp, C.POINTER(ScfParams), dp, i, dp, dp, C.POINTER(ScfResult), dp,
+                              dp, dp, vp],
+            "chs_scf_context_create": [vp, C.POINTER(ScfSystem), C.POINTER(ScfParams),
+                                       C.POINTER(vp)],
+            "chs_scf_context_run": [vp, C.POINTER(ScfResult), dp, dp, dp, vp],
+            "chs_scf_get_psi": [vp, i, vp],
+            "chs_scf_context_destroy": [vp],
             "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
             "chs_band_cg": [vp, vp, i, i, d, i, vp, dp, vp, vp],
             "chs_cgemm": [vp, C.c_char, C.c_char, i, i, i, C.POINTER(C.c_float), vp, i, vp, i,
@@ -275,10 +297,23 @@ def gsphere(ctx, L, ecut, n_grid, kfrac=(0.0, 0.0, 0.0)):
 
 
 def fft3d_c2c(ctx, grid, sign):
-    """BatchedFftPlan3d c2c (sign +1 inverse, -1 forward) in place on a device array
-    of shape (batch, n, n, n)."""
-    batch, n = grid.shape[0], grid.shape[-1]
-    ctx.check(lib().chs_fft3d_c2c(ctx.h, precision_of(grid.dtype), grid.ptr, n, batch, sign))
+    """BatchedFftPlan3d c2c (plan_batched_c2c(nx, ny, nz, batch); sign +1
+    inverse, -1 forward) in place on a device array of shape
+    (batch, nz, ny, nx) (x fastest)."""
+    batch, nz, ny, nx = grid.shape
+    ctx.check(lib().chs_fft3d_c2c(ctx.h, precision_of(grid.dtype), grid.ptr, nx, ny, nz, batch,
+                                  sign))
+
+
+def fft3d_z2d(ctx, half, nx):
+    """BatchedFftPlan3d z2d (plan_batched_z2d): Hermitian half grids, device
+    complex128 of shape (batch, nz, ny, nx // 2 + 1), to real grids
+    (batch, nz, ny, nx), unnormalised inverse."""
+    batch, nz, ny, nh = half.shape
+    assert nh == nx // 2 + 1
+    out = Gpu(ctx, (batch, nz, ny, nx), np.float64)
+    ctx.check(lib().chs_fft3d_z2d(ctx.h, half.ptr, nx, ny, nz, batch, out.ptr))
+    return out
 
 
 # ---- Hamiltonian and eigensolver ----------------------------------------
@@ -465,6 +500,128 @@ def run_scf(ctx, params, pos, want_rho=False):
     out["rho"] = rho
     out["status"] = rc
     return out
+
+
+def run_scf_k(ctx, params, pos, kpts, weights, want_rho=False):
+    """k-point SCF of the synthetic cell (the k loop of ScfIteration::
+    diagonalize, src/scf.rs:169-265): one Fermi level, rho = sum_k w_k rho_k,
+    all k-points' filters in one graph launch. eps is (n_k, n_bands)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    kpts = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(weights, np.float64)
+    ctx._bound_hamiltonian = None
+    ctx._bound_kset = None
+    res = ScfResult()
+    energies = np.zeros(params.max_iter)
+    drho = np.zeros(params.max_iter)
+    eps = np.zeros((len(kpts), params.n_bands))
+    rho = np.zeros(params.n_grid ** 3) if want_rho else None
+    rc = lib().chs_scf_run_k(ctx.h, C.byref(params), _dptr(pos), len(kpts), _dptr(kpts), _dptr(w),
+                             C.byref(res), _dptr(energies), _dptr(drho), _dptr(eps),
+                             rho.ctypes.data_as(vp) if rho is not None else None)
+    if rc not in (OK, -11):
+        ctx.check(rc)
+    out = {f: getattr(res, f) for f, _ in ScfResult._fields_}
+    out["energies"] = energies[: res.n_iter]
+    out["drho_trace"] = drho[: res.n_iter]
+    out["eps"] = eps
+    out["rho"] = rho
+    out["status"] = rc
+    return out
+
+
+class ScfContext:
+    """A caller-supplied system on the device (chs_scf_context_create /
+    chs_scf_context_run / chs_scf_context_destroy; the reference's planned
+    scf_context_create / scf_run / scf_context_destroy,
+    PROJECT_ROOT_PLAN.md:880-1029). Arrays are host numpy arrays:
+    lattice (3, 3), positions (n_atoms, 3), v_ion (n^3), miller per k
+    (n_pw_k, 3), beta per k (P, n_pw_k) complex, d0 (P), rho0 (n^3),
+    psi0 per k (n_bands, n_pw_k) complex. `solver` is a scf_params() whose
+    solver fields are used."""
+
+    def __init__(self, ctx, grid, lattice, positions, n_electrons, v_ion, n_bands, kpts,
+                 weights, miller, solver, proj_per_atom=0, beta=None, d0=None, q0=0.0,
+                 q_rc=1.2, rho0=None, psi0=None):
+        self.ctx = ctx
+        self.n_bands = n_bands
+        keep = []
+
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        def ptrs(lst, dt):
+            if lst is None:
+                return None
+            ps = (C.c_void_p * len(lst))(*[arr(a, dt) for a in lst])
+            keep.append(ps)
+            return C.cast(ps, C.c_void_p)
+
+        kpts = np.ascontiguousarray(kpts, np.float64).reshape(-1, 3)
+        self.n_pw = [int(np.asarray(m).shape[0]) for m in miller]
+        sy = ScfSystem()
+        for i in range(3):
+            sy.grid[i] = grid[i]
+        for i, v in enumerate(np.asarray(lattice, np.float64).reshape(9)):
+            sy.real_lattice[i] = v
+        pos = np.asarray(positions, np.float64).reshape(-1, 3)
+        sy.n_atoms = pos.shape[0]
+        sy.positions = arr(pos, np.float64)
+        sy.n_electrons = n_electrons
+        sy.v_ion = arr(v_ion, np.float64)
+        sy.n_bands = n_bands
+        sy.n_k = len(kpts)
+        sy.k_frac = arr(kpts, np.float64)
+        sy.k_weight = arr(weights, np.float64)
+        sy.n_pw = arr(self.n_pw, np.int32)
+        sy.miller = ptrs(miller, np.int32)
+        sy.proj_per_atom = proj_per_atom
+        sy.beta = ptrs(beta, np.complex128)
+        sy.d0 = arr(d0, np.float64) if d0 is not None else None
+        sy.q0 = q0
+        sy.q_rc = q_rc
+        sy.rho0 = arr(rho0, np.float64) if rho0 is not None else None
+        sy.psi0 = ptrs(psi0, np.complex128)
+        self.n_grid = grid[0]
+        self.max_iter = solver.max_iter
+        h = vp()
+        ctx.check(lib().chs_scf_context_create(ctx.h, C.byref(sy), C.byref(solver), C.byref(h)))
+        self.h = h
+
+    def run(self, want_rho=False):
+        res = ScfResult()
+        energies = np.zeros(self.max_iter)
+        drho = np.zeros(self.max_iter)
+        eps = np.zeros((len(self.n_pw), self.n_bands))
+        rho = np.zeros(self.n_grid ** 3) if want_rho else None
+        self.ctx._bound_hamiltonian = None
+        self.ctx._bound_kset = None
+        rc = lib().chs_scf_context_run(self.h, C.byref(res), _dptr(energies), _dptr(drho),
+                                       _dptr(eps), rho.ctypes.data_as(vp) if rho is not None else None)
+        if rc not in (OK, -11):
+            self.ctx.check(rc)
+        out = {f: getattr(res, f) for f, _ in ScfResult._fields_}
+        out["energies"] = energies[: res.n_iter]
+        out["drho_trace"] = drho[: res.n_iter]
+        out["eps"] = eps
+        out["rho"] = rho
+        out["status"] = rc
+        return out
+
+    def psi(self, k=0):
+        out = np.zeros((self.n_bands, self.n_pw[k]), np.complex128)
+        self.ctx.check(lib().chs_scf_get_psi(self.h, k, out.ctypes.data_as(vp)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().chs_scf_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def profile_projector_products(ctx, n_b, precision=C64, reps=5):

@@ -385,6 +385,7 @@ int BandCg::setup_precond(bool uspp) {
   std::vector<double> kin(n_pw);
   CHS_HIP(ctx, hipMemcpyAsync(kin.data(), H.kin, n_pw * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += n_pw * sizeof(double);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   double sum = 0;
   for (double k : kin) sum += k;
@@ -541,6 +542,7 @@ int BandCg::run(z64* psi, int max_steps, double tol, const float* mlp, double* e
     CHS_LAUNCH_CHECK(ctx);
     CHS_HIP(ctx, hipMemcpyAsync(hst.data(), st, n_b * sizeof(BandState), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    ctx->d2h += n_b * sizeof(BandState);
     CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     bool any = false;
     for (const BandState& b : hst) any = any || b.active;
@@ -548,6 +550,7 @@ int BandCg::run(z64* psi, int max_steps, double tol, const float* mlp, double* e
   }
   CHS_HIP(ctx, hipMemcpyAsync(hst.data(), st, n_b * sizeof(BandState), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += n_b * sizeof(BandState);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int b = 0; b < n_b; ++b) {
     if (hst[b].flag) return set_error(ctx, CHS_ERR_INVALID_ARG,

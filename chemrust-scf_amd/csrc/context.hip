@@ -56,6 +56,7 @@ int twiddles(chs_context* ctx, int n, int precision, const void** out) {
     }
     CHS_HIP(ctx, hipMalloc(&d, h.size() * sizeof(double)));
     CHS_HIP(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->h2d += h.size() * sizeof(double);
   } else {
     std::vector<float> h(2 * n);
     for (int t = 0; t < n; ++t) {
@@ -65,6 +66,7 @@ int twiddles(chs_context* ctx, int n, int precision, const void** out) {
     }
     CHS_HIP(ctx, hipMalloc(&d, h.size() * sizeof(float)));
     CHS_HIP(ctx, hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    ctx->h2d += h.size() * sizeof(float);
   }
   ctx->impl->twiddles[key] = d;
   *out = d;
@@ -110,10 +112,12 @@ int chs_context_destroy(chs_context* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   auto* im = ctx->impl;
+  for (chs_scf* s : im->scf_children) scf_orphan(s);
+  im->scf_children.clear();
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
                     &im->ws4, &im->ws5, &im->ws6, &im->rr32, &im->lanczos_v0, &im->gemm_ws, &im->gemv_ws, &im->h3_bws, &im->lin_ws, &im->scf_ws,
-                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->kset_ws,
+                    &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->fft_ws, &im->kset_ws,
                     &im->gemm_tune_ws, &im->gemm_tune_c};
   for (DevBuf* b : bufs) release(*b);
   release_hamiltonian(im->ham);

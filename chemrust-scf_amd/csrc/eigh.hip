@@ -797,15 +797,17 @@ int heev_tridiag(chs_context* ctx, z64* A, int n, double* w, z64* U) {
   EighGraph& P = ctx->impl->eigh_graph;
   if (ctx->impl->use_graphs && n > 1) {
     if (!P.exec || P.n != n || P.gA != A || P.gws != ctx->impl->eig_ws.p) {
+      // the key names every address the graph holds: A and the workspace
+      // (all of ws lies in eig_ws, laid out by n)
       if (P.exec) (void)hipGraphExecDestroy(P.exec);
-      P.exec = nullptr;
-      hipGraph_t graph;
-      CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      tridiagonalise_launches(ctx, A, n, ws);
-      CHS_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
-      const hipError_t ie = hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      CHS_HIP(ctx, ie);
+      P = EighGraph{};
+      CHS_TRY(capture_graph(
+          ctx,
+          [&](chs_context* c) {
+            tridiagonalise_launches(c, A, n, ws);
+            return CHS_OK;
+          },
+          &P.exec));
       P.n = n;
       P.gA = A;
       P.gws = ctx->impl->eig_ws.p;

@@ -7,33 +7,34 @@ namespace chs {
 
 constexpr int kLines = 16;
 
-// One pass over axis `axis` (0 = x, 1 = y, 2 = z) of `batch` grids. A block
-// transforms kLines lines: for x, kLines consecutive rows; for y and z,
-// kLines consecutive ix at fixed (iz) resp. (iy), so global loads coalesce.
+// One pass over axis `axis` (0 = x, 1 = y, 2 = z; N = that axis' length) of
+// `batch` nx x ny x nz grids (x fastest). A block transforms kLines lines:
+// for x, kLines consecutive rows; for y and z, kLines consecutive ix at fixed
+// iz resp. iy, so global loads coalesce.
 template <int N, int S, typename T>
-__global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ g, int axis,
+__global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ g, int axis, int nx,
+                                                  int ny, int nz,
                                                   const cplx<T>* __restrict__ tw) {
   constexpr int LS = N + 1;
   __shared__ cplx<T> lds[kLines * LS];
-  const size_t n2 = (size_t)N * N, n3 = n2 * N;
-  const int groups_per_plane = (N + kLines - 1) / kLines;
-  const int b = blockIdx.y;
-  cplx<T>* grid = g + (size_t)b * n3;
+  const size_t nxy = (size_t)nx * ny, n3 = nxy * nz;
+  cplx<T>* grid = g + (size_t)blockIdx.y * n3;
   size_t base, si, se;
   int nl = kLines;
   if (axis == 0) {
-    const size_t row0 = (size_t)blockIdx.x * kLines;
+    const size_t row0 = (size_t)blockIdx.x * kLines, rows = (size_t)ny * nz;
     base = row0 * N;
     si = N;
     se = 1;
-    if (row0 + nl > n2) nl = (int)(n2 - row0);
+    if (row0 + nl > rows) nl = (int)(rows - row0);
   } else {
-    const int o = blockIdx.x / groups_per_plane;  // iz (axis 1) or iy (axis 2)
-    const int ix0 = (blockIdx.x % groups_per_plane) * kLines;
-    base = (axis == 1 ? (size_t)o * n2 : (size_t)o * N) + ix0;
+    const int gpp = (nx + kLines - 1) / kLines;  // line groups per plane
+    const int o = blockIdx.x / gpp;              // iz (axis 1) or iy (axis 2)
+    const int ix0 = (blockIdx.x % gpp) * kLines;
+    base = (axis == 1 ? (size_t)o * nxy : (size_t)o * nx) + ix0;
     si = 1;
-    se = axis == 1 ? N : n2;
-    if (ix0 + nl > N) nl = N - ix0;
+    se = axis == 1 ? (size_t)nx : nxy;
+    if (ix0 + nl > nx) nl = nx - ix0;
   }
   const int tot = kLines * N;
   for (int t = threadIdx.x; t < tot; t += blockDim.x) {
@@ -50,40 +51,104 @@ __global__ void __launch_bounds__(256) k_fft_axis(cplx<T>* __restrict__ g, int a
 }
 
 template <typename T>
-static int fft3d_impl(chs_context* ctx, int precision, T* grid, int n, int batch, int sign) {
-  const void* tw = nullptr;
-  CHS_TRY(twiddles(ctx, n, precision, &tw));
+static int fft3d_impl(chs_context* ctx, int precision, T* grid, const int dims[3], int batch,
+                      int sign) {
   using C = cplx<T>;
-  CHS_DISPATCH_N(n, {
-    const int groups = (NN + kLines - 1) / kLines;
-    dim3 gx(ceil_div((size_t)NN * NN, kLines), batch), gyz(NN * groups, batch);
-    for (int axis = 0; axis < 3; ++axis) {
-      dim3 grd = axis == 0 ? gx : gyz;
+  const int nx = dims[0], ny = dims[1], nz = dims[2];
+  const int gpp = (nx + kLines - 1) / kLines;
+  for (int axis = 0; axis < 3; ++axis) {
+    const int len = dims[axis];
+    const void* tw = nullptr;
+    CHS_TRY(twiddles(ctx, len, precision, &tw));
+    const dim3 grd(axis == 0 ? ceil_div((size_t)ny * nz, kLines)
+                             : (unsigned)((axis == 1 ? nz : ny) * gpp),
+                   batch);
+    CHS_DISPATCH_N(len, {
       if (sign > 0)
         hipLaunchKernelGGL((k_fft_axis<NN, 1, T>), grd, dim3(256), 0, ctx->stream, (C*)grid,
-                           axis, (const C*)tw);
+                           axis, nx, ny, nz, (const C*)tw);
       else
         hipLaunchKernelGGL((k_fft_axis<NN, -1, T>), grd, dim3(256), 0, ctx->stream, (C*)grid,
-                           axis, (const C*)tw);
+                           axis, nx, ny, nz, (const C*)tw);
       CHS_LAUNCH_CHECK(ctx);
-    }
-  });
+    });
+  }
   return CHS_OK;
 }
 
+int fft3d_xyz(chs_context* ctx, int precision, void* grid, const int dims[3], int batch,
+              int sign) {
+  if (precision == CHS_C128) return fft3d_impl(ctx, precision, (double*)grid, dims, batch, sign);
+  return fft3d_impl(ctx, precision, (float*)grid, dims, batch, sign);
+}
+
 int fft3d(chs_context* ctx, int precision, void* grid, int n, int batch, int sign) {
-  if (precision == CHS_C128) return fft3d_impl(ctx, precision, (double*)grid, n, batch, sign);
-  return fft3d_impl(ctx, precision, (float*)grid, n, batch, sign);
+  const int dims[3] = {n, n, n};
+  return fft3d_xyz(ctx, precision, grid, dims, batch, sign);
+}
+
+// Hermitian half [nz][ny][nx/2 + 1] (x fastest) -> full complex grid
+// [nz][ny][nx]: entries past the half are conj(a(-x, -y, -z)).
+__global__ void k_hermitian_expand(const z64* __restrict__ half, int nx, int ny, int nz,
+                                   z64* __restrict__ full) {
+  const size_t n3 = (size_t)nx * ny * nz;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n3) return;
+  const int b = blockIdx.y;
+  const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((size_t)nx * ny));
+  const int nh = nx / 2 + 1;
+  const z64* hb = half + (size_t)b * nh * ny * nz;
+  z64 v;
+  if (ix < nh) {
+    v = hb[ix + (size_t)nh * (iy + (size_t)ny * iz)];
+  } else {
+    const int jx = nx - ix, jy = (ny - iy) % ny, jz = (nz - iz) % nz;
+    v = conj(hb[jx + (size_t)nh * (jy + (size_t)ny * jz)]);
+  }
+  full[(size_t)b * n3 + i] = v;
+}
+
+__global__ void k_real_of(const z64* __restrict__ a, double* __restrict__ r, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = a[i].x;
 }
 
 }  // namespace chs
 
-extern "C" int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int n, int batch,
-                             int sign) {
-  if (!ctx || !d_grid || batch <= 0 || (sign != 1 && sign != -1) ||
+extern "C" {
+
+int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int nx, int ny, int nz,
+                  int batch, int sign) {
+  if (!ctx || !d_grid || batch <= 0 || batch > 65535 || (sign != 1 && sign != -1) ||
       (precision != CHS_C128 && precision != CHS_C64))
     return CHS_ERR_INVALID_ARG;
-  if (!chs::fft_length_supported(n))
-    return chs::set_error(ctx, CHS_ERR_FFT, "unsupported FFT length %d", n);
-  return chs::fft3d(ctx, precision, d_grid, n, batch, sign);
+  const int dims[3] = {nx, ny, nz};
+  for (int d : dims)
+    if (!chs::fft_length_supported(d))
+      return chs::set_error(ctx, CHS_ERR_FFT, "unsupported FFT length %d", d);
+  return chs::fft3d_xyz(ctx, precision, d_grid, dims, batch, sign);
 }
+
+int chs_fft3d_z2d(chs_context* ctx, const double* d_half, int nx, int ny, int nz, int batch,
+                  double* d_real) {
+  if (!ctx || !d_half || !d_real || batch <= 0 || batch > 65535) return CHS_ERR_INVALID_ARG;
+  const int dims[3] = {nx, ny, nz};
+  for (int d : dims)
+    if (!chs::fft_length_supported(d))
+      return chs::set_error(ctx, CHS_ERR_FFT, "unsupported FFT length %d", d);
+  using namespace chs;
+  const size_t n3 = (size_t)nx * ny * nz;
+  auto* im = ctx->impl;
+  CHS_TRY(ensure(ctx, im->fft_ws, n3 * batch * sizeof(z64)));
+  z64* full = (z64*)im->fft_ws.p;
+  hipLaunchKernelGGL(k_hermitian_expand, dim3(ceil_div(n3, 256), batch), dim3(256), 0,
+                     ctx->stream, (const z64*)d_half, nx, ny, nz, full);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(fft3d_xyz(ctx, CHS_C128, full, dims, batch, 1));
+  hipLaunchKernelGGL(k_real_of, dim3(ceil_div(n3 * batch, 256)), dim3(256), 0, ctx->stream, full,
+                     d_real, n3 * batch);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+}  // extern "C"

@@ -349,12 +349,12 @@ __global__ void __launch_bounds__(TPB) k_density_plane_w(const cplx<T>* __restri
 }
 
 __global__ void k_density_reduce(const double* __restrict__ part, int n_chunks, size_t n3,
-                                 double inv_omega, double* __restrict__ rho) {
+                                 double inv_omega, int accumulate, double* __restrict__ rho) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n3) return;
   double s = 0;
   for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * n3 + i];
-  rho[i] = s * inv_omega;
+  rho[i] = accumulate ? rho[i] + s * inv_omega : s * inv_omega;
 }
 
 // ---------------------------------------------------------------------------
@@ -703,10 +703,10 @@ int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, voi
 int ensure_precision_ready(chs_context* ctx, int precision);
 
 // rho = sum_b occ_b |psi_b(r)|^2 / omega from a band-major block in either
-// precision (FP64 accumulation).
+// precision (FP64 accumulation); with `accumulate` the sum is added to rho.
 template <typename T>
 static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double* occ_host,
-                     double omega, double* d_rho) {
+                     double omega, double* d_rho, bool accumulate) {
   Hamiltonian& H = ctx->impl->ham;
   if (!H.bound) return set_error(ctx, CHS_ERR_INVALID_ARG, "no Hamiltonian bound");
   auto* im = ctx->impl;
@@ -727,7 +727,7 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
                               ctx->stream));
   ctx->h2d += n_b * sizeof(double);
   if (nb == 0) {
-    CHS_HIP(ctx, hipMemsetAsync(d_rho, 0, n3 * sizeof(double), ctx->stream));
+    if (!accumulate) CHS_HIP(ctx, hipMemsetAsync(d_rho, 0, n3 * sizeof(double), ctx->stream));
     return CHS_OK;
   }
   CHS_TRY(ensure(ctx, im->stick_buf, (size_t)nb * n * stick_ld(st.n_st) * sizeof(cplx<T>)));
@@ -797,19 +797,19 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
     }
   });
   hipLaunchKernelGGL(k_density_reduce, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, part,
-                     n_chunks, n3, 1.0 / omega, d_rho);
+                     n_chunks, n3, 1.0 / omega, accumulate ? 1 : 0, d_rho);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
 }
 
 int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
-                 double* d_rho) {
-  return density_t<double>(ctx, psi, n_b, occ_host, omega, d_rho);
+                 double* d_rho, bool accumulate) {
+  return density_t<double>(ctx, psi, n_b, occ_host, omega, d_rho, accumulate);
 }
 
 int density_c64(chs_context* ctx, const c32* psi, int n_b, const double* occ_host, double omega,
-                double* d_rho) {
-  return density_t<float>(ctx, psi, n_b, occ_host, omega, d_rho);
+                double* d_rho, bool accumulate) {
+  return density_t<float>(ctx, psi, n_b, occ_host, omega, d_rho, accumulate);
 }
 
 __global__ void k_d2f(const double* __restrict__ a, float* __restrict__ b, size_t n) {

@@ -80,6 +80,19 @@ inline void release_hamiltonian(Hamiltonian& h) {
   h = Hamiltonian{};
 }
 
+// Forget the caller arrays the bound Hamiltonian points at (plane waves,
+// V_eff, projectors); its own buffers stay allocated for the next binding.
+inline void unbind_hamiltonian_fields(Hamiltonian& h) {
+  h.bound = false;
+  h.miller = nullptr;
+  h.kin = nullptr;
+  h.veff = nullptr;
+  h.veff_f_valid = false;
+  h.n_proj = 0;
+  h.beta = h.dion = h.qion = nullptr;
+  h.has_s = false;
+}
+
 // Captured launch sequence of the Householder tridiagonal reduction, valid
 // for one (n, matrix, workspace) triple.
 struct EighGraph {
@@ -122,6 +135,8 @@ struct VlocTune {
 
 }  // namespace chs
 
+struct chs_scf;  // a caller-supplied SCF system (scf_driver.hip)
+
 struct chs_context::Impl {
   std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
   std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
@@ -138,7 +153,7 @@ struct chs_context::Impl {
   chs::DevBuf lanczos_v0;
   uint64_t lanczos_seed = 0;
   int lanczos_n = 0;
-  chs::DevBuf gemm_ws, gemv_ws, h3_bws, lin_ws, scf_ws, proj_ws, chol_ws, eig_ws, mlp_ws;
+  chs::DevBuf gemm_ws, gemv_ws, h3_bws, lin_ws, scf_ws, proj_ws, chol_ws, eig_ws, mlp_ws, fft_ws;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // Replay launch-bound loops (Jacobi sweeps) as captured HIP graphs. Off
   // when CHS_NO_GRAPHS is set to a non-zero value: some kernel tracers
@@ -170,14 +185,21 @@ struct chs_context::Impl {
   chs::JacobiPlan jacobi;     // Jacobi eigensolver schedule (CHS_EIGH_JACOBI)
   std::vector<uintptr_t> kset_key;
   uint64_t kset_key_gen = 0;
+  // SCF systems created on this context: destroying the context frees their
+  // device arrays and detaches them, so a later chs_scf_context_destroy only
+  // frees the host side
+  std::vector<chs_scf*> scf_children;
 };
 
 namespace chs {
 
 // Length-N root table e^{+2 pi i t/N} (t < N) in the given precision.
 int twiddles(chs_context* ctx, int n, int precision, const void** out);
-// Batched in-place full-grid 3D FFT (fftgrid.hip).
+// Batched in-place full-grid 3D FFT (fftgrid.hip): n^3 grids, or
+// dims[0] x dims[1] x dims[2] (x fastest).
 int fft3d(chs_context* ctx, int precision, void* grid, int n, int batch, int sign);
+int fft3d_xyz(chs_context* ctx, int precision, void* grid, const int dims[3], int batch,
+              int sign);
 // Stick tables for the bound plane-wave set.
 int build_sticks(chs_context* ctx);
 // Bind a plane-wave set + V_eff as the current Hamiltonian (no k-set swap).
@@ -185,6 +207,16 @@ int bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_mi
                      const double* d_kin, const double* d_veff);
 // Make k-point k of the bound set the current Hamiltonian (k < 0: none).
 void kset_swap(chs_context* ctx, int k);
+// Drop the k-point set (its bindings, graph and per-k buffers).
+void kset_release(chs_context* ctx);
+// Free an SCF system's device arrays and detach it from its (dying) context.
+void scf_orphan(chs_scf* s);
+// Unbind the current Hamiltonian (a selected k-point first goes back to its
+// set): the operator calls then report that nothing is bound.
+inline void unbind_hamiltonian(chs_context* ctx) {
+  kset_swap(ctx, -1);
+  unbind_hamiltonian_fields(ctx->impl->ham);
+}
 
 // Local-potential H|psi> pipeline pieces (hamiltonian.hip).
 // d_coef: device {c, inv_e} read by the kernels instead of c, inv_e (so a
@@ -194,6 +226,31 @@ int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, voi
                   double* d_norm_part, int* n_part);
 int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
                   VlocTune* best, double* best_ms);
+
+// Records the launches `enqueue(ctx)` makes on the context stream into an
+// executable graph. Every exit leaves the stream out of capture mode and frees
+// the intermediate graph; *exec is written only when instantiation succeeded
+// (a failed instantiate leaves no handle that a later call could launch or
+// destroy).
+template <typename F>
+int capture_graph(chs_context* ctx, F&& enqueue, hipGraphExec_t* exec) {
+  CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue(ctx);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != CHS_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc != CHS_OK) return rc;
+    return set_error(ctx, CHS_ERR_DEVICE, "stream capture failed: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t e = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&e, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess)
+    return set_error(ctx, CHS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ie));
+  *exec = e;
+  return CHS_OK;
+}
 
 // Modes of the stick epilogue.
 enum EpiMode {

@@ -37,6 +37,14 @@ static void kset_drop_graph(chs_context* ctx) {
   im->kset_key.clear();
 }
 
+void kset_release(chs_context* ctx) {
+  auto* im = ctx->impl;
+  kset_swap(ctx, -1);
+  kset_drop_graph(ctx);
+  for (Hamiltonian& h : im->kset) release_hamiltonian(h);
+  im->kset.clear();
+}
+
 // Enqueue every k-point's filter; each result ends in its caller buffer.
 static int kset_enqueue(chs_context* ctx, int precision, void* const* d_psi, int n_b, int degree) {
   auto* im = ctx->impl;
@@ -161,19 +169,17 @@ int chs_kset_filter(chs_context* ctx, int precision, void* const* d_psi, int n_b
       kset_drop_graph(ctx);
       CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
       const uint64_t gen = im->ws_gen;
-      hipGraph_t graph;
-      CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      const int rc = kset_enqueue(ctx, precision, d_psi, n_b, degree);
-      const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
-      CHS_TRY(rc);
-      CHS_HIP(ctx, ec);
-      if (im->ws_gen != gen) {
-        (void)hipGraphDestroy(graph);
-        return set_error(ctx, CHS_ERR_DEVICE, "k-set capture allocated workspace");
-      }
-      const hipError_t ie = hipGraphInstantiate(&im->kset_graph, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      CHS_HIP(ctx, ie);
+      hipGraphExec_t exec = nullptr;
+      CHS_TRY(capture_graph(
+          ctx,
+          [&](chs_context* c) {
+            const int rc = kset_enqueue(c, precision, d_psi, n_b, degree);
+            if (rc == CHS_OK && im->ws_gen != gen)
+              return set_error(c, CHS_ERR_DEVICE, "k-set capture allocated workspace");
+            return rc;
+          },
+          &exec));
+      im->kset_graph = exec;
       im->kset_key = key;
       im->kset_key_gen = im->ws_gen;
     }

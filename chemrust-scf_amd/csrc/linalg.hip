@@ -378,6 +378,7 @@ static int jacobi_plan(chs_context* ctx, int n, JacobiPlan& P) {
   CHS_TRY(ensure(ctx, P.sched, s.size() * sizeof(int2)));
   CHS_HIP(ctx, hipMemcpyAsync(P.sched.p, s.data(), s.size() * sizeof(int2), hipMemcpyHostToDevice,
                               ctx->stream));
+  ctx->h2d += s.size() * sizeof(int2);
   CHS_TRY(ensure(ctx, P.rot, (size_t)P.pairs * sizeof(double4)));
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CHS_OK;
@@ -400,6 +401,7 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
     CHS_LAUNCH_CHECK(ctx);
     double h[2];
     CHS_HIP(ctx, hipMemcpyAsync(h, dn, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->d2h += 2 * sizeof(double);
     CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (h[0] <= 1e-30 * h[1] || h[0] == 0.0) break;
     // one sweep = 2 (n-1) small launches: captured once into a graph and
@@ -416,20 +418,24 @@ int heev_jacobi(chs_context* ctx, z64* A, int n, double* w, z64* U, z64* V, int 
       continue;
     }
     if (!P.exec || P.gA != A || P.gV != V) {
+      // the schedule and rotation buffers are fixed for this n (jacobi_plan
+      // drops the graph when n changes); A and V are the key
       if (P.exec) (void)hipGraphExecDestroy(P.exec);
       P.exec = nullptr;
-      hipGraph_t graph;
-      CHS_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      for (int r = 0; r < P.rounds; ++r) {
-        const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
-        hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, ctx->stream, A, V, n, pr,
-                           (double4*)P.rot.p);
-        hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, ctx->stream, A, n, pr,
-                           (const double4*)P.rot.p);
-      }
-      CHS_HIP(ctx, hipStreamEndCapture(ctx->stream, &graph));
-      CHS_HIP(ctx, hipGraphInstantiate(&P.exec, graph, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(graph);
+      P.gA = P.gV = nullptr;
+      CHS_TRY(capture_graph(
+          ctx,
+          [&](chs_context* c) {
+            for (int r = 0; r < P.rounds; ++r) {
+              const int2* pr = (const int2*)P.sched.p + (size_t)r * P.pairs;
+              hipLaunchKernelGGL(k_jacobi_cols, dim3(P.pairs), dim3(256), 0, c->stream, A, V, n,
+                                 pr, (double4*)P.rot.p);
+              hipLaunchKernelGGL(k_jacobi_rows, dim3(P.pairs), dim3(256), 0, c->stream, A, n, pr,
+                                 (const double4*)P.rot.p);
+            }
+            return CHS_OK;
+          },
+          &P.exec));
       P.gA = A;
       P.gV = V;
     }
@@ -466,6 +472,7 @@ int cholesky(chs_context* ctx, z64* A, int n, int* info_host) {
                      A, n, dscr, dinfo);
   CHS_LAUNCH_CHECK(ctx);
   CHS_HIP(ctx, hipMemcpyAsync(info_host, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->d2h += sizeof(int);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CHS_OK;
 }

@@ -625,6 +625,7 @@ int chs_mlp_set_params(chs_context* ctx, chs_mlp* m, const float* theta) {
   CHS_TRY(ensure(ctx, m->wbf, kNP * sizeof(unsigned short)));
   CHS_HIP(ctx, hipMemcpyAsync(m->theta.p, theta, kNP * sizeof(float), hipMemcpyHostToDevice,
                               ctx->stream));
+  ctx->h2d += kNP * sizeof(float);
   CHS_HIP(ctx, hipMemsetAsync(m->m.p, 0, kNP * sizeof(float), ctx->stream));
   CHS_HIP(ctx, hipMemsetAsync(m->v.p, 0, kNP * sizeof(float), ctx->stream));
   hipLaunchKernelGGL(k_mlp_to_bf16, dim3(ceil_div(kNP, 256)), dim3(256), 0, ctx->stream,
@@ -658,9 +659,12 @@ int chs_mlp_loss_grad(chs_context* ctx, chs_mlp* m, const float* d_feat, const f
   CHS_TRY(mlp_batch(ctx, m, d_feat, d_r, d_e, d_w, NB, band_len, (int)NB, 0, 1, dl));
   double h = 0.0;
   CHS_HIP(ctx, hipMemcpyAsync(&h, dl, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  if (grad)
+  ctx->d2h += sizeof(double);
+  if (grad) {
     CHS_HIP(ctx, hipMemcpyAsync(grad, m->grad.p, kNP * sizeof(float), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    ctx->d2h += kNP * sizeof(float);
+  }
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (loss) *loss = h;
   return CHS_OK;
@@ -688,9 +692,11 @@ int chs_mlp_train(chs_context* ctx, chs_mlp* m, const float* d_feat, const float
                        lr, bc1, bc2, (unsigned short*)m->wbf.p);
     CHS_LAUNCH_CHECK(ctx);
   }
-  if (loss_trace && steps > 0)
+  if (loss_trace && steps > 0) {
     CHS_HIP(ctx, hipMemcpyAsync(loss_trace, dtrace, steps * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
+    ctx->d2h += steps * sizeof(double);
+  }
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CHS_OK;
 }
@@ -789,6 +795,7 @@ int chs_mlp_pairs(chs_context* ctx, const double* d_psi, const double* d_psi_ref
   CHS_LAUNCH_CHECK(ctx);
   CHS_HIP(ctx, hipMemcpyAsync(hd.data(), dots, hd.size() * sizeof(z64), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += hd.size() * sizeof(z64);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   int used = 0;
   for (int b = 0; b < n_b; ++b) {

@@ -153,6 +153,7 @@ int filter_check(chs_context* ctx, const double* flag) {
   double h_flag[4];
   CHS_HIP(ctx, hipMemcpyAsync(h_flag, flag, 4 * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += 4 * sizeof(double);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (h_flag[0] != 0.0) {
     ctx->diag.iteration = (int)h_flag[0];
@@ -187,6 +188,7 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
   const double coef[2] = {b4[2], 1.0 / b4[3]};
   CHS_HIP(ctx, hipMemsetAsync(flag, 0, 8 * sizeof(double), ctx->stream));
   CHS_HIP(ctx, hipMemcpyAsync(d_coef, coef, sizeof(coef), hipMemcpyHostToDevice, ctx->stream));
+  ctx->h2d += sizeof(coef);
   void* a = im->ws1.p;
   void* b = im->ws2.p;
   CHS_HIP(ctx, hipMemcpyAsync(a, psi, bytes, hipMemcpyDeviceToDevice, ctx->stream));
@@ -534,6 +536,7 @@ __global__ void k_pattern(double* __restrict__ a, size_t n) {
 using namespace chs;
 
 extern "C" {
+
 
 int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,
                      double* bytes3, int32_t* dims4) {

@@ -515,12 +515,14 @@ int augment_density(chs_context* ctx, const z64* psi, int n_b, const double* d_o
   std::vector<double> hp(P);
   CHS_HIP(ctx, hipMemcpyAsync(hp.data(), rho_p, P * sizeof(double), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += P * sizeof(double);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<double> hra(n_atoms, 0.0);
   for (int a = 0; a < n_atoms; ++a)
     for (int j = 0; j < np; ++j) hra[a] += q0 * hp[a * np + j];
   CHS_HIP(ctx, hipMemcpyAsync(ra, hra.data(), n_atoms * sizeof(double), hipMemcpyHostToDevice,
                               ctx->stream));
+  ctx->h2d += n_atoms * sizeof(double);
   const size_t n3 = (size_t)n * n * n;
   hipLaunchKernelGGL(k_aug_g, dim3(ceil_div(n3, 256)), dim3(256), 0, ctx->stream, cgrid, n,
                      n_atoms, d_ph, ra, 2.0 * M_PI / L, rq, L * L * L);

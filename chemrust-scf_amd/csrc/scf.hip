@@ -248,6 +248,7 @@ int reduce2(chs_context* ctx, double* out2) {
                      dst);
   CHS_LAUNCH_CHECK(ctx);
   CHS_HIP(ctx, hipMemcpyAsync(out2, dst, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->d2h += 2 * sizeof(double);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return CHS_OK;
 }
@@ -262,6 +263,16 @@ double dot_real(chs_context* ctx, const double* a, const double* b, size_t n, in
   *rc = prep_reduce(ctx);
   if (*rc) return 0;
   hipLaunchKernelGGL(k_dot_real, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream, a, b, n,
+                     (double*)ctx->impl->part.p);
+  double o[2] = {0, 0};
+  *rc = reduce2(ctx, o);
+  return o[0];
+}
+
+double l1_diff(chs_context* ctx, const double* a, const double* b, size_t n, int* rc) {
+  *rc = prep_reduce(ctx);
+  if (*rc) return 0;
+  hipLaunchKernelGGL(k_abs_diff, dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream, a, b, n,
                      (double*)ctx->impl->part.p);
   double o[2] = {0, 0};
   *rc = reduce2(ctx, o);
@@ -515,6 +526,58 @@ double fermi_occupations(const double* eps, int n, double n_e, double kt, double
     occ[i] = fd(eps[i], mu);
     const double f = occ[i] / 2.0;
     if (f > 1e-300 && f < 1.0 - 1e-16) t += f * std::log(f) + (1.0 - f) * std::log(1.0 - f);
+  }
+  *ts = 2.0 * kt * t;
+  return mu;
+}
+
+// Occupations of states with weights w_i (the k-point weight of each
+// state): sum_i w_i f_i = n_e, -TS = 2 kT sum_i w_i [f ln f + (1-f) ln(1-f)]
+// (f = f_i / 2), mu by bisection (notes/plans/phase-2/TASKS.md:342-343).
+// kT = 0: states filled in ascending order, two electrons of weight w_i each.
+double fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                           double* occ, double* ts) {
+  *ts = 0;
+  double emin = eps[0], emax = eps[0];
+  for (int i = 1; i < n; ++i) {
+    emin = std::min(emin, eps[i]);
+    emax = std::max(emax, eps[i]);
+  }
+  if (kt <= 0) {
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return eps[a] < eps[b]; });
+    double left = n_e, mu = emin;
+    for (int i : ord) {
+      const double take = std::min(2.0, std::max(0.0, left / w[i]));
+      occ[i] = take;
+      left -= take * w[i];
+      if (take > 0) mu = eps[i];
+    }
+    return mu;
+  }
+  auto fd = [&](double e, double mu) {
+    const double x = (e - mu) / kt;
+    if (x > 200) return 0.0;
+    if (x < -200) return 2.0;
+    return 2.0 / (1.0 + std::exp(x));
+  };
+  double lo = emin - 50 * kt, hi = emax + 50 * kt;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += w[i] * fd(eps[i], mid);
+    const double nlo = s > n_e ? lo : mid, nhi = s > n_e ? mid : hi;
+    if (nlo == lo && nhi == hi) break;  // fixed point: later steps repeat it
+    lo = nlo;
+    hi = nhi;
+  }
+  const double mu = 0.5 * (lo + hi);
+  double t = 0;
+  for (int i = 0; i < n; ++i) {
+    occ[i] = fd(eps[i], mu);
+    const double f = occ[i] / 2.0;
+    if (f > 1e-300 && f < 1.0 - 1e-16) t += w[i] * (f * std::log(f) + (1.0 - f) * std::log(1.0 - f));
   }
   *ts = 2.0 * kt * t;
   return mu;

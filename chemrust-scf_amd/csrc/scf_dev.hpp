@@ -43,12 +43,18 @@ struct PulayDev {
 
 double dot_real(chs_context* ctx, const double* a, const double* b, size_t n, int* rc);
 int reduce2(chs_context* ctx, double* out2);
+// integral-free L1 distance sum_i |a_i - b_i| (FP64, fixed reduction order)
+double l1_diff(chs_context* ctx, const double* a, const double* b, size_t n, int* rc);
 double fermi_occupations(const double* eps, int n, double n_e, double kt, double* occ, double* ts);
+// Occupations of n states with weights w (k-point weights): sum w_i f_i = n_e
+double fermi_occupations_w(const double* eps, const double* w, int n, double n_e, double kt,
+                           double* occ, double* ts);
 // rho(r) = sum_b occ_b |psi_b(r)|^2 / omega for a c128 band-major block
+// (added to rho with `accumulate`)
 int density_impl(chs_context* ctx, const z64* psi, int n_b, const double* occ_host, double omega,
-                 double* d_rho);
+                 double* d_rho, bool accumulate = false);
 // the same from a complex64 block (FP64 accumulation; zero-window stick tables)
 int density_c64(chs_context* ctx, const c32* psi, int n_b, const double* occ_host, double omega,
-                double* d_rho);
+                double* d_rho, bool accumulate = false);
 
 }  // namespace chs

@@ -138,6 +138,7 @@ int chs_gsphere(chs_context* ctx, double L, double ecut, const double kfrac[3], 
   CHS_LAUNCH_CHECK(ctx);
   int32_t h_tot[2] = {0, 0};
   CHS_HIP(ctx, hipMemcpyAsync(h_tot, tot, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->d2h += sizeof(int32_t);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *n_pw = h_tot[0];
   if (!d_miller) return CHS_OK;
@@ -149,6 +150,7 @@ int chs_gsphere(chs_context* ctx, double L, double ecut, const double kfrac[3], 
   CHS_LAUNCH_CHECK(ctx);
   CHS_HIP(ctx, hipMemcpyAsync(h_tot + 1, tot + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
                               ctx->stream));
+  ctx->d2h += sizeof(int32_t);
   CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (h_tot[1]) return set_error(ctx, CHS_ERR_INVALID_ARG, "plane-wave sphere exceeds the %d^3 grid", n_grid);
   return CHS_OK;

@@ -101,10 +101,23 @@ int chs_pw_coords_to_fft_indices(chs_context* ctx, const int32_t* d_miller, int 
 int chs_gsphere(chs_context* ctx, double L, double ecut, const double kfrac[3], int n_grid,
                 int max_pw, int* n_pw, int32_t* d_miller, int32_t* d_idx, double* d_kin);
 
-/* ---- 3D FFT (BatchedFftPlan3d::{c2c_forward, c2c_inverse},
- * src/device/fft.rs:116-228): batched, in place, unnormalised, n^3 grids
- * stored back to back. sign = +1 inverse, -1 forward. ------------------- */
-int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int n, int batch, int sign);
+/* ---- 3D FFT (BatchedFftPlan3d, src/device/fft.rs:116-228) -----------
+ * plan_batched_c2c(nx, ny, nz, batch) + c2c_forward / c2c_inverse
+ * (:181-228): batched, in place, unnormalised, nx x ny x nz grids (x
+ * fastest) stored back to back; sign = +1 inverse (e^{+iG.r}), -1 forward.
+ * Each length one of 8, 12, 16, 18, 24, 32, 36, 48, 64, 72, 96, 108, 128,
+ * 144 (else CHS_ERR_FFT); batch <= 65535. */
+int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int nx, int ny, int nz,
+                  int batch, int sign);
+/* plan_batched_z2d + z2d (:127-174): complex-to-real inverse transform of
+ * Hermitian-symmetric grids, unnormalised (e^{+iG.r}). The non-redundant
+ * half is the fastest dimension: input (device c128) [nz][ny][nx/2 + 1]
+ * per batch, output (device f64) [nz][ny][nx] per batch. (The reference
+ * halves the dimension cuFFT stores fastest, which on its ix-fastest grids
+ * is x; its comment calls it nz, discrepancy D2.) Uses a context workspace
+ * of batch nx ny nz complex values. */
+int chs_fft3d_z2d(chs_context* ctx, const double* d_half, int nx, int ny, int nz, int batch,
+                  double* d_real);
 
 /* ---- Hamiltonian ------------------------------------------------------ */
 /* Bind the plane-wave set, grid and local potential used by the operator
@@ -252,18 +265,86 @@ typedef struct chs_scf_params {
 } chs_scf_params;
 
 typedef struct chs_scf_result {
-  int32_t n_iter, converged, n_pw, n_proj;
+  int32_t n_iter, converged, n_pw, n_proj;  /* n_pw: of the first k-point */
   double energy, mu, seconds;
   double t_filter, t_rr, t_density, t_veff, t_lanczos; /* seconds per phase */
   int32_t n_iter_fp32;     /* SCF iterations whose filter ran in complex64 */
+  double drho;             /* integral of |rho_out - rho_in| of the last iteration (electrons);
+                              the density criterion of src/scf.rs:357 */
 } chs_scf_result;
 
 /* Full SCF (run_scf, src/scf.rs:358-376, with the todo!() transitions
- * build_v_eff / construct_density / check / mix filled in): atom positions
- * (host f64, 3 * n_atoms), energies (host, max_iter), eps (host, n_bands),
- * rho (host, n^3, may be NULL). */
+ * build_v_eff / construct_density / check / mix filled in) of the synthetic
+ * cell of p at the Gamma point (p->kfrac): atom positions (host f64,
+ * 3 * n_atoms), energies (host, max_iter), eps (host, n_bands), rho (host,
+ * n^3, may be NULL). Convergence: |dE|/atom < etol on two consecutive
+ * iterations; with CHS_C64 both differences are between FP64 iterations. */
 int chs_scf_run(chs_context* ctx, const chs_scf_params* p, const double* pos, chs_scf_result* res,
                 double* energies, double* eps, double* rho);
+/* The same over n_k k-points (the k loop of ScfIteration::diagonalize,
+ * src/scf.rs:169-265): kfrac (host, 3 n_k, fractions of the reciprocal
+ * lattice vectors; p->kfrac is not read), weights kweight (host, n_k, sum 1).
+ * Every k-point has its own plane-wave sphere, start block (seed_psi),
+ * projectors and Lanczos bound; V_eff and screened D are shared; one Fermi
+ * level over all states; rho = sum_k w_k rho_k. With n_k > 1 the filters of
+ * all k-points run as one captured graph launch per pass (chs_kset_filter).
+ * Outputs: energies and drho (host, max_iter each, may be NULL), eps (host,
+ * n_k * n_bands, k-major), rho (host n^3, may be NULL). */
+int chs_scf_run_k(chs_context* ctx, const chs_scf_params* p, const double* pos, int n_k,
+                  const double* kfrac, const double* kweight, chs_scf_result* res,
+                  double* energies, double* drho, double* eps, double* rho);
+
+/* ---- caller-supplied system (scf_context_create / scf_run /
+ * scf_context_destroy, PROJECT_ROOT_PLAN.md:880-1029): the host hands in
+ * the cell, grid, start density and wavefunctions, the plane-wave sets and
+ * the projectors instead of a synthetic-cell recipe. Host arrays are copied
+ * to the device at create and may be freed afterwards. ------------------ */
+typedef struct chs_scf_system {
+  int32_t grid[3];          /* wave (= density) grid [nx, ny, nz]; the fused FFT kernels are
+                               cubic: nx = ny = nz, else CHS_ERR_NOT_IMPLEMENTED */
+  double real_lattice[9];   /* a1, a2, a3 as rows (bohr); a cubic cell a I (the reciprocal
+                               lattice is (2 pi / a) I), else CHS_ERR_NOT_IMPLEMENTED */
+  int32_t n_atoms;
+  const double* positions;  /* 3 n_atoms Cartesian (bohr) */
+  double n_electrons;
+  const double* v_ion;      /* n^3 local ionic potential on the grid (Ha, ix fastest) */
+  int32_t n_bands;
+  int32_t n_k;
+  const double* k_frac;     /* 3 n_k, fractions of the reciprocal lattice vectors */
+  const double* k_weight;   /* n_k, sum 1 */
+  const int32_t* n_pw;      /* n_k */
+  const int32_t* const* miller; /* per k: 3 n_pw[k] Miller triples [h, k, l] */
+  int32_t proj_per_atom;    /* 0, 1 (l = 0) or 4 (l = 0, 1); projectors atom-major */
+  const double* const* beta; /* per k: (proj_per_atom n_atoms) x n_pw[k] c128 (VnlBatchData,
+                               src/eigensolver/vnl_data.rs:25-89); NULL without projectors */
+  const double* d0;         /* proj_per_atom n_atoms unscreened diagonal D (Ha) */
+  double q0, q_rc;          /* augmentation Q_ij = q0 delta_ij, a normalised Gaussian of width
+                               q_rc on the projector's atom; q0 = 0: S = 1 */
+  const double* rho0;       /* n^3 start density, or NULL: uniform n_electrons / Omega */
+  const double* const* psi0; /* per k: n_bands x n_pw[k] c128 band-major, or NULL: the
+                               seeded start of seed_psi */
+} chs_scf_system;
+
+typedef struct chs_scf chs_scf;
+/* Uploads the system. solver: the solver fields of chs_scf_params
+ * (cheb_degree, first_passes, lanczos_k, pulay_depth, mix_alpha, kerker_q,
+ * etol, smear_kt, max_iter, precision, fp64_final, fp32_until, seed_psi,
+ * seed_lanczos); its cell fields are not read. */
+int chs_scf_context_create(chs_context* ctx, const chs_scf_system* sys,
+                           const chs_scf_params* solver, chs_scf** out);
+/* Runs the SCF from the context's current state (the start density and
+ * wavefunctions, or the state the last run left). Returns CHS_OK when
+ * converged, CHS_ERR_NOT_CONVERGED otherwise (as scf_run's -1); either way
+ * the final density and wavefunctions become the state of the next run.
+ * Outputs as chs_scf_run_k. */
+int chs_scf_context_run(chs_scf* s, chs_scf_result* res, double* energies, double* drho,
+                        double* eps, double* rho);
+/* Wavefunctions of k-point k (host, n_bands x n_pw[k] c128 band-major). */
+int chs_scf_get_psi(chs_scf* s, int k, double* psi);
+/* Frees the system. Destroying its chs_context first frees the device arrays
+ * and detaches the system: run and get_psi then return CHS_ERR_INVALID_ARG and
+ * destroy frees the host side only. */
+int chs_scf_context_destroy(chs_scf* s);
 
 /* ---- system set-up on the device (the construction run_scf performs
  * before its loop, src/scf.rs:358-376 with the synthetic cell of the

@@ -9,6 +9,11 @@
 
 using namespace orc;
 
+#include <omp.h>
+
+void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_max_threads(void) { return omp_get_max_threads(); }
+
 static ScfParams to_params(const orc_params* p) {
   ScfParams s;
   s.n_rep = p->n_rep;

@@ -26,6 +26,10 @@ typedef struct orc_params {
 } orc_params;
 
 void orc_default_params(orc_params* p);
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline's 1-thread
+ * and all-core timings) */
+void orc_set_num_threads(int n);
+int orc_get_max_threads(void);
 
 /* generator and setup */
 double orc_lcg_uniforms(uint64_t seed, int n, double* out);

@@ -44,6 +44,8 @@ def lib():
         vp = C.c_void_p
         sig = {
             "orc_default_params": (None, [C.POINTER(Params)]),
+            "orc_set_num_threads": (None, [C.c_int]),
+            "orc_get_max_threads": (C.c_int, []),
             "orc_lcg_uniforms": (C.c_double, [C.c_uint64, C.c_int, f64p]),
             "orc_lcg_normals": (None, [C.c_uint64, C.c_int, f64p]),
             "orc_synth_positions": (None, [C.c_int, C.c_double, C.c_double, C.c_uint64, f64p]),
@@ -112,6 +114,15 @@ def lib():
             fn.argtypes = args
         _LIB = L
     return _LIB
+
+
+def set_num_threads(n):
+    """OpenMP threads of the oracle's parallel loops."""
+    lib().orc_set_num_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_get_max_threads())
 
 
 def default_params(**kw):

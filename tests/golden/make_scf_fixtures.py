@@ -13,13 +13,17 @@ tests/test_gpu_golden.py.
   scf_8atom_kpts.npz       8-atom USPP cell, 2x2x2 Gamma-centred k-point set
                            {0, 1/2}^3 (8 k-points, uneven plane-wave counts),
                            full k-point SCF.
+  scf_64atom_kpts_first.npz  64-atom USPP cell (72^3, 160 bands, degree 12)
+                           on the same {0, 1/2}^3 set (n_pw 23,847-23,984),
+                           the first SCF iteration with one filter +
+                           Rayleigh-Ritz pass per k-point.
 
 Each file holds the energy of every iteration, the eigenvalues of the last
 one, the per-iteration density change, and the last output density on a
 coarse sub-grid (every `stride`-th point per axis) plus its integral, so the
 fixtures stay small. Run from the repository root after `make -C oracle`:
 
-  OMP_NUM_THREADS=8 python tests/golden/make_scf_fixtures.py [b] [c] [k]
+  OMP_NUM_THREADS=8 python tests/golden/make_scf_fixtures.py [b] [c] [k] [d]
 
 The 64-atom run takes about ten minutes on 8 cores, the 216-atom first
 iteration about forty.
@@ -43,8 +47,13 @@ CASES = {
 }
 # Gamma-centred 2x2x2 set {0, 1/2}^3, equal weights
 KPTS = np.array([[i, j, k] for i in (0.0, 0.5) for j in (0.0, 0.5) for k in (0.0, 0.5)])
-KCASE = ("scf_8atom_kpts.npz", dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10,
-                                   n_proj_per_atom=4, q0=0.25, max_iter=60), 1)
+KCASES = {
+    "k": ("scf_8atom_kpts.npz", dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10,
+                                     n_proj_per_atom=4, q0=0.25, max_iter=60), 1),
+    "d": ("scf_64atom_kpts_first.npz", dict(n_rep=2, n_grid=72, n_bands=160, cheb_degree=12,
+                                            n_proj_per_atom=4, q0=0.25, max_iter=1,
+                                            first_passes=1), 3),
+}
 
 
 def sub(rho, n, stride):
@@ -52,11 +61,11 @@ def sub(rho, n, stride):
 
 
 def main():
-    which = sys.argv[1:] or ["b", "c", "k"]
+    which = sys.argv[1:] or ["b", "c", "k", "d"]
     for w in which:
         t0 = time.time()
-        if w == "k":
-            name, kw, stride = KCASE
+        if w in KCASES:
+            name, kw, stride = KCASES[w]
             p = O.default_params(**kw)
             pos = O.positions(kw["n_rep"])
             r = O.scf_run_k(p, pos, KPTS, np.full(len(KPTS), 1.0 / len(KPTS)))

@@ -54,6 +54,46 @@ def test_fft3d_vs_numpy(ctx, n, dtype, tol):
     assert np.abs(back / n ** 3 - x).max() / np.abs(x).max() < tol
 
 
+@pytest.mark.parametrize("dims", [(36, 24, 16), (8, 72, 12), (18, 12, 108)])
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-13), (np.complex64, 2e-6)])
+def test_fft3d_non_cubic_vs_numpy(ctx, dims, dtype, tol):
+    """plan_batched_c2c(nx, ny, nz) (src/device/fft.rs:181-199) on grids with
+    three different edges, x fastest: forward = numpy fftn, inverse round
+    trip = N x."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(nx * ny + nz)
+    shape = (3, nz, ny, nx)
+    x = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+    g = chs.Gpu.from_host(ctx, x)
+    chs.fft3d_c2c(ctx, g, -1)
+    ref = np.fft.fftn(x.astype(np.complex128), axes=(1, 2, 3))
+    assert np.abs(g.sync_to_host() - ref).max() / np.abs(ref).max() < tol
+    chs.fft3d_c2c(ctx, g, +1)
+    assert np.abs(g.sync_to_host() / (nx * ny * nz) - x).max() / np.abs(x).max() < tol
+
+
+@pytest.mark.parametrize("dims", [(36, 36, 36), (24, 16, 12), (18, 36, 8)])
+def test_fft3d_z2d(ctx, dims):
+    """plan_batched_z2d + z2d (src/device/fft.rs:127-174): the unnormalised
+    inverse of a Hermitian-symmetric grid is real; equals numpy's irfftn
+    times N; a DC-only input gives a constant grid per batch (the
+    reference's z2d KAT, fft.rs:284-401)."""
+    nx, ny, nz = dims
+    n3 = nx * ny * nz
+    rng = np.random.default_rng(n3)
+    r = rng.standard_normal((2, nz, ny, nx))
+    half = np.fft.rfftn(r, axes=(1, 2, 3))  # (2, nz, ny, nx // 2 + 1)
+    out = chs.fft3d_z2d(ctx, chs.Gpu.from_host(ctx, half), nx).sync_to_host()
+    ref = np.fft.irfftn(half, s=(nz, ny, nx), axes=(1, 2, 3)) * n3
+    assert np.abs(out - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(out / n3 - r).max() < 1e-12 * np.abs(r).max()
+    dc = np.zeros((3, nz, ny, nx // 2 + 1), complex)
+    dc[:, 0, 0, 0] = [1.5, -2.0, 0.25]
+    got = chs.fft3d_z2d(ctx, chs.Gpu.from_host(ctx, dc), nx).sync_to_host()
+    for b, v in enumerate([1.5, -2.0, 0.25]):
+        assert np.abs(got[b] - v).max() < 1e-15
+
+
 def bind(ctx, m, veff=None):
     mil, idx, kin = m.pw()
     v = m.vion() if veff is None else veff
