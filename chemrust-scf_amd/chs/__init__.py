@@ -147,6 +147,7 @@ def lib():
                                        C.POINTER(vp)],
             "chs_scf_context_run": [vp, C.POINTER(ScfResult), dp, dp, dp, vp],
             "chs_scf_get_psi": [vp, i, vp],
+            "chs_initial_psi": [vp, vp, i, i, u64, vp],
             "chs_scf_context_destroy": [vp],
             "chs_profile_hpsi": [vp, i, i, i, dp, dp, vp],
             "chs_band_cg": [vp, vp, i, i, d, i, vp, dp, vp, vp],
@@ -638,6 +639,14 @@ def profile_gemm(ctx, precision, opa, opb, m, n, k, reps=5):
     ctx.check(lib().chs_profile_gemm(ctx.h, precision, opa.encode(), opb.encode(), m, n, k, reps,
                                      C.byref(ms)))
     return ms.value
+
+
+def initial_psi(ctx, kin, n_b, seed):
+    """The SCF's seeded start block (n_b, n_pw) for device kinetic energies
+    `kin` (the LCG + Box-Muller stream of the driver)."""
+    out = Gpu(ctx, (n_b, kin.shape[0]), np.complex128)
+    ctx.check(lib().chs_initial_psi(ctx.h, kin.ptr, kin.shape[0], n_b, seed, out.ptr))
+    return out
 
 
 def build_system(ctx, params, pos):

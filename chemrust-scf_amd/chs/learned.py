@@ -16,22 +16,6 @@ import numpy as np
 import chs
 
 
-def start_block(n_b, kin, seed):
-    """Complex N(0,1) / (1 + 2 kin) start vectors from the 48-bit LCG with
-    Box-Muller normals, the SCF driver's generator: per plane wave, the real
-    then the imaginary part, each from two uniforms."""
-    kin = np.asarray(kin)
-    n = n_b * kin.size * 4
-    st = int(seed) & 0xFFFFFFFFFFFF
-    u = np.empty(n)
-    for i in range(n):
-        st = (st * 25214903917 + 11) & 0xFFFFFFFFFFFF
-        u[i] = st / 281474976710656.0
-    z = np.sqrt(-2.0 * np.log(1.0 - u[0::2])) * np.cos(2.0 * np.pi * u[1::2])
-    x = z.reshape(n_b, kin.size, 2)
-    return (x[..., 0] + 1j * x[..., 1]) / (1.0 + 2.0 * kin)
-
-
 class Sample:
     """One seeded system bound on a context."""
 
@@ -68,7 +52,7 @@ class Sample:
 
     def converged(self, degree=16, max_passes=60, tol=1e-12):
         """psi* and eps* by plain subspace iteration to eigenvalue changes < tol."""
-        psi = chs.Gpu.from_host(self.ctx, start_block(self.n_b, self.kin, 1000 + self.seed))
+        psi = chs.initial_psi(self.ctx, self.sys["kin"], self.n_b, 1000 + self.seed)
         eps = self.chfsi(psi, 2, degree)
         for _ in range(max_passes):
             new = self.chfsi(psi, 1, degree, eps)
@@ -80,7 +64,7 @@ class Sample:
 
     def rough(self, passes=3, degree=8):
         """An unconverged block: `passes` low-degree passes from a fresh start."""
-        psi = chs.Gpu.from_host(self.ctx, start_block(self.n_b, self.kin, 2000 + self.seed))
+        psi = chs.initial_psi(self.ctx, self.sys["kin"], self.n_b, 2000 + self.seed)
         eps = self.chfsi(psi, passes, degree)
         return psi, eps
 
@@ -106,6 +90,18 @@ def collect_pairs(ctx, samples, pairs, passes=(2, 3), cg_steps=(2, 5), gap_min=1
                 b2 = copy_block(ctx, blk)
                 chs.band_cg_minimize(ctx, b2, k, 1e-12, precond=0)
                 used += chs.mlp_pairs(ctx, b2, ref, eps_ref, pairs, gap_min)
+    return used
+
+
+def collect_pairs_seeded(ctx, seeds, make, pairs, **kw):
+    """collect_pairs over samples built one at a time by make(seed) and
+    dropped after use (large cells: one sample's projectors on the device at
+    a time)."""
+    used = 0
+    for sd in seeds:
+        s = make(sd)
+        used += collect_pairs(ctx, [s], pairs, **kw)
+        del s
     return used
 
 

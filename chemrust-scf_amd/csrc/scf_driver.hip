@@ -308,6 +308,9 @@ static int build_from_caller(chs_context* ctx, const chs_scf_system* sy, const c
       CHS_HIP(ctx, hipMemcpy(K.psi.p, sy->psi0[k], ne * sizeof(z64), hipMemcpyHostToDevice));
       ctx->h2d += ne * sizeof(z64);
     } else {
+      // the kinetic energies come from k_kin_miller on the context stream
+      // (non-blocking: a null-stream copy would not wait for it)
+      CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
       std::vector<double> kin(n_pw);
       CHS_HIP(ctx, hipMemcpy(kin.data(), K.kin.p, n_pw * sizeof(double), hipMemcpyDeviceToHost));
       ctx->d2h += n_pw * sizeof(double);
@@ -660,6 +663,22 @@ int chs_density(chs_context* ctx, const double* d_psi, int n_b, const double* oc
   return density_impl(ctx, (const z64*)d_psi, n_b, occ, omega, d_rho);
 }
 
+int chs_initial_psi(chs_context* ctx, const double* d_kin, int n_pw, int n_b, uint64_t seed,
+                    double* d_psi) {
+  if (!ctx || !d_kin || !d_psi || n_pw <= 0 || n_b <= 0) return CHS_ERR_INVALID_ARG;
+  std::vector<double> kin(n_pw);
+  CHS_HIP(ctx, hipMemcpyAsync(kin.data(), d_kin, n_pw * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += n_pw * sizeof(double);
+  const size_t ne = (size_t)n_b * n_pw;
+  std::unique_ptr<double[]> h(new double[2 * ne]);
+  initial_psi_host(kin, n_b, seed, h.get());
+  CHS_HIP(ctx, hipMemcpy(d_psi, h.get(), ne * sizeof(z64), hipMemcpyHostToDevice));
+  ctx->h2d += ne * sizeof(z64);
+  return CHS_OK;
+}
+
 int chs_scf_run_k(chs_context* ctx, const chs_scf_params* p, const double* pos, int n_k,
                   const double* kfrac, const double* kweight, chs_scf_result* res,
                   double* energies, double* drho, double* eps, double* rho) {
@@ -715,6 +734,7 @@ int chs_scf_get_psi(chs_scf* s, int k, double* psi) {
   if (!s || !psi || !s->ctx || k < 0 || k >= (int)s->sys.ks.size()) return CHS_ERR_INVALID_ARG;
   const ScfKpt& K = s->sys.ks[k];
   const size_t bytes = (size_t)s->sys.n_b * K.n_pw * sizeof(z64);
+  CHS_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
   CHS_HIP(s->ctx, hipMemcpy(psi, K.psi.p, bytes, hipMemcpyDeviceToHost));
   s->ctx->d2h += bytes;
   return CHS_OK;

@@ -346,6 +346,15 @@ int chs_scf_get_psi(chs_scf* s, int k, double* psi);
  * destroy frees the host side only. */
 int chs_scf_context_destroy(chs_scf* s);
 
+/* The SCF's seeded start block: psi_{b,g} = (x + i y) / (1 + 2 kin_g) with
+ * x, y ~ N(0,1) from the reference's 48-bit LCG and Box-Muller
+ * (src/eigensolver/subspace_diag.rs:434-447), drawn band-major then
+ * plane-wave order; d_kin device f64 (n_pw), d_psi device c128 (n_b x n_pw).
+ * Generated on the host (by up to 16 threads with exact LCG jump-ahead) and
+ * uploaded. */
+int chs_initial_psi(chs_context* ctx, const double* d_kin, int n_pw, int n_b, uint64_t seed,
+                    double* d_psi);
+
 /* ---- system set-up on the device (the construction run_scf performs
  * before its loop, src/scf.rs:358-376 with the synthetic cell of the
  * tests, tests/phase0_gate1_consistency.rs:75-112,187-197) --------------

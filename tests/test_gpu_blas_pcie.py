@@ -40,9 +40,8 @@ def test_zdotc_kat(ctx):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(100_001) + 1j * rng.standard_normal(100_001)
     y = rng.standard_normal(100_001) + 1j * rng.standard_normal(100_001)
-    ctx.check(chs.lib().chs_zdotc(ctx.h, len(x), chs.Gpu.from_host(ctx, x).ptr,
-                                  chs.Gpu.from_host(ctx, y).ptr,
-                                  out.ctypes.data_as(dp)))
+    dx, dy = chs.Gpu.from_host(ctx, x), chs.Gpu.from_host(ctx, y)
+    ctx.check(chs.lib().chs_zdotc(ctx.h, len(x), dx.ptr, dy.ptr, out.ctypes.data_as(dp)))
     ref = np.vdot(x, y)
     assert abs(complex(*out) - ref) < 1e-11 * abs(ref)
 

@@ -14,6 +14,17 @@ from chs import learned
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("n_rep,nb,seed", [(1, 20, 1000), (2, 3, 2021)])
+def test_initial_psi_matches_oracle(ctx, n_rep, nb, seed):
+    """chs_initial_psi (host threads with LCG jump-ahead) gives the oracle's
+    sequential start block bit for bit."""
+    p = O.default_params(n_rep=n_rep, n_grid=36 * n_rep, n_bands=nb)
+    m = O.Model(p, O.positions(n_rep))
+    mil, idx, kin = m.pw()
+    got = chs.initial_psi(ctx, chs.Gpu.from_host(ctx, kin), nb, seed).sync_to_host()
+    assert np.array_equal(got, m.initial_psi(nb, seed))
+
+
 def _theta(seed=5):
     rng = np.random.default_rng(seed)
     th = M.init_params(11).astype(np.float64) + rng.standard_normal(M.N_PARAMS) * 0.05

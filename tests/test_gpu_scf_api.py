@@ -52,12 +52,13 @@ def test_context_api_matches_oracle(ctx, npa):
     # the density criterion beside |dE| (src/scf.rs:357): the oracle's
     # per-iteration integral of |rho_out - rho_in|
     assert np.abs(got["drho_trace"] - ref["drho"]).max() < 1e-6
-    # the run leaves its final state: a second run starts converged (its
-    # first iteration repeats the first-iteration passes, so two more
-    # iterations of |dE| below etol follow at once)
+    # the run leaves its final state (wavefunctions, last output density): a
+    # second run starts there and converges within a few iterations to the
+    # same fixed point (|dE| < etol = 1e-6 Ha/atom per step bounds how far
+    # either run's last energy sits from it)
     again = s.run()
     assert again["converged"] and again["n_iter"] <= 5
-    assert abs(again["energy"] - got["energy"]) / 8 < 1e-7
+    assert abs(again["energy"] - got["energy"]) / 8 < 1e-5
     psi = s.psi(0)
     assert psi.shape == (20, len(sysd["miller"][0]))
     s.close()
