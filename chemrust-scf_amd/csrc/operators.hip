@@ -2,6 +2,7 @@
 // the Lanczos upper bound; level-1 primitives.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "blas1.hpp"
@@ -531,11 +532,65 @@ __global__ void k_pattern(double* __restrict__ a, size_t n) {
   }
 }
 
+// ---- allocation-free dot product ---------------------------------------
+// Scratch of chs_zdotc_noalloc: kRedBlocks real then kRedBlocks imaginary
+// partial sums, then the two totals (514 doubles, 4,112 bytes). Every GPU the
+// library runs on holds its own copy until the process ends. Host code never
+// passes the array's name to a kernel or copy: the name is a host-side
+// stand-in, not a device address; it uses the per-device address that
+// hipGetSymbolAddress returned for that device (noalloc_scratch).
+static __device__ __align__(256) double scratch[2 * kRedBlocks + 2];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+// The context device's scratch address, looked up once per device with that
+// device current (chs_context_create calls this at set-up).
+int noalloc_scratch(chs_context* ctx, double** out) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices)
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "device %d beyond the scratch table", ctx->device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[ctx->device]) {
+    int cur = -1;
+    CHS_HIP(ctx, hipGetDevice(&cur));
+    if (cur != ctx->device) CHS_HIP(ctx, hipSetDevice(ctx->device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != ctx->device) (void)hipSetDevice(cur);
+    CHS_HIP(ctx, e);
+    if (!p) return set_error(ctx, CHS_ERR_DEVICE, "no device address for the dot-product scratch");
+    g_scratch_ptr[ctx->device] = (double*)p;
+  }
+  *out = g_scratch_ptr[ctx->device];
+  return CHS_OK;
+}
+
 }  // namespace chs
 
 using namespace chs;
 
 extern "C" {
+
+int chs_zdotc_noalloc(chs_context* ctx, int n, const double* d_x, const double* d_y,
+                      double out[2]) {
+  if (!ctx || n < 0 || !out || (n && (!d_x || !d_y))) return CHS_ERR_INVALID_ARG;
+  double* s = nullptr;
+  CHS_TRY(noalloc_scratch(ctx, &s));
+  // every launch writes all 2 kRedBlocks partials and both totals before they
+  // are read, so nothing carries over from an earlier call
+  hipLaunchKernelGGL((k_dot_part<double>), dim3(kRedBlocks), dim3(kRedThreads), 0, ctx->stream,
+                     (const z64*)d_x, (const z64*)d_y, (size_t)n, s);
+  CHS_LAUNCH_CHECK(ctx);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kRedThreads), 0, ctx->stream, (const double*)s,
+                     kRedBlocks, s + 2 * kRedBlocks);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_HIP(ctx, hipMemcpyAsync(out, s + 2 * kRedBlocks, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  CHS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->d2h += 2 * sizeof(double);
+  return CHS_OK;
+}
 
 
 int chs_profile_hpsi(chs_context* ctx, int precision, int n_b, int reps, double* ms3,

@@ -208,6 +208,16 @@ int chs_cgemm(chs_context* ctx, char opa, char opb, int m, int n, int k, const f
               const float* d_a, int lda, const float* d_b, int ldb, const float beta[2],
               float* d_c, int ldc);
 int chs_zdotc(chs_context* ctx, int n, const double* d_x, const double* d_y, double out[2]);
+/* chs_zdotc that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Its partial sums
+ * live in a fixed device array of 514 doubles (4,112 bytes) compiled into
+ * the library, one copy per GPU, held until the process ends; the partials
+ * do not grow with n, so every n >= 0 is accepted. Calls on one GPU share
+ * that array (each launch rewrites all of it before reading it): calls that
+ * overlap on one GPU, from contexts on different streams, race. */
+int chs_zdotc_noalloc(chs_context* ctx, int n, const double* d_x, const double* d_y,
+                      double out[2]);
 int chs_zaxpy(chs_context* ctx, int n, const double alpha[2], const double* d_x, double* d_y);
 int chs_zscal(chs_context* ctx, int n, const double alpha[2], double* d_x);
 
