@@ -301,8 +301,8 @@ def config_e_report(chs, ctx, n_train=50, n_held=2, n_bands=64, train_steps=2000
     pairs from plain runs (fixed V_ion, Chebyshev-filtered subspace iteration
     to convergence) of n_train seeded displacement samples (seeds 1000 +),
     the lowest n_bands bands each; judged on held-out samples (seeds 2000 +)
-    by band-CG refinement after ChFSI (three degree-8 filter + Rayleigh-Ritz
-    passes): steps to |d eps| < 1e-9 and the eigenvalue error against the
+    by band-CG refinement after ChFSI (six degree-12 filter + Rayleigh-Ritz
+    passes from the seeded start): steps to |d eps| < 1e-9 and the eigenvalue error against the
     converged FP64 eigenvalues, TPA with the USPP term against the network
     (whose table replaces the TPA diagonal, the USPP term kept)."""
     from chs import learned
@@ -311,10 +311,10 @@ def config_e_report(chs, ctx, n_train=50, n_held=2, n_bands=64, train_steps=2000
     probe = make(999)
     npw = probe.sys["n_pw"]
     del probe
-    sets = 4  # two rough blocks, each also after 3 band-CG (TPA) steps
+    sets = 4  # two ChFSI blocks, each also after 3 band-CG (TPA) steps
     pairs = chs.mlp_pair_buffers(ctx, n_train * sets * n_bands * npw)
     vpairs = chs.mlp_pair_buffers(ctx, n_held * sets * n_bands * npw)
-    kw = dict(passes=(2, 3), cg_steps=(3,))
+    kw = dict(passes=(5, 7), cg_steps=(3,), degree=12)
     used = learned.collect_pairs_seeded(ctx, range(1000, 1000 + n_train), make, pairs, **kw)
     vused = learned.collect_pairs_seeded(ctx, range(2000, 2000 + n_held), make, vpairs, **kw)
     ctx.synchronize()
@@ -330,7 +330,8 @@ def config_e_report(chs, ctx, n_train=50, n_held=2, n_bands=64, train_steps=2000
     evals = []
     for sd in range(2000, 2000 + n_held):
         smp = make(sd)
-        evals.append(learned.evaluate(ctx, smp, net, tol=1e-9, max_steps=300, uspp_term=True))
+        evals.append(learned.evaluate(ctx, smp, net, tol=1e-9, max_steps=300, uspp_term=True,
+                                      passes=6, degree=12))
         del smp
     t3 = time.perf_counter()
     tot = lambda k, f: sum(e[k][f] for e in evals)
@@ -350,11 +351,12 @@ def config_e_report(chs, ctx, n_train=50, n_held=2, n_bands=64, train_steps=2000
     }
 
 
-def run_form(chs, ctx, cfg, precision, steps, warmup, kpts=None):
+def run_form(chs, ctx, cfg, precision, steps, warmup, kpts=None, refine=0):
     """Warm-up and timed SCF runs of one precision form; returns the runs."""
     pos = chs.synth_positions(cfg["n_rep"])
     prec = chs.C128 if precision == "c128" else chs.C64
-    params = chs.scf_params(**cfg, max_iter=100, precision=prec)
+    params = chs.scf_params(**cfg, max_iter=100, precision=prec,
+                            fp64_refine=refine if prec == chs.C64 and not kpts else 0)
 
     def run():
         if kpts:
@@ -392,6 +394,10 @@ def main():
                          "FP64 form's time and the ratio beside it (the default for config C)")
     ap.add_argument("--no-compare", action="store_true",
                     help="config C: time only the form given by --precision")
+    ap.add_argument("--refine", default="residual", choices=["residual", "c128"],
+                    help="complex64 form: the filter of the final FP64 iterations, the "
+                         "residual-based complex64 filter with FP64 residual (R-ChFSI) or the "
+                         "complex128 filter (k-point sets always use the latter)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner report")
     ap.add_argument("--hpsi-only", action="store_true",
@@ -437,7 +443,8 @@ def main():
         return
     ctx.synchronize()
     t0 = time.perf_counter()
-    runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup, kpts)
+    refine = 1 if args.refine == "residual" and not kpts else 0
+    runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup, kpts, refine)
     ctx.synchronize()
     t1 = time.perf_counter()
     value = sum(r["seconds"] for r in runs) / len(runs)
@@ -454,10 +461,12 @@ def main():
                    "projectors": last["n_proj"], "filter_precision": precision,
                    "measured_form": (
                        "complex64 Chebyshev filter (FP32 projector products on the matrix "
-                       "cores) with FP64 Rayleigh-Ritz, density and energies, the last SCF "
-                       "iterations with the complex128 filter; the learned preconditioner is "
-                       "not in the timed SCF (reported separately)") if precision == "c64"
-                   else "complex128 throughout",
+                       "cores) with FP64 Rayleigh-Ritz, density and energies; the last SCF "
+                       "iterations with " + ("the residual-based complex64 filter and its FP64 "
+                                             "residual (R-ChFSI)" if refine else
+                                             "the complex128 filter") +
+                       "; the learned preconditioner is not in the timed SCF (reported "
+                       "separately)") if precision == "c64" else "complex128 throughout",
                    "learned_in_timed_run": False,
                    **({"k_points": len(kpts), "filter_graph_launches_per_pass": 1}
                       if kpts else {}),

@@ -67,7 +67,7 @@ class ScfParams(C.Structure):
         ("mix_alpha", C.c_double), ("kerker_q", C.c_double), ("etol", C.c_double),
         ("smear_kt", C.c_double),
         ("max_iter", C.c_int32), ("precision", C.c_int32), ("fp64_final", C.c_int32),
-        ("fp32_until", C.c_double),
+        ("fp32_until", C.c_double), ("fp64_refine", C.c_int32),
     ]
 
 
@@ -113,6 +113,7 @@ def lib():
             "chs_diag_get": [vp, C.POINTER(Diag)],
             "chs_transfer_bytes": [vp, C.POINTER(u64), C.POINTER(u64)],
             "chs_transfer_reset": [vp],
+            "chs_tuning_entries": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
             "chs_alloc": [vp, sz, C.POINTER(vp)],
             "chs_free": [vp, vp],
             "chs_h2d": [vp, vp, vp, sz],
@@ -123,6 +124,7 @@ def lib():
             "chs_gsphere": [vp, d, d, dp, i, i, C.POINTER(i), vp, vp, vp],
             "chs_fft3d_c2c": [vp, i, vp, i, i, i, i, i],
             "chs_fft3d_z2d": [vp, vp, i, i, i, i, vp],
+            "chs_downsample_to_wave_grid": [vp, vp, C.POINTER(i), C.POINTER(i), vp],
             "chs_bind_hamiltonian": [vp, i, i, vp, vp, vp],
             "chs_set_veff": [vp, vp],
             "chs_bind_projectors": [vp, i, vp, vp, vp],
@@ -132,6 +134,7 @@ def lib():
             "chs_grid_lambda_max": [vp, d, dp],
             "chs_lanczos_bound": [vp, i, u64, dp, dp],
             "chs_chebyshev_filter": [vp, i, vp, i, i, dp, vp],
+            "chs_chebyshev_filter_residual": [vp, i, vp, i, dp, i, dp, vp],
             "chs_rayleigh_ritz": [vp, vp, i, dp, vp],
             "chs_zhegvd": [vp, i, vp, vp, dp],
             "chs_cholqr": [vp, vp, i],
@@ -225,6 +228,12 @@ class Context:
     def transfer_reset(self):
         self.check(lib().chs_transfer_reset(self.h))
 
+    def tuning_entries(self):
+        """(H|psi> launch shapes, GEMM variants, density workgroup sizes) held."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.check(lib().chs_tuning_entries(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
 
 class Gpu:
     """Device array owned by the caller (Gpu<T>, src/device/mod.rs:50-58)."""
@@ -304,6 +313,18 @@ def fft3d_c2c(ctx, grid, sign):
     batch, nz, ny, nx = grid.shape
     ctx.check(lib().chs_fft3d_c2c(ctx.h, precision_of(grid.dtype), grid.ptr, nx, ny, nz, batch,
                                   sign))
+
+
+def downsample_to_wave_grid(ctx, fine, wave_dims):
+    """downsample_array_to_wave_grid (src/scf.rs:413-454): device f64 array of
+    shape (nz, ny, nx) on the fine grid -> a new device array on the wave
+    grid wave_dims = (wx, wy, wz)."""
+    fz, fy, fx = fine.shape
+    fd = (C.c_int * 3)(fx, fy, fz)
+    wd = (C.c_int * 3)(*wave_dims)
+    out = Gpu(ctx, (wave_dims[2], wave_dims[1], wave_dims[0]), np.float64)
+    ctx.check(lib().chs_downsample_to_wave_grid(ctx.h, fine.ptr, fd, wd, out.ptr))
+    return out
 
 
 def fft3d_z2d(ctx, half, nx):
@@ -392,6 +413,17 @@ def chebyshev_filter(ctx, psi, degree, bounds, out=None):
     return out
 
 
+def chebyshev_filter_residual(ctx, x, eps, degree, bounds, precision=C64, out=None):
+    """Residual-based filter (R-ChFSI) of the c128 Rayleigh-Ritz block x with
+    eigenvalues eps; recurrence in `precision`, residual and assembly FP64."""
+    out = out if out is not None else Gpu(ctx, x.shape, np.complex128)
+    b4 = np.ascontiguousarray(bounds, np.float64)
+    e = np.ascontiguousarray(eps, np.float64)
+    ctx.check(lib().chs_chebyshev_filter_residual(ctx.h, precision, x.ptr, x.shape[0], _dptr(e),
+                                                  degree, _dptr(b4), out.ptr))
+    return out
+
+
 def zgemm(ctx, opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc):
     """BlasHandle::gemm_c64 (src/device/blas.rs:24-61), column-major."""
     al = np.array([complex(alpha).real, complex(alpha).imag])
@@ -471,7 +503,7 @@ def scf_params(**kw):
              n_proj_per_atom=0, beta_rc=1.3, q_rc=1.2, q0=0.0, d_min=0.5, d_max=2.0, seed_d=3,
              seed_psi=2, seed_lanczos=4, cheb_degree=10, first_passes=4, lanczos_k=10,
              pulay_depth=6, mix_alpha=0.5, kerker_q=0.8, etol=1e-6, smear_kt=0.01,
-             max_iter=60, precision=C128, fp64_final=2, fp32_until=0.0)
+             max_iter=60, precision=C128, fp64_final=2, fp32_until=0.0, fp64_refine=0)
     d.update(kw)
     for k, v in d.items():
         if k == "kfrac":

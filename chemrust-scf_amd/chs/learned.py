@@ -75,16 +75,16 @@ def copy_block(ctx, psi):
     return out
 
 
-def collect_pairs(ctx, samples, pairs, passes=(2, 3), cg_steps=(2, 5), gap_min=1e-3):
+def collect_pairs(ctx, samples, pairs, passes=(2, 3), cg_steps=(2, 5), gap_min=1e-3, degree=8):
     """Training pairs of every sample: its early subspace-iteration blocks
-    (`passes` passes of degree 8) and band-CG (TPA) from them after each of
-    cg_steps steps."""
+    (`passes` passes of degree `degree`) and band-CG (TPA) from them after
+    each of cg_steps steps."""
     used = 0
     for s in samples:
         s.bind()
         ref, eps_ref = s.converged()
         for p in passes:
-            blk, _ = s.rough(p)
+            blk, _ = s.rough(p, degree)
             used += chs.mlp_pairs(ctx, blk, ref, eps_ref, pairs, gap_min)
             for k in cg_steps:
                 b2 = copy_block(ctx, blk)
@@ -105,13 +105,13 @@ def collect_pairs_seeded(ctx, seeds, make, pairs, **kw):
     return used
 
 
-def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False):
-    """Band-CG from the same rough block with TPA and with the network.
-    Returns dict with total steps, converged bands and max |eps - eps*| for
-    both."""
+def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False, passes=3, degree=8):
+    """Band-CG from the same ChFSI block (`passes` passes of degree `degree`)
+    with TPA and with the network. Returns dict with total steps, converged
+    bands and max |eps - eps*| for both."""
     sample.bind()
     ref, eps_ref = sample.converged()
-    blk, _ = sample.rough()
+    blk, _ = sample.rough(passes, degree)
     out = {}
     for name in ("tpa", "mlp"):
         b = copy_block(ctx, blk)

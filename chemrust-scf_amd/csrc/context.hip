@@ -4,9 +4,97 @@
 #include <cstdarg>
 #include <cstring>
 
+#include <dlfcn.h>
+
+#include <fstream>
+#include <mutex>
+#include <sstream>
+
 #include "impl.hpp"
 
 namespace chs {
+
+// ---- on-disk tuning cache ------------------------------------------------
+// Header "chs-tune 1 <library hash> <device>": entries count only for the
+// library build (FNV-1a of the loaded .so) and device (architecture and CU
+// count) that wrote them; another header means the file is rewritten.
+static std::mutex g_tune_mu;
+
+static std::string library_hash() {
+  static std::string h;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    Dl_info info{};
+    uint64_t f = 1469598103934665603ull;
+    if (dladdr((void*)&chs_context_create, &info) && info.dli_fname) {
+      std::ifstream in(info.dli_fname, std::ios::binary);
+      char buf[1 << 16];
+      while (in.read(buf, sizeof(buf)) || in.gcount() > 0) {
+        for (std::streamsize i = 0; i < in.gcount(); ++i) {
+          f ^= (unsigned char)buf[i];
+          f *= 1099511628211ull;
+        }
+      }
+    }
+    char hex[17];
+    snprintf(hex, sizeof(hex), "%016llx", (unsigned long long)f);
+    h = hex;
+  });
+  return h;
+}
+
+static std::string tune_header(chs_context* ctx) {
+  hipDeviceProp_t prop{};
+  std::string dev = "unknown";
+  if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess)
+    dev = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+  for (char& c : dev)
+    if (c == ' ') c = '_';
+  return "chs-tune 1 " + library_hash() + " " + dev;
+}
+
+void tune_cache_load(chs_context* ctx) {
+  auto* im = ctx->impl;
+  if (im->tune_path.empty()) return;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  std::ifstream in(im->tune_path);
+  std::string line;
+  if (!in || !std::getline(in, line) || line != tune_header(ctx)) return;
+  while (std::getline(in, line)) {
+    std::istringstream ss(line);
+    std::string kind;
+    ss >> kind;
+    if (kind == "vloc") {
+      int n, prec, sb, div, thr;
+      if (ss >> n >> prec >> sb >> div >> thr) im->vloc_tune[{n, prec}] = VlocTune{sb, div, thr};
+    } else if (kind == "gemm") {
+      GemmKey k{};
+      int choice;
+      if (ss >> k.tsize >> k.oa >> k.ob >> k.m >> k.n >> k.k >> choice) im->gemm_tune[k] = choice;
+    } else if (kind == "dens") {
+      int key, tpb;
+      if (ss >> key >> tpb) im->density_tpb[key] = tpb;
+    }
+  }
+}
+
+void tune_cache_store(chs_context* ctx, const std::string& line) {
+  auto* im = ctx->impl;
+  if (im->tune_path.empty()) return;
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const std::string head = tune_header(ctx);
+  std::string first;
+  {
+    std::ifstream in(im->tune_path);
+    if (in) std::getline(in, first);
+  }
+  if (first != head) {
+    std::ofstream out(im->tune_path, std::ios::trunc);
+    out << head << "\n";
+  }
+  std::ofstream out(im->tune_path, std::ios::app);
+  out << line << "\n";
+}
 
 int set_error(chs_context* ctx, int code, const char* fmt, ...) {
   char buf[1024];
@@ -96,6 +184,7 @@ int chs_context_create(int device, chs_context** out) {
   if (const char* ht = std::getenv("CHS_HPSI_TUNE")) ctx->impl->hpsi_autotune = !(ht[0] == '0');
   if (const char* pw = std::getenv("CHS_PLANE_WINDOW")) ctx->impl->plane_window = !(pw[0] == '0');
   if (const char* g3 = std::getenv("CHS_GEMM_FP16X3")) ctx->impl->use_h3 = g3[0] && g3[0] != '0';
+  if (const char* tc = std::getenv("CHS_TUNE_CACHE")) ctx->impl->tune_path = tc;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev0) != hipSuccess ||
       hipEventCreate(&ctx->impl->ev1) != hipSuccess) {
@@ -103,6 +192,7 @@ int chs_context_create(int device, chs_context** out) {
     delete ctx;
     return CHS_ERR_DEVICE;
   }
+  tune_cache_load(ctx);
   *out = ctx;
   return CHS_OK;
 }
@@ -116,7 +206,7 @@ int chs_context_destroy(chs_context* ctx) {
   im->scf_children.clear();
   for (auto& kv : im->twiddles) (void)hipFree(kv.second);
   DevBuf* bufs[] = {&im->stick_buf, &im->part, &im->small, &im->ws1, &im->ws2, &im->ws3,
-                    &im->ws4, &im->ws5, &im->ws6, &im->rr32, &im->lanczos_v0, &im->gemm_ws, &im->gemv_ws, &im->h3_bws, &im->lin_ws, &im->scf_ws,
+                    &im->ws4, &im->ws5, &im->ws6, &im->rr32, &im->res64, &im->res32, &im->res_br, &im->res_tab, &im->lanczos_v0, &im->gemm_ws, &im->gemv_ws, &im->h3_bws, &im->lin_ws, &im->scf_ws,
                     &im->proj_ws, &im->chol_ws, &im->eig_ws, &im->mlp_ws, &im->fft_ws, &im->kset_ws,
                     &im->gemm_tune_ws, &im->gemm_tune_c};
   for (DevBuf* b : bufs) release(*b);
@@ -153,6 +243,14 @@ int chs_transfer_bytes(chs_context* ctx, uint64_t* h2d, uint64_t* d2h) {
   if (!ctx) return CHS_ERR_INVALID_ARG;
   if (h2d) *h2d = ctx->h2d;
   if (d2h) *d2h = ctx->d2h;
+  return CHS_OK;
+}
+
+int chs_tuning_entries(chs_context* ctx, int* n_hpsi, int* n_gemm, int* n_density) {
+  if (!ctx) return CHS_ERR_INVALID_ARG;
+  if (n_hpsi) *n_hpsi = (int)ctx->impl->vloc_tune.size();
+  if (n_gemm) *n_gemm = (int)ctx->impl->gemm_tune.size();
+  if (n_density) *n_density = (int)ctx->impl->density_tpb.size();
   return CHS_OK;
 }
 

@@ -113,6 +113,37 @@ __global__ void k_real_of(const z64* __restrict__ a, double* __restrict__ r, siz
   if (i < n) r[i] = a[i].x;
 }
 
+__global__ void k_real_to_complex(const double* __restrict__ r, z64* __restrict__ c, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = mk(r[i], 0.0);
+}
+
+// Wave-grid point (ix, iy, iz) takes the fine-grid coefficient of its
+// frequency: f = i for i <= n/2, i - n above (an even wave grid's Nyquist
+// plane takes the fine grid's +n/2 coefficient), wrapped onto the fine grid.
+__device__ __forceinline__ int wave_to_fine(int i, int n, int m) {
+  const int f = i <= n / 2 ? i : i - n;
+  return f >= 0 ? f : m + f;
+}
+
+__global__ void k_wave_gather(const z64* __restrict__ fine, int fx, int fy, int fz, int wx, int wy,
+                              int wz, z64* __restrict__ wave) {
+  const size_t n = (size_t)wx * wy * wz;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ix = (int)(i % wx), iy = (int)((i / wx) % wy), iz = (int)(i / ((size_t)wx * wy));
+  const int jx = wave_to_fine(ix, wx, fx), jy = wave_to_fine(iy, wy, fy),
+            jz = wave_to_fine(iz, wz, fz);
+  wave[i] = fine[(size_t)jx + (size_t)fx * (jy + (size_t)fy * jz)];
+}
+
+// out = Re(a) / n_fine
+__global__ void k_real_scaled(const z64* __restrict__ a, double s, double* __restrict__ r,
+                              size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = a[i].x * s;
+}
+
 }  // namespace chs
 
 extern "C" {
@@ -147,6 +178,37 @@ int chs_fft3d_z2d(chs_context* ctx, const double* d_half, int nx, int ny, int nz
   CHS_TRY(fft3d_xyz(ctx, CHS_C128, full, dims, batch, 1));
   hipLaunchKernelGGL(k_real_of, dim3(ceil_div(n3 * batch, 256)), dim3(256), 0, ctx->stream, full,
                      d_real, n3 * batch);
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int chs_downsample_to_wave_grid(chs_context* ctx, const double* d_fine, const int fine_dims[3],
+                                const int wave_dims[3], double* d_wave) {
+  if (!ctx || !d_fine || !d_wave || !fine_dims || !wave_dims) return CHS_ERR_INVALID_ARG;
+  using namespace chs;
+  for (int a = 0; a < 3; ++a) {
+    if (!fft_length_supported(fine_dims[a]) || !fft_length_supported(wave_dims[a]))
+      return set_error(ctx, CHS_ERR_FFT, "unsupported FFT length");
+    if (wave_dims[a] > fine_dims[a])
+      return set_error(ctx, CHS_ERR_INVALID_ARG, "wave grid larger than the fine grid");
+  }
+  const size_t nf = (size_t)fine_dims[0] * fine_dims[1] * fine_dims[2];
+  const size_t nw = (size_t)wave_dims[0] * wave_dims[1] * wave_dims[2];
+  auto* im = ctx->impl;
+  CHS_TRY(ensure(ctx, im->fft_ws, (nf + nw) * sizeof(z64)));
+  z64* g = (z64*)im->fft_ws.p;
+  z64* w = g + nf;
+  hipLaunchKernelGGL(k_real_to_complex, dim3(ceil_div(nf, 256)), dim3(256), 0, ctx->stream, d_fine,
+                     g, nf);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(fft3d_xyz(ctx, CHS_C128, g, fine_dims, 1, -1));
+  hipLaunchKernelGGL(k_wave_gather, dim3(ceil_div(nw, 256)), dim3(256), 0, ctx->stream,
+                     (const z64*)g, fine_dims[0], fine_dims[1], fine_dims[2], wave_dims[0],
+                     wave_dims[1], wave_dims[2], w);
+  CHS_LAUNCH_CHECK(ctx);
+  CHS_TRY(fft3d_xyz(ctx, CHS_C128, w, wave_dims, 1, +1));
+  hipLaunchKernelGGL(k_real_scaled, dim3(ceil_div(nw, 256)), dim3(256), 0, ctx->stream,
+                     (const z64*)w, 1.0 / (double)nf, d_wave, nw);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
 }

@@ -637,6 +637,10 @@ static int gemm_variant(chs_context* ctx, int oa, int ob, int m, int n, int k, c
     }
   }
   im->gemm_tune[key] = *choice;
+  tune_cache_store(ctx, "gemm " + std::to_string(key.tsize) + " " + std::to_string(key.oa) + " " +
+                            std::to_string(key.ob) + " " + std::to_string(m) + " " +
+                            std::to_string(n) + " " + std::to_string(k) + " " +
+                            std::to_string(*choice));
   return CHS_OK;
 }
 

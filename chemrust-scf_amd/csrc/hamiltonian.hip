@@ -187,7 +187,9 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
                                                    cplx<T>* out, const cplx<T>* aux0, T c, T inv_e,
                                                    const double* __restrict__ coef, T inv_n3,
                                                    double* __restrict__ part,
-                                                   const cplx<T>* __restrict__ tw) {
+                                                   const cplx<T>* __restrict__ tw,
+                                                   const cplx<T>* __restrict__ resw,
+                                                   const double* __restrict__ ress) {
   constexpr int LS = N + 1;
   __shared__ cplx<T> lds[SB * LS];
   __shared__ double red[2][4];
@@ -226,6 +228,12 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
     } else {
       const cplx<T> q = aux0[boff + gi];
       o = mk((T)2 * (h.x - c * p.x) * inv_e - q.x, (T)2 * (h.y - c * p.y) * inv_e - q.y);
+      if (MODE == EPI_CHEBK_RES) {
+        // residual recurrence: + (2/e) T_k(x_b) W_b, the band's scale in ress
+        const cplx<T> w = resw[boff + gi];
+        const T sb = (T)ress[b];
+        o = mk(o.x + sb * w.x, o.y + sb * w.y);
+      }
     }
     out[boff + gi] = o;
     so += (double)o.x * o.x + (double)o.y * o.y;
@@ -524,7 +532,7 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
                        const cplx<T>* psi, cplx<T>* sbuf, const T* veff, const cplx<T>* tw,
                        const double* kin, cplx<T>* out, const cplx<T>* aux0, int mode, double c,
                        double inv_e, const double* coef, double inv_n3, double* part,
-                       hipEvent_t* ev) {
+                       hipEvent_t* ev, const cplx<T>* resw, const double* ress) {
   const int gz = ceil_div(sa.n_st, SB);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
   hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
@@ -566,11 +574,14 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
 #define CHS_FWD(M)                                                                            \
   hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \
-                     sbuf, sa, psi, kin, out, aux0, (T)c, (T)inv_e, coef, (T)inv_n3, part, tw)
+                     sbuf, sa, psi, kin, out, aux0, (T)c, (T)inv_e, coef, (T)inv_n3, part, tw,  \
+                     resw, ress)
   if (mode == EPI_HPSI)
     CHS_FWD(EPI_HPSI);
   else if (mode == EPI_CHEB1)
     CHS_FWD(EPI_CHEB1);
+  else if (mode == EPI_CHEBK_RES)
+    CHS_FWD(EPI_CHEBK_RES);
   else
     CHS_FWD(EPI_CHEBK);
 #undef CHS_FWD
@@ -585,7 +596,8 @@ static bool sb_fits(int n, size_t csz, int sb) { return (size_t)sb * (n + 1) * c
 template <typename T>
 static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, int n_b, void* out_v,
                            int mode, const void* aux0_v, double c, double inv_e,
-                           const double* coef, double* part, int* n_part, int reps, double* ms) {
+                           const double* coef, double* part, int* n_part, int reps, double* ms,
+                           const void* resw_v = nullptr, const double* ress = nullptr) {
   Hamiltonian& H = ctx->impl->ham;
   const StickTable& st = H.st;
   const int n = H.n_grid;
@@ -600,6 +612,9 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   const C* psi = (const C*)psi_v;
   C* out = (C*)out_v;
   const C* aux0 = (const C*)aux0_v;
+  const C* resw = (const C*)resw_v;
+  if (mode == EPI_CHEBK_RES && (!resw || !ress))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "residual epilogue without its residual");
   const double inv_n3 = 1.0 / ((double)n * n * n);
   VlocTune tn = get_tune(ctx, n, precision);
   if (!sb_fits(n, sizeof(C), tn.sb)) tn.sb = 16;
@@ -618,13 +633,13 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
       hipEvent_t* evp = ms ? ev : nullptr;
       if (tn.sb == 16)
         CHS_TRY((vloc_launch<NN, T, 16>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
-                                        mode, c, inv_e, coef, inv_n3, part, evp)));
+                                        mode, c, inv_e, coef, inv_n3, part, evp, resw, ress)));
       else if (tn.sb == 64)
         CHS_TRY((vloc_launch<NN, T, 64>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
-                                        mode, c, inv_e, coef, inv_n3, part, evp)));
+                                        mode, c, inv_e, coef, inv_n3, part, evp, resw, ress)));
       else
         CHS_TRY((vloc_launch<NN, T, 32>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
-                                        mode, c, inv_e, coef, inv_n3, part, evp)));
+                                        mode, c, inv_e, coef, inv_n3, part, evp, resw, ress)));
     });
     if (ms) {
       CHS_HIP(ctx, hipEventSynchronize(ev[3]));
@@ -644,12 +659,12 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
 
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
                   const void* aux0, const double* d_coef, double c, double inv_e,
-                  double* d_norm_part, int* n_part) {
+                  double* d_norm_part, int* n_part, const void* res_w, const double* res_s) {
   if (precision == CHS_C128)
     return vloc_pipeline_t<double>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_coef,
-                                   d_norm_part, n_part, 1, nullptr);
+                                   d_norm_part, n_part, 1, nullptr, res_w, res_s);
   return vloc_pipeline_t<float>(ctx, precision, psi, n_b, out, mode, aux0, c, inv_e, d_coef,
-                                d_norm_part, n_part, 1, nullptr);
+                                d_norm_part, n_part, 1, nullptr, res_w, res_s);
 }
 
 // The three kernels of one steady-state Chebyshev step `reps` times with HIP
@@ -695,6 +710,9 @@ int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, voi
       }
   }
   ctx->impl->vloc_tune[key] = best;
+  tune_cache_store(ctx, "vloc " + std::to_string(n) + " " + std::to_string(precision) + " " +
+                            std::to_string(best.sb) + " " + std::to_string(best.nrb_div) + " " +
+                            std::to_string(best.threads));
   if (best_out) *best_out = best;
   if (best_ms) *best_ms = best_t;
   return CHS_OK;
@@ -792,6 +810,7 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
         }
       }
       im->density_tpb[key] = best;
+      tune_cache_store(ctx, "dens " + std::to_string(key) + " " + std::to_string(best));
     } else {
       CHS_TRY(planes(tpb_it->second));
     }

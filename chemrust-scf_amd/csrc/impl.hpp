@@ -1,6 +1,7 @@
 // Context internals shared by the translation units of libchs_gpu.
 #pragma once
 #include <map>
+#include <string>
 #include <vector>
 #include <utility>
 
@@ -149,6 +150,9 @@ struct chs_context::Impl {
   // workspaces
   chs::DevBuf stick_buf, part, small, ws1, ws2, ws3, ws4, ws5, ws6;
   chs::DevBuf rr32;  // complex64 H|Y> and beta^H Y of Rayleigh-Ritz in the complex64 phase
+  // residual filter: FP64 residual, its complex64 copy, its projector
+  // coefficients, per-band tables (+ guard flags and window)
+  chs::DevBuf res64, res32, res_br, res_tab;
   // Lanczos start vector of the last (seed, n_pw): generated on the host once
   chs::DevBuf lanczos_v0;
   uint64_t lanczos_seed = 0;
@@ -189,6 +193,9 @@ struct chs_context::Impl {
   // device arrays and detaches them, so a later chs_scf_context_destroy only
   // frees the host side
   std::vector<chs_scf*> scf_children;
+  // on-disk tuning cache (CHS_TUNE_CACHE=<path>): launch shapes and GEMM
+  // variants keyed by shape, precision, library build and device
+  std::string tune_path;
 };
 
 namespace chs {
@@ -207,6 +214,12 @@ int bind_hamiltonian(chs_context* ctx, int n_grid, int n_pw, const int32_t* d_mi
                      const double* d_kin, const double* d_veff);
 // Make k-point k of the bound set the current Hamiltonian (k < 0: none).
 void kset_swap(chs_context* ctx, int k);
+// Tuning cache: load the entries of this build and device (context set-up)
+// and append one decision ("vloc n prec sb div threads", "gemm tsize oa ob m
+// n k choice", "dens key tpb").
+void tune_cache_load(chs_context* ctx);
+void tune_cache_store(chs_context* ctx, const std::string& line);
+
 // Drop the k-point set (its bindings, graph and per-k buffers).
 void kset_release(chs_context* ctx);
 // Free an SCF system's device arrays and detach it from its (dying) context.
@@ -221,9 +234,12 @@ inline void unbind_hamiltonian(chs_context* ctx) {
 // Local-potential H|psi> pipeline pieces (hamiltonian.hip).
 // d_coef: device {c, inv_e} read by the kernels instead of c, inv_e (so a
 // captured graph replays with new bounds); nullptr to use the values.
+// EPI_CHEBK_RES also adds res_s[b] * res_w (band b's residual, filter
+// precision; res_s: per-band scales, device f64).
 int vloc_pipeline(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int mode,
                   const void* aux0, const double* d_coef, double c, double inv_e,
-                  double* d_norm_part, int* n_part);
+                  double* d_norm_part, int* n_part, const void* res_w = nullptr,
+                  const double* res_s = nullptr);
 int vloc_autotune(chs_context* ctx, int precision, const void* psi, int n_b, void* out, int reps,
                   VlocTune* best, double* best_ms);
 
@@ -257,6 +273,7 @@ enum EpiMode {
   EPI_HPSI = 0,    // out = T psi + V psi
   EPI_CHEB1 = 1,   // out = (H psi - c psi) * inv_e
   EPI_CHEBK = 2,   // out = 2 (H psi - c psi) * inv_e - aux0
+  EPI_CHEBK_RES = 3,  // EPI_CHEBK + res_s[b] res_w (residual-based filter)
 };
 
 }  // namespace chs

@@ -93,18 +93,25 @@ __global__ void k_norm_guard_roll(const double* __restrict__ part, int n_part, i
 }
 
 // One Chebyshev step Y_out = [2] (A Y_in - c Y_in)/e [- Y_prev] with the
-// growth guard; A = H (fused epilogue) or S^{-1} H (projector path).
+// growth guard; A = H (fused epilogue) or S^{-1} H (projector path). With a
+// residual block (resw, per-band scales ress, its projector coefficients br)
+// the step is the residual filter's, Y_out += ress[b] resw_b; guard_first
+// then marks the first step the growth guard sees.
 static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, void* yout,
-                     const void* yprev, bool first, const double* d_coef, int k, double* flag) {
+                     const void* yprev, bool first, const double* d_coef, int k, double* flag,
+                     const void* resw = nullptr, const double* ress = nullptr,
+                     const void* br = nullptr, bool guard_first = false) {
   Hamiltonian& H = ctx->impl->ham;
   auto* im = ctx->impl;
   const size_t ne = (size_t)n_b * H.n_pw;
+  const int mode = resw ? EPI_CHEBK_RES : first ? EPI_CHEB1 : EPI_CHEBK;
+  if (resw) first = false;
   if (H.n_proj == 0) {
     const int gz = ceil_div(H.st.n_st, 16);  // partials of the smallest stick chunk
     CHS_TRY(ensure(ctx, im->part, sizeof(double) * 2 * (size_t)gz * n_b + 64));
     int n_part = 0;
-    CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
-                          d_coef, 0.0, 0.0, (double*)im->part.p, &n_part));
+    CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, mode, yprev, d_coef, 0.0, 0.0,
+                          (double*)im->part.p, &n_part, resw, ress));
     if (!first) {
       hipLaunchKernelGGL(k_norm_guard2, dim3(1), dim3(256), 0, ctx->stream,
                          (const double*)im->part.p, n_part, k, flag);
@@ -114,9 +121,9 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
   }
   // projector path: the local step with the fused epilogue, then the
   // projector terms in coefficient space (cheb_nl_step), then the norm
-  CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, first ? EPI_CHEB1 : EPI_CHEBK, yprev,
-                        d_coef, 0.0, 0.0, nullptr, nullptr));
-  CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, d_coef));
+  CHS_TRY(vloc_pipeline(ctx, precision, yin, n_b, yout, mode, yprev, d_coef, 0.0, 0.0, nullptr,
+                        nullptr, resw, ress));
+  CHS_TRY(cheb_nl_step(ctx, precision, yout, n_b, first, d_coef, resw ? br : nullptr, ress));
   // |Y_k|^2 over the whole block: enough workgroups to stream it at HBM rate
   constexpr int kNormBlocks = 2048;
   CHS_TRY(ensure(ctx, im->part, sizeof(double) * kNormBlocks + 64));
@@ -127,7 +134,8 @@ static int cheb_step(chs_context* ctx, int precision, const void* yin, int n_b, 
     hipLaunchKernelGGL((k_sqnorm_part<float>), dim3(kNormBlocks), dim3(kRedThreads), 0,
                        ctx->stream, (const c32*)yout, ne, (double*)im->part.p);
   hipLaunchKernelGGL(k_norm_guard_roll, dim3(1), dim3(256), 0, ctx->stream,
-                     (const double*)im->part.p, kNormBlocks, k, first ? 1 : 0, flag);
+                     (const double*)im->part.p, kNormBlocks, k, (resw ? guard_first : first) ? 1 : 0,
+                     flag);
   CHS_LAUNCH_CHECK(ctx);
   return CHS_OK;
 }
@@ -196,6 +204,152 @@ int chebyshev_filter_impl(chs_context* ctx, int precision, const void* psi, int 
   void* res = nullptr;
   CHS_TRY(filter_steps(ctx, precision, a, b, n_b, degree, d_coef, flag, &res));
   CHS_HIP(ctx, hipMemcpyAsync(out, res, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return filter_check(ctx, flag);
+}
+
+// ---- residual-based filter (R-ChFSI) ------------------------------------
+// 2025 R-ChFSI paper (reference_paper/2025-rchfsi-inexact-mv-paper,
+// main.tex:539-610, Algorithm 3 with the approximate inverse D^{-1} taken
+// as the exact Woodbury S^{-1}). With X the Rayleigh-Ritz block and Lambda
+// its eigenvalues, the unscaled recurrence Y_{k+1} = 2 (A - c) Y_k / e -
+// Y_{k-1} (A = S^{-1} H, chebyshev.rs:737-762) is carried for the residuals
+// R_k = Y_k - X T_k((Lambda - c)/e):
+//   R_0 = 0,  R_1 = W / e,  W = A X - X Lambda (formed in FP64),
+//   R_{k+1} = 2 (A - c) R_k / e - R_{k-1} + (2/e) T_k(x_b) W_b,
+//   Y_m = R_m + X T_m(x_b),  x_b = (eps_b - c) / e.
+// In exact arithmetic Y_m is the plain filter's; run in complex64 the
+// products' rounding enters in proportion to |W|, which shrinks as the
+// subspace converges, so the filtered block keeps FP64 accuracy where the
+// plain complex64 filter stalls near FP32 rounding.
+
+// W = SH - eps_b X (SH may alias W)
+__global__ void k_res_start(const z64* __restrict__ X, const z64* SH, const double* __restrict__ eps,
+                            size_t n_pw, size_t ne, z64* W) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double e = eps[i / n_pw];
+  const z64 x = X[i], h = SH[i];
+  W[i] = mk(h.x - e * x.x, h.y - e * x.y);
+}
+
+// R_0 = 0, R_1 = inv_e W (W in the filter precision)
+template <typename T>
+__global__ void k_res_init(const cplx<T>* __restrict__ W, const double* __restrict__ coef, size_t n,
+                           cplx<T>* __restrict__ R0, cplx<T>* __restrict__ R1) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T f = (T)coef[1];
+  R0[i] = mk((T)0, (T)0);
+  R1[i] = mk(f * W[i].x, f * W[i].y);
+}
+
+// Y = R + lam_b X (FP64; Y may alias X)
+template <typename T>
+__global__ void k_res_assemble(const cplx<T>* __restrict__ R, const z64* X,
+                               const double* __restrict__ lam, size_t n_pw, size_t ne, z64* Y) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double l = lam[i / n_pw];
+  const z64 x = X[i];
+  Y[i] = mk((double)R[i].x + l * x.x, (double)R[i].y + l * x.y);
+}
+
+template <typename A, typename B>
+__global__ void k_res_convert(const A* __restrict__ a, B* __restrict__ b, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = mk((decltype(b[i].x))a[i].x, (decltype(b[i].x))a[i].y);
+}
+
+int chebyshev_filter_res_impl(chs_context* ctx, int precision, const z64* X, int n_b,
+                              const double* eps, int degree, const double* b4, const z64* wx,
+                              z64* Y) {
+  CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
+  if (precision == CHS_C64) CHS_TRY(ensure_precision_ready(ctx, CHS_C64));
+  Hamiltonian& H = ctx->impl->ham;
+  auto* im = ctx->impl;
+  const int n_pw = H.n_pw, P = H.n_proj, m = degree;
+  const size_t ne = (size_t)n_b * n_pw, cs = csize(precision);
+  if (m == 0) {
+    if (Y != X)
+      CHS_HIP(ctx, hipMemcpyAsync(Y, X, ne * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
+    return CHS_OK;
+  }
+  // per-band tables: eps | s_1 .. s_{m-1} | T_m, with s_k = (2/e) T_k(x_b)
+  const double c = b4[2], inv_e = 1.0 / b4[3];
+  std::vector<double> tab((size_t)(m + 1) * n_b);
+  for (int b = 0; b < n_b; ++b) {
+    const double x = (eps[b] - c) * inv_e;
+    double t0 = 1.0, t1 = x;  // T_{k-1}, T_k at k = 1
+    tab[b] = eps[b];
+    for (int k = 1; k < m; ++k) {
+      tab[(size_t)k * n_b + b] = 2.0 * inv_e * t1;
+      const double t2 = 2.0 * x * t1 - t0;
+      t0 = t1;
+      t1 = t2;
+    }
+    tab[(size_t)m * n_b + b] = t1;  // T_m
+  }
+  CHS_TRY(ensure(ctx, im->res_tab, tab.size() * sizeof(double) + 16 * sizeof(double)));
+  double* d_tab = (double*)im->res_tab.p;
+  double* flag = d_tab + tab.size();      // 8 doubles
+  double* d_coef = flag + 8;              // {c, inv_e}
+  const double coef[2] = {c, inv_e};
+  CHS_HIP(ctx, hipMemcpyAsync(d_tab, tab.data(), tab.size() * sizeof(double),
+                              hipMemcpyHostToDevice, ctx->stream));
+  ctx->h2d += tab.size() * sizeof(double);
+  CHS_HIP(ctx, hipMemcpyAsync(d_coef, coef, sizeof(coef), hipMemcpyHostToDevice, ctx->stream));
+  ctx->h2d += sizeof(coef);
+  CHS_HIP(ctx, hipMemsetAsync(flag, 0, 8 * sizeof(double), ctx->stream));
+  // FP64 residual W = S^{-1} H X - X Lambda
+  CHS_TRY(ensure(ctx, im->res64, ne * sizeof(z64)));
+  z64* W64 = (z64*)im->res64.p;
+  CHS_TRY(vloc_pipeline(ctx, CHS_C128, X, n_b, W64, EPI_HPSI, nullptr, nullptr, 0.0, 0.0, nullptr,
+                        nullptr));
+  if (P > 0) CHS_TRY(sinv_h_nonlocal(ctx, X, n_b, wx, W64));
+  hipLaunchKernelGGL(k_res_start, dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream, X,
+                     (const z64*)W64, (const double*)d_tab, (size_t)n_pw, ne, W64);
+  CHS_LAUNCH_CHECK(ctx);
+  void* Wf = W64;
+  if (precision == CHS_C64) {
+    CHS_TRY(ensure(ctx, im->res32, ne * sizeof(c32)));
+    hipLaunchKernelGGL((k_res_convert<z64, c32>), dim3(ceil_div(ne, 256)), dim3(256), 0,
+                       ctx->stream, (const z64*)W64, (c32*)im->res32.p, ne);
+    CHS_LAUNCH_CHECK(ctx);
+    Wf = im->res32.p;
+  }
+  // R_0, R_1 and their projector coefficients
+  CHS_TRY(ensure(ctx, im->ws1, ne * cs));
+  CHS_TRY(ensure(ctx, im->ws2, ne * cs));
+  void* a = im->ws1.p;
+  void* bb = im->ws2.p;
+  if (precision == CHS_C128)
+    hipLaunchKernelGGL((k_res_init<double>), dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                       (const z64*)Wf, (const double*)d_coef, ne, (z64*)a, (z64*)bb);
+  else
+    hipLaunchKernelGGL((k_res_init<float>), dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                       (const c32*)Wf, (const double*)d_coef, ne, (c32*)a, (c32*)bb);
+  CHS_LAUNCH_CHECK(ctx);
+  void* br = nullptr;
+  if (P > 0) {
+    CHS_TRY(ensure(ctx, im->res_br, (size_t)P * n_b * cs));
+    br = im->res_br.p;
+    CHS_TRY(cheb_nl_begin_res(ctx, precision, Wf, n_b, d_coef, br));
+  }
+  for (int k = 1; k < m; ++k) {
+    CHS_TRY(cheb_step(ctx, precision, bb, n_b, a, a, false, d_coef, k + 1, flag, Wf,
+                      d_tab + (size_t)k * n_b, br, k == 1));
+    std::swap(a, bb);
+  }
+  // Y = R_m + X T_m
+  if (precision == CHS_C128)
+    hipLaunchKernelGGL((k_res_assemble<double>), dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                       (const z64*)bb, X, (const double*)(d_tab + (size_t)m * n_b), (size_t)n_pw, ne,
+                       Y);
+  else
+    hipLaunchKernelGGL((k_res_assemble<float>), dim3(ceil_div(ne, 256)), dim3(256), 0, ctx->stream,
+                       (const c32*)bb, X, (const double*)(d_tab + (size_t)m * n_b), (size_t)n_pw, ne,
+                       Y);
+  CHS_LAUNCH_CHECK(ctx);
   return filter_check(ctx, flag);
 }
 
@@ -641,6 +795,18 @@ int chs_chebyshev_filter(chs_context* ctx, int precision, const void* d_psi, int
   if (!(bounds4[3] > 0.0))
     return set_error(ctx, CHS_ERR_INVALID_ARG, "filter half width must be positive");
   return chebyshev_filter_impl(ctx, precision, d_psi, n_b, degree, bounds4, d_out);
+}
+
+int chs_chebyshev_filter_residual(chs_context* ctx, int precision, const double* d_x, int n_b,
+                                  const double* eps, int degree, const double* bounds4,
+                                  double* d_out) {
+  if (!ctx || !d_x || !d_out || !eps || !bounds4 || n_b <= 0 || degree < 0 ||
+      (precision != CHS_C128 && precision != CHS_C64))
+    return CHS_ERR_INVALID_ARG;
+  if (!(bounds4[3] > 0.0))
+    return set_error(ctx, CHS_ERR_INVALID_ARG, "filter half width must be positive");
+  return chebyshev_filter_res_impl(ctx, precision, (const z64*)d_x, n_b, eps, degree, bounds4,
+                                   nullptr, (z64*)d_out);
 }
 
 int chs_lanczos_bound(chs_context* ctx, int k, uint64_t seed, double* upper, double* ritz_min) {

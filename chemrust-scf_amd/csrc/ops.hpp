@@ -27,8 +27,21 @@ int sinv_apply(chs_context* ctx, int precision, void* x, int n_b);
 // cheb_nl_begin sets W_0 = beta^H X_0; cheb_nl_step turns the local-potential
 // step O (in place) into X_{k+1} and advances W.
 int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b);
-// d_coef: device {c, inv_e} of the filter window.
+// d_coef: device {c, inv_e} of the filter window. With br (beta^H of the
+// residual block, filter precision) and ress (device per-band scales) the
+// step is the residual filter's: O carries + ress[b] R_b.
 int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
-                 const double* d_coef);
+                 const double* d_coef, const void* br = nullptr, const double* ress = nullptr);
+// Residual filter start: br = beta^H R, W_0 = 0, W_1 = inv_e br.
+int cheb_nl_begin_res(chs_context* ctx, int precision, const void* R, int n_b,
+                      const double* d_coef, void* br);
+// out (H_loc X on entry, c128) <- S^{-1} H X; wx = beta^H X or null.
+int sinv_h_nonlocal(chs_context* ctx, const z64* X, int n_b, const z64* wx, z64* out);
+// Residual-based Chebyshev filter (R-ChFSI): Y (c128) from the Rayleigh-Ritz
+// block X (c128) and its eigenvalues eps (host); the recurrence in
+// `precision`, residual and assembly in FP64. wx = beta^H X or null.
+int chebyshev_filter_res_impl(chs_context* ctx, int precision, const z64* X, int n_b,
+                              const double* eps, int degree, const double* b4, const z64* wx,
+                              z64* Y);
 
 }  // namespace chs

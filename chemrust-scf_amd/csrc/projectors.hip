@@ -133,11 +133,15 @@ static int nonlocal_t(chs_context* ctx, const cplx<T>* psi, int n_b, cplx<T>* hp
 // coef = {c, inv_e} on the device (graph-replayable): Z = a1 G + a2 W_{k-1}
 // + c W with (a1, a2) = (e, 0) on the first step and (e/2, e/2) after;
 // DW = D W. Without S the scaled coefficients f D W go straight to Cs.
+// Residual-based filter: O carries the extra term s_b R_b (R the residual
+// block, s_b = ress[b]), so G = beta^H O holds s_b beta^H R_b too; BR =
+// beta^H R removes it before Z is formed.
 template <typename T>
 __global__ void k_cheb_coef(const cplx<T>* __restrict__ G, const cplx<T>* __restrict__ Wp,
                             const cplx<T>* __restrict__ W, const double* __restrict__ d,
                             const double* __restrict__ coef, int first, int P, size_t n,
-                            cplx<T>* __restrict__ Z, cplx<T>* __restrict__ DW) {
+                            cplx<T>* __restrict__ Z, cplx<T>* __restrict__ DW,
+                            const cplx<T>* __restrict__ BR, const double* __restrict__ ress) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const T c = (T)coef[0], e = (T)(1.0 / coef[1]);
@@ -146,7 +150,13 @@ __global__ void k_cheb_coef(const cplx<T>* __restrict__ G, const cplx<T>* __rest
   if (Z) {
     const T a1 = first ? e : (T)0.5 * e, a2 = first ? (T)0 : (T)0.5 * e;
     // W_{k-1} does not exist on the first step (not read)
-    const cplx<T> g = G[i], wp = first ? mk((T)0, (T)0) : Wp[i];
+    cplx<T> g = G[i];
+    const cplx<T> wp = first ? mk((T)0, (T)0) : Wp[i];
+    if (BR) {
+      const T sb = (T)ress[i / P];
+      const cplx<T> br = BR[i];
+      g = mk(g.x - sb * br.x, g.y - sb * br.y);
+    }
     Z[i] = mk(a1 * g.x + a2 * wp.x + c * w.x, a1 * g.y + a2 * wp.y + c * w.y);
     DW[i] = mk(dp * w.x, dp * w.y);
   } else {
@@ -193,7 +203,8 @@ static int cheb_nl_begin_t(chs_context* ctx, const cplx<T>* X0, int n_b) {
 }
 
 template <typename T>
-static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, const double* coef) {
+static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, const double* coef,
+                          const cplx<T>* BR, const double* ress) {
   Hamiltonian& H = ctx->impl->ham;
   const int P = H.n_proj, n_pw = H.n_pw;
   const bool f32 = sizeof(T) == 4;
@@ -226,7 +237,7 @@ static int cheb_nl_step_t(chs_context* ctx, cplx<T>* O, int n_b, bool first, con
   }
   if (!h3) CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, n_pw, 1.0, beta, n_pw, O, n_pw, 0.0, G, P));
   hipLaunchKernelGGL((k_cheb_coef<T>), dim3(gp), dim3(256), 0, ctx->stream, G, Wp, W, H.dion,
-                     coef, (int)first, P, pb, H.has_s ? Z : nullptr, Cs);
+                     coef, (int)first, P, pb, H.has_s ? Z : nullptr, Cs, BR, ress);
   CHS_LAUNCH_CHECK(ctx);
   if (H.has_s) {
     // Z += B (D W); C = D W - M Z; Cs = f C
@@ -259,9 +270,86 @@ int cheb_nl_begin(chs_context* ctx, int precision, const void* X0, int n_b) {
 }
 
 int cheb_nl_step(chs_context* ctx, int precision, void* O, int n_b, bool first,
-                 const double* d_coef) {
-  if (precision == CHS_C128) return cheb_nl_step_t<double>(ctx, (z64*)O, n_b, first, d_coef);
-  return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, d_coef);
+                 const double* d_coef, const void* br, const double* ress) {
+  if (precision == CHS_C128)
+    return cheb_nl_step_t<double>(ctx, (z64*)O, n_b, first, d_coef, (const z64*)br, ress);
+  return cheb_nl_step_t<float>(ctx, (c32*)O, n_b, first, d_coef, (const c32*)br, ress);
+}
+
+// Coefficients of the residual filter's first two iterates: W_0 = beta^H R_0
+// = 0 and W_1 = beta^H R_1 = inv_e BR (BR = beta^H R, R the residual block).
+template <typename T>
+__global__ void k_res_coef_start(const cplx<T>* __restrict__ BR, const double* __restrict__ coef,
+                                 size_t n, cplx<T>* __restrict__ W0, cplx<T>* __restrict__ W1) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T f = (T)coef[1];
+  W0[i] = mk((T)0, (T)0);
+  W1[i] = mk(f * BR[i].x, f * BR[i].y);
+}
+
+template <typename T>
+static int cheb_nl_begin_res_t(chs_context* ctx, const cplx<T>* R, int n_b, const double* coef,
+                               cplx<T>* BR) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj;
+  CHS_TRY(ensure(ctx, H.nl_ws, 5 * (size_t)P * n_b * sizeof(cplx<T>)));
+  H.nl_wp = 0;
+  H.nl_w = 1;
+  H.nl_g = 2;
+  const cplx<T>* beta = sizeof(T) == 8 ? (const cplx<T>*)H.beta : (const cplx<T>*)H.beta_f.p;
+  CHS_TRY(gemm_any<T>(ctx, 'C', 'N', P, n_b, H.n_pw, 1.0, beta, H.n_pw, R, H.n_pw, 0.0, BR, P));
+  const size_t pb = (size_t)P * n_b;
+  hipLaunchKernelGGL((k_res_coef_start<T>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream,
+                     (const cplx<T>*)BR, coef, pb, nl_slot<T>(H, n_b, H.nl_wp),
+                     nl_slot<T>(H, n_b, H.nl_w));
+  CHS_LAUNCH_CHECK(ctx);
+  return CHS_OK;
+}
+
+int cheb_nl_begin_res(chs_context* ctx, int precision, const void* R, int n_b,
+                      const double* d_coef, void* br) {
+  if (precision == CHS_C128)
+    return cheb_nl_begin_res_t<double>(ctx, (const z64*)R, n_b, d_coef, (z64*)br);
+  CHS_TRY(ensure_c64_projectors(ctx));
+  return cheb_nl_begin_res_t<float>(ctx, (const c32*)R, n_b, d_coef, (c32*)br);
+}
+
+// out += beta c with c = D w - M (g + B D w) (S present) or c = D w, where
+// w = beta^H X (wx, or formed here when null) and g = beta^H out, out holding
+// H_loc X on entry: out becomes S^{-1} H X (FP64). Three (two with wx)
+// n_pw-long products.
+int sinv_h_nonlocal(chs_context* ctx, const z64* X, int n_b, const z64* wx, z64* out) {
+  Hamiltonian& H = ctx->impl->ham;
+  const int P = H.n_proj, n_pw = H.n_pw;
+  if (P == 0) return CHS_OK;
+  const size_t pb = (size_t)P * n_b;
+  CHS_TRY(ensure(ctx, ctx->impl->proj_ws, 4 * pb * sizeof(z64)));
+  z64* w = (z64*)ctx->impl->proj_ws.p;
+  z64* g = w + pb;
+  z64* dw = g + pb;
+  z64* qw = dw + pb;
+  const z64* beta = (const z64*)H.beta;
+  if (wx)
+    CHS_HIP(ctx, hipMemcpyAsync(w, wx, pb * sizeof(z64), hipMemcpyDeviceToDevice, ctx->stream));
+  else
+    CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0, 0.0), beta, n_pw, X, n_pw, mk(0.0, 0.0), w,
+                  P));
+  hipLaunchKernelGGL((k_row_scale2<double>), dim3(ceil_div(pb, 256)), dim3(256), 0, ctx->stream,
+                     w, P, n_b, H.dion, H.qion, dw, (z64*)nullptr);
+  CHS_LAUNCH_CHECK(ctx);
+  if (H.has_s) {
+    // g = beta^H H_loc X + B D w;  c = D w - M g  (into dw)
+    CHS_TRY(zgemm(ctx, 'C', 'N', P, n_b, n_pw, mk(1.0, 0.0), beta, n_pw, out, n_pw, mk(0.0, 0.0), g,
+                  P));
+    CHS_TRY(zgemm(ctx, 'N', 'N', P, n_b, P, mk(1.0, 0.0), (const z64*)H.bmat.p, P, dw, P,
+                  mk(1.0, 0.0), g, P));
+    CHS_TRY(zgemm(ctx, 'N', 'N', P, n_b, P, mk(-1.0, 0.0), (const z64*)H.sinv_m.p, P, g, P,
+                  mk(1.0, 0.0), dw, P));
+  }
+  (void)qw;
+  return zgemm(ctx, 'N', 'N', n_pw, n_b, P, mk(1.0, 0.0), beta, n_pw, dw, P, mk(1.0, 0.0), out,
+               n_pw);
 }
 
 // Projector parts of the Rayleigh-Ritz subspace matrices of a c128 block Y:

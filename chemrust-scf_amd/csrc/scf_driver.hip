@@ -491,7 +491,15 @@ static int scf_loop(chs_context* ctx, const chs_scf_params* p, ScfSystem& S, chs
         fptr[k] = use32 ? kp.psi32.p : kp.psi.p;
       }
       const int fprec = use32 ? CHS_C64 : CHS_C128;
-      if (K == 1)
+      // final iterations of the complex64 form with the residual-based
+      // filter: complex64 products, FP64 residual and assembly
+      const bool res = mixed && fp64_phase && p->fp64_refine == 1 && K == 1 &&
+                       (int)S.ks[0].eps.size() == n_b;
+      if (res)
+        CHS_TRY(chebyshev_filter_res_impl(ctx, CHS_C64, (const z64*)S.ks[0].psi.p, n_b,
+                                          S.ks[0].eps.data(), p->cheb_degree, b4all.data(),
+                                          nullptr, (z64*)S.ks[0].psi.p));
+      else if (K == 1)
         CHS_TRY(chebyshev_filter_impl(ctx, fprec, fptr[0], n_b, p->cheb_degree, b4all.data(),
                                       fptr[0]));
       else if (p->cheb_degree > 0)
@@ -633,7 +641,9 @@ static int scf_loop(chs_context* ctx, const chs_scf_params* p, ScfSystem& S, chs
 }
 
 static int check_solver(chs_context* ctx, const chs_scf_params* p) {
-  if (p->max_iter <= 0 || p->cheb_degree < 0 || (p->precision != CHS_C128 && p->precision != CHS_C64))
+  if (p->max_iter <= 0 || p->cheb_degree < 0 ||
+      (p->precision != CHS_C128 && p->precision != CHS_C64) ||
+      (p->fp64_refine != 0 && p->fp64_refine != 1))
     return set_error(ctx, CHS_ERR_INVALID_ARG, "bad SCF parameters");
   return CHS_OK;
 }

@@ -76,6 +76,13 @@ int chs_diag_get(chs_context* ctx, chs_diag* out);
  * context since creation or the last reset. */
 int chs_transfer_bytes(chs_context* ctx, uint64_t* h2d, uint64_t* d2h);
 int chs_transfer_reset(chs_context* ctx);
+/* Tuning decisions the context holds: H|psi> launch shapes (per grid and
+ * precision), GEMM kernel variants (per shape) and density-plane workgroup
+ * sizes. With CHS_TUNE_CACHE=<path> set when the context is created, the
+ * decisions of earlier processes for this library build and device are read
+ * from that file at creation and every new one is appended to it (no
+ * counterpart in the reference: its plans are rebuilt per call, D8). */
+int chs_tuning_entries(chs_context* ctx, int* n_hpsi, int* n_gemm, int* n_density);
 
 /* ---- memory (Gpu<T>::from_host / sync_to_host, src/device/mod.rs:216-265;
  * Gpu::from_host_with / sync_to_host_with, src/device/pcie.rs:44-70) ---- */
@@ -118,6 +125,16 @@ int chs_fft3d_c2c(chs_context* ctx, int precision, void* d_grid, int nx, int ny,
  * of batch nx ny nz complex values. */
 int chs_fft3d_z2d(chs_context* ctx, const double* d_half, int nx, int ny, int nz, int batch,
                   double* d_real);
+/* downsample_array_to_wave_grid (src/scf.rs:413-454): a real array on the
+ * fine grid (device f64, fine_dims x fastest) to the wave grid (device f64)
+ * by G-space truncation: forward FFT on the fine grid, each wave-grid point
+ * takes the fine coefficient of its frequency (f = i for i <= n/2, i - n
+ * above; an even wave grid's Nyquist plane takes the +n/2 coefficient),
+ * inverse FFT on the wave grid, real part / (fine point count). The
+ * identity when the grids are equal. Uses a context workspace of
+ * (fine + wave) complex values. */
+int chs_downsample_to_wave_grid(chs_context* ctx, const double* d_fine, const int fine_dims[3],
+                                const int wave_dims[3], double* d_wave);
 
 /* ---- Hamiltonian ------------------------------------------------------ */
 /* Bind the plane-wave set, grid and local potential used by the operator
@@ -167,6 +184,19 @@ int chs_lanczos_bound(chs_context* ctx, int k, uint64_t seed, double* upper, dou
  * the given precision; d_out may equal d_psi. */
 int chs_chebyshev_filter(chs_context* ctx, int precision, const void* d_psi, int n_b, int degree,
                          const double* bounds4, void* d_out);
+/* Residual-based Chebyshev filter (R-ChFSI: Algorithm 3 of the reference's
+ * reference_paper/2025-rchfsi-inexact-mv-paper, main.tex:539-610, with the
+ * approximate inverse taken as the exact Woodbury S^{-1}): the degree-m
+ * filter of chs_chebyshev_filter on the Rayleigh-Ritz block d_x (c128,
+ * n_b x n_pw) with eigenvalues eps (host, n_b), carried for the residuals
+ * R_k = Y_k - X T_k((eps - c)/e) from the FP64 residual S^{-1}H X - X eps.
+ * The recurrence runs in `precision` (CHS_C64: complex64 products whose
+ * rounding scales with the residual); the residual and Y_m = R_m +
+ * X T_m are FP64. Equal to the plain filter in exact arithmetic. d_out
+ * (c128) may equal d_x. */
+int chs_chebyshev_filter_residual(chs_context* ctx, int precision, const double* d_x, int n_b,
+                                  const double* eps, int degree, const double* bounds4,
+                                  double* d_out);
 /* rayleigh_ritz (src/eigensolver/rayleigh_ritz.rs:48-245) with the
  * semantics of diagonalize_and_rotate (src/eigensolver/subspace_diag.rs:
  * 229-391): H_sub = Y^H H Y, S_sub = Y^H S Y (Hermitised), Cholesky-QR
@@ -272,6 +302,10 @@ typedef struct chs_scf_params {
                               stalls (> 0.9 x the previous) below 10 fp32_until. The
                               convergence test (|dE|/atom < etol twice in a row) then runs
                               on FP64 steps only */
+  int32_t fp64_refine;     /* with CHS_C64, the filter of those final FP64 iterations:
+                              0 = the complex128 filter, 1 = the residual-based filter
+                              (chs_chebyshev_filter_residual) in complex64 with the FP64
+                              residual (Gamma-point runs; k-point sets use 0) */
 } chs_scf_params;
 
 typedef struct chs_scf_result {

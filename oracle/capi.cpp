@@ -12,6 +12,9 @@ using namespace orc;
 #include <omp.h>
 
 void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+void orc_downsample(const double* fine, int nf, int nw, double* out) {
+  orc::downsample(fine, nf, nw, out);
+}
 int orc_get_max_threads(void) { return omp_get_max_threads(); }
 
 static ScfParams to_params(const orc_params* p) {

@@ -122,4 +122,28 @@ void fft3d(cd* g, int n, int sign) {
   }
 }
 
+// Fine -> wave grid by G-space truncation, the reference's
+// downsample_array_to_wave_grid (src/scf.rs:413-454): forward FFT of the fine
+// real array, each wave-grid point takes the fine coefficient of its
+// frequency (f = i for i <= n/2, i - n above: the Nyquist plane of an even
+// wave grid takes the fine grid's +n/2 coefficient), inverse FFT on the wave
+// grid, divided by the fine grid's point count; the real part is the result.
+void downsample(const double* fine, int nf, int nw, double* out) {
+  const size_t nf3 = (size_t)nf * nf * nf, nw3 = (size_t)nw * nw * nw;
+  std::vector<cd> g(nf3), w(nw3);
+  for (size_t i = 0; i < nf3; ++i) g[i] = cd(fine[i], 0.0);
+  fft3d(g.data(), nf, -1);
+  auto f2i = [](int i, int n, int m) {  // wave index i (grid n) -> fine index (grid m)
+    const int f = i <= n / 2 ? i : i - n;
+    return f >= 0 ? f : m + f;
+  };
+  for (int iz = 0; iz < nw; ++iz)
+    for (int iy = 0; iy < nw; ++iy)
+      for (int ix = 0; ix < nw; ++ix)
+        w[(size_t)ix + (size_t)nw * (iy + (size_t)nw * iz)] =
+            g[(size_t)f2i(ix, nw, nf) + (size_t)nf * (f2i(iy, nw, nf) + (size_t)nf * f2i(iz, nw, nf))];
+  fft3d(w.data(), nw, +1);
+  for (size_t i = 0; i < nw3; ++i) out[i] = w[i].real() / (double)nf3;
+}
+
 }  // namespace orc

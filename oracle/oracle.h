@@ -40,6 +40,8 @@ int orc_gsphere(double L, double ecut, const double kfrac[3], int n_grid, int ma
 void orc_pw_coords_to_fft_indices(const int32_t* miller, int n_pw, int ngx, int ngy, int ngz,
                                   int32_t* idx);
 void orc_fft3d(double* grid, int n, int sign);
+/* downsample_array_to_wave_grid (src/scf.rs:413-454), cubic grids */
+void orc_downsample(const double* fine, int nf, int nw, double* out);
 
 /* model: cell + plane waves + projectors + V_eff */
 void* orc_model_create(const orc_params* p, const double* pos);

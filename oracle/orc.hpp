@@ -84,6 +84,8 @@ void pw_coords_to_fft_indices(const int32_t* miller, int n_pw, int ngx, int ngy,
 // ---------------------------------------------------------------------------
 void fft1d(cd* x, int n, int stride, int sign);
 void fft3d(cd* grid, int n, int sign);
+// fine (nf^3) -> wave (nw^3) real array by G-space truncation (scf.rs:413-454)
+void downsample(const double* fine, int nf, int nw, double* out);
 
 // Signed frequency of FFT index i on an n-point axis; the Nyquist index
 // (i == n/2 for even n) is reported through *nyq.

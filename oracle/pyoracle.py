@@ -45,6 +45,7 @@ def lib():
         sig = {
             "orc_default_params": (None, [C.POINTER(Params)]),
             "orc_set_num_threads": (None, [C.c_int]),
+            "orc_downsample": (None, [f64p, C.c_int, C.c_int, f64p]),
             "orc_get_max_threads": (C.c_int, []),
             "orc_lcg_uniforms": (C.c_double, [C.c_uint64, C.c_int, f64p]),
             "orc_lcg_normals": (None, [C.c_uint64, C.c_int, f64p]),
@@ -114,6 +115,16 @@ def lib():
             fn.argtypes = args
         _LIB = L
     return _LIB
+
+
+def downsample(fine, nw):
+    """Fine n_f^3 real array -> wave n_w^3 by G-space truncation
+    (downsample_array_to_wave_grid, src/scf.rs:413-454)."""
+    fine = np.ascontiguousarray(fine, np.float64).reshape(-1)
+    nf = round(len(fine) ** (1 / 3))
+    out = np.zeros(nw ** 3)
+    lib().orc_downsample(fine, nf, nw, out)
+    return out
 
 
 def set_num_threads(n):

@@ -94,6 +94,35 @@ def test_fft3d_z2d(ctx, dims):
         assert np.abs(got[b] - v).max() < 1e-15
 
 
+@pytest.mark.parametrize("nf,nw", [(72, 36), (36, 36), (144, 72)])
+def test_downsample_to_wave_grid_vs_oracle(ctx, nf, nw):
+    """downsample_array_to_wave_grid (src/scf.rs:413-454) on the device
+    against the oracle's restatement (itself pinned against numpy and band-
+    limited closed forms)."""
+    rng = np.random.default_rng(nf)
+    fine = rng.standard_normal((nf, nf, nf))
+    got = chs.downsample_to_wave_grid(ctx, chs.Gpu.from_host(ctx, fine), (nw, nw, nw)).sync_to_host()
+    ref = O.downsample(fine, nw).reshape(nw, nw, nw)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_downsample_to_wave_grid_non_cubic(ctx):
+    """Three different edges on both grids, against numpy's FFT."""
+    fdims, wdims = (48, 36, 24), (24, 18, 12)  # (x, y, z)
+    rng = np.random.default_rng(5)
+    fine = rng.standard_normal(fdims[::-1])
+    got = chs.downsample_to_wave_grid(ctx, chs.Gpu.from_host(ctx, fine), wdims).sync_to_host()
+    g = np.fft.fftn(fine)
+    idx = []
+    for n, m in zip(wdims[::-1], fine.shape):
+        i = np.arange(n)
+        f = np.where(i <= n // 2, i, i - n)
+        idx.append(np.where(f >= 0, f, m + f))
+    w = g[np.ix_(*idx)]
+    ref = (np.fft.ifftn(w) * w.size / fine.size).real
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
 def bind(ctx, m, veff=None):
     mil, idx, kin = m.pw()
     v = m.vion() if veff is None else veff

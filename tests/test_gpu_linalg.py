@@ -195,3 +195,36 @@ def test_cholqr(ctx):
     # same span: projector onto span(psi) leaves Y unchanged
     Q, _ = np.linalg.qr(psi.T)
     assert np.abs(Q @ (Q.conj().T @ Y.T) - Y.T).max() < 1e-10
+
+
+def test_tuning_cache_on_disk(tmp_path, monkeypatch):
+    """CHS_TUNE_CACHE: one context's tuning decisions (H|psi> launch shape,
+    GEMM variants) are written to the file and read back by the next
+    context, keyed by this library build and device; a file of another
+    build (different header) is ignored."""
+    path = tmp_path / "tune.txt"
+    monkeypatch.setenv("CHS_TUNE_CACHE", str(path))
+    c1 = chs.Context(0)
+    assert c1.tuning_entries() == (0, 0, 0)
+    p = O.default_params(n_rep=1, n_grid=36, n_bands=8)
+    m = O.Model(p, O.positions(1))
+    mil, idx, kin = m.pw()
+    H = chs.Hamiltonian(c1, 36, mil, kin, m.vion())
+    chs.autotune_hpsi(c1, 8, chs.C128)
+    a = chs.Gpu.from_host(c1, np.ones((40, 300), complex))
+    out = chs.Gpu(c1, (40, 40), np.complex128)
+    chs.zgemm(c1, "C", "N", 40, 40, 300, 1.0, a, 300, a, 300, 0.0, out, 40)
+    c1.synchronize()
+    n1 = c1.tuning_entries()
+    assert n1[0] == 1 and n1[1] >= 1
+    del H
+    c1.close()
+    lines = path.read_text().splitlines()
+    assert lines[0].startswith("chs-tune 1 ") and len(lines) >= 1 + n1[0] + n1[1]
+    c2 = chs.Context(0)
+    assert c2.tuning_entries()[:2] == n1[:2]
+    c2.close()
+    path.write_text("chs-tune 1 0000000000000000 other\nvloc 36 0 16 1 256\n")
+    c3 = chs.Context(0)
+    assert c3.tuning_entries() == (0, 0, 0)
+    c3.close()

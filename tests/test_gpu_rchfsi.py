@@ -1,0 +1,110 @@
+"""The residual-based Chebyshev filter (R-ChFSI; Algorithm 3 of the
+reference's reference_paper/2025-rchfsi-inexact-mv-paper, main.tex:539-610)
+through chs_chebyshev_filter_residual, on the 8-atom cell with and without
+USPP projectors:
+
+- in complex128 it equals the plain filter chs_chebyshev_filter (itself
+  pinned to the oracle): the two recurrences are algebraically identical
+  (Y_m = R_m + X T_m(Lambda)), so only rounding differs;
+- in complex64 its error against the FP64 filter scales with the residual
+  of the start block: on a nearly converged block it is far below the plain
+  complex64 filter's, whose error stays at FP32 rounding of the block;
+- an SCF whose final iterations use it converges to the FP64 SCF's energy.
+"""
+import numpy as np
+import pytest
+
+import chs
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(ctx, npa, nb=16):
+    p = O.default_params(n_rep=1, n_grid=36, n_bands=nb, n_proj_per_atom=npa,
+                         q0=0.25 if npa else 0.0)
+    m = O.Model(p, O.positions(1))
+    mil, idx, kin = m.pw()
+    H = chs.Hamiltonian(ctx, 36, mil, kin, m.vion())
+    if npa:
+        H.bind_projectors(chs.Gpu.from_host(ctx, m.beta()), chs.Gpu.from_host(ctx, m.d()),
+                          chs.Gpu.from_host(ctx, np.full(m.n_proj, 0.25)))
+    return m, H
+
+
+def rr_block(ctx, m, nb, passes):
+    """A Rayleigh-Ritz block after `passes` degree-10 filter passes."""
+    lmax = m.grid_lambda_max()
+    psi = chs.Gpu.from_host(ctx, m.initial_psi(nb))
+    eps = None
+    for _ in range(passes):
+        b4 = O.spectral_bounds(eps, lmax)
+        psi = chs.chebyshev_filter(ctx, psi, 10, b4)
+        eps, _ = chs.rayleigh_ritz(ctx, psi)
+    return psi, eps, lmax
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.mark.parametrize("npa", [0, 4])
+def test_residual_filter_equals_plain_filter_c128(npa):
+    c = chs.Context(0)
+    m, H = setup(c, npa)
+    x, eps, lmax = rr_block(c, m, 16, 2)
+    b4 = O.spectral_bounds(eps, lmax)
+    plain = chs.chebyshev_filter(c, x, 10, b4).sync_to_host()
+    res = chs.chebyshev_filter_residual(c, x, eps, 10, b4, precision=chs.C128).sync_to_host()
+    assert rel(res, plain) < 1e-11
+    del H
+    c.close()
+
+
+@pytest.mark.parametrize("npa", [0, 4])
+def test_residual_filter_complex64_error_scales_with_residual(npa):
+    c = chs.Context(0)
+    m, H = setup(c, npa)
+    x, eps, lmax = rr_block(c, m, 16, 8)  # nearly converged block
+    b4 = O.spectral_bounds(eps, lmax)
+    ref = chs.chebyshev_filter(c, x, 10, b4).sync_to_host()
+    x32 = chs.Gpu.from_host(c, x.sync_to_host().astype(np.complex64))
+    plain32 = chs.chebyshev_filter(c, x32, 10, b4).sync_to_host().astype(np.complex128)
+    res32 = chs.chebyshev_filter_residual(c, x, eps, 10, b4, precision=chs.C64).sync_to_host()
+    e_plain, e_res = rel(plain32, ref), rel(res32, ref)
+    print(f"npa={npa}: complex64 filter error vs FP64: plain {e_plain:.2e}, residual {e_res:.2e}")
+    assert e_plain > 1e-8
+    assert e_res < 0.02 * e_plain
+    del H
+    c.close()
+
+
+@pytest.mark.parametrize("npa", [0, 4])
+def test_scf_with_residual_refinement_matches_fp64(ctx, npa):
+    """complex64 filter, then the residual filter in complex64 for the final
+    iterations (fp64_refine = 1): energy within 1e-7 Ha/atom of the plain
+    FP64 SCF, eigenvalues and density no further from the tightly converged
+    FP64 solution than the plain FP64 run at the same tolerance."""
+    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=100,
+              n_proj_per_atom=npa, q0=0.25 if npa else 0.0)
+    pos = O.positions(1)
+    r64 = chs.run_scf(ctx, chs.scf_params(**kw), pos, want_rho=True)
+    rr = chs.run_scf(ctx, chs.scf_params(**kw, precision=chs.C64, fp64_refine=1), pos,
+                     want_rho=True)
+    rt = chs.run_scf(ctx, chs.scf_params(**kw, etol=1e-8), pos, want_rho=True)
+    assert r64["converged"] and rr["converged"] and rt["converged"]
+    assert rr["n_iter"] - rr["n_iter_fp32"] >= 3
+    dv = 10.26 ** 3 / 36 ** 3
+
+    def err(r):
+        return (abs(r["energy"] - rt["energy"]) / 8, np.abs(r["eps"] - rt["eps"]).max(),
+                np.abs(r["rho"] - rt["rho"]).sum() * dv)
+
+    e64, er = err(r64), err(rr)
+    print(f"npa={npa}: residual-refined {rr['n_iter']} iterations ({rr['n_iter_fp32']} FP32) vs "
+          f"FP64 {r64['n_iter']}; distance to the tight solution (E/atom, eps, rho L1): "
+          f"FP64 {e64[0]:.2e} {e64[1]:.2e} {e64[2]:.2e}, refined {er[0]:.2e} {er[1]:.2e} {er[2]:.2e}")
+    assert rr["n_iter"] <= r64["n_iter"] + 3
+    assert abs(rr["energy"] - r64["energy"]) / 8 < 1e-7
+    assert er[1] <= 3 * e64[1] + 1e-7
+    assert er[2] <= 3 * e64[2] + 1e-6

@@ -649,3 +649,45 @@ def test_band_cg_converges_to_ground_state():
     h = [0.5, 1.0, 2.0, 4.0, 7.0]
     r = band_cg(h, [1, 0.8, 0.6, 0.4, 0.2], 40, 1e-13)
     assert abs(r["eig"] - 0.5) < 1e-8
+
+
+# ---------------------------------------------------------------------------
+# fine -> wave grid resampling (downsample_array_to_wave_grid, scf.rs:413-454)
+# ---------------------------------------------------------------------------
+def _numpy_downsample(fine, wave_shape):
+    """The same truncation written with numpy's FFT (axes z, y, x)."""
+    g = np.fft.fftn(fine)
+    idx = []
+    for n, m in zip(wave_shape, fine.shape):
+        i = np.arange(n)
+        f = np.where(i <= n // 2, i, i - n)
+        idx.append(np.where(f >= 0, f, m + f))
+    w = g[np.ix_(*idx)]
+    return (np.fft.ifftn(w) * w.size / fine.size).real
+
+
+@pytest.mark.parametrize("nf,nw", [(24, 12), (36, 18), (36, 36), (72, 36)])
+def test_downsample_matches_numpy(nf, nw):
+    rng = np.random.default_rng(nf + nw)
+    fine = rng.standard_normal((nf, nf, nf))
+    got = O.downsample(fine, nw).reshape(nw, nw, nw)
+    ref = _numpy_downsample(fine, (nw, nw, nw))
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    if nf == nw:
+        assert np.abs(got - fine).max() < 1e-12  # identity on one grid
+
+
+def test_downsample_band_limited_and_cut_modes():
+    """A field whose frequencies lie below the wave grid's Nyquist is sampled
+    exactly on the wave grid; a mode above it is removed."""
+    nf, nw = 36, 12
+    x = np.arange(nf) / nf
+    z, y, xx = np.meshgrid(x, x, x, indexing="ij")
+    low = 1.5 + np.cos(2 * np.pi * (2 * xx - 3 * y)) + 0.5 * np.sin(2 * np.pi * (z + 4 * y - xx))
+    got = O.downsample(low, nw).reshape(nw, nw, nw)
+    xw = np.arange(nw) / nw
+    zw, yw, xxw = np.meshgrid(xw, xw, xw, indexing="ij")
+    exact = 1.5 + np.cos(2 * np.pi * (2 * xxw - 3 * yw)) + 0.5 * np.sin(2 * np.pi * (zw + 4 * yw - xxw))
+    assert np.abs(got - exact).max() < 1e-12
+    high = np.cos(2 * np.pi * 9 * xx)  # frequency 9 > 12 / 2
+    assert np.abs(O.downsample(high, nw)).max() < 1e-13
