@@ -356,7 +356,7 @@ def run_form(chs, ctx, cfg, precision, steps, warmup, kpts=None, refine=0):
     pos = chs.synth_positions(cfg["n_rep"])
     prec = chs.C128 if precision == "c128" else chs.C64
     params = chs.scf_params(**cfg, max_iter=100, precision=prec,
-                            fp64_refine=refine if prec == chs.C64 and not kpts else 0)
+                            fp64_refine=refine if prec == chs.C64 else 0)
 
     def run():
         if kpts:
@@ -397,7 +397,7 @@ def main():
     ap.add_argument("--refine", default="residual", choices=["residual", "c128"],
                     help="complex64 form: the filter of the final FP64 iterations, the "
                          "residual-based complex64 filter with FP64 residual (R-ChFSI) or the "
-                         "complex128 filter (k-point sets always use the latter)")
+                         "complex128 filter")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-preconditioner report")
     ap.add_argument("--hpsi-only", action="store_true",
@@ -443,7 +443,7 @@ def main():
         return
     ctx.synchronize()
     t0 = time.perf_counter()
-    refine = 1 if args.refine == "residual" and not kpts else 0
+    refine = 1 if args.refine == "residual" else 0
     runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup, kpts, refine)
     ctx.synchronize()
     t1 = time.perf_counter()

@@ -493,13 +493,17 @@ static int scf_loop(chs_context* ctx, const chs_scf_params* p, ScfSystem& S, chs
       const int fprec = use32 ? CHS_C64 : CHS_C128;
       // final iterations of the complex64 form with the residual-based
       // filter: complex64 products, FP64 residual and assembly
-      const bool res = mixed && fp64_phase && p->fp64_refine == 1 && K == 1 &&
+      // (k-point sets: one residual filter per k-point, outside the graph)
+      const bool res = mixed && fp64_phase && p->fp64_refine == 1 &&
                        (int)S.ks[0].eps.size() == n_b;
-      if (res)
-        CHS_TRY(chebyshev_filter_res_impl(ctx, CHS_C64, (const z64*)S.ks[0].psi.p, n_b,
-                                          S.ks[0].eps.data(), p->cheb_degree, b4all.data(),
-                                          nullptr, (z64*)S.ks[0].psi.p));
-      else if (K == 1)
+      if (res) {
+        for (int k = 0; k < K; ++k) {
+          CHS_TRY(select(k));
+          CHS_TRY(chebyshev_filter_res_impl(ctx, CHS_C64, (const z64*)S.ks[k].psi.p, n_b,
+                                            S.ks[k].eps.data(), p->cheb_degree, &b4all[4 * k],
+                                            nullptr, (z64*)S.ks[k].psi.p));
+        }
+      } else if (K == 1)
         CHS_TRY(chebyshev_filter_impl(ctx, fprec, fptr[0], n_b, p->cheb_degree, b4all.data(),
                                       fptr[0]));
       else if (p->cheb_degree > 0)

@@ -305,7 +305,7 @@ typedef struct chs_scf_params {
   int32_t fp64_refine;     /* with CHS_C64, the filter of those final FP64 iterations:
                               0 = the complex128 filter, 1 = the residual-based filter
                               (chs_chebyshev_filter_residual) in complex64 with the FP64
-                              residual (Gamma-point runs; k-point sets use 0) */
+                              residual (k-point sets: one such filter per k-point) */
 } chs_scf_params;
 
 typedef struct chs_scf_result {
