@@ -88,6 +88,13 @@ template <int A, int B, int S, typename T> struct DftCT {
         const int e = (n2 * k1) % R;
         if (e == 0) {
           y[k1 * B + n2] = t[k1];
+        } else if ((4 * e) % R == 0) {
+          // w = (i S)^q, q = 4 e / R: an exact rotation (a multiply by the
+          // table's 0 and 1 entries gives the same values, but the compiler
+          // keeps x * 0, which is not 0 for an infinite or NaN x)
+          const int q = 4 * e / R;
+          y[k1 * B + n2] = q == 1 ? rot_i<S>(t[k1]) : q == 2 ? mk(-t[k1].x, -t[k1].y)
+                                                             : rot_i<-S>(t[k1]);
         } else {
           const cplx<T> w = mk((T)Roots<R>::re(e), (T)(S * Roots<R>::im(e)));
           y[k1 * B + n2] = t[k1] * w;
@@ -284,6 +291,22 @@ __device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
   wave_sync();
 }
 
+// N = 108 = 12 x 9: the second stage of a line has N1 = 12 rows k1 of 9
+// points for the 9 lanes of a group, so rows k1 = 9, 10, 11 would take a
+// second round on 3 lanes at the cost of a full one. Those three 9-point
+// transforms are instead spread over all 9 lanes, lane = 3 t + j doing row
+// k1 = 9 + t as a 3 x 3 Cooley-Tukey (n = 3 a + b, k = c + 3 d):
+//   X[c + 3 d] = sum_b w_3^{b d} w_9^{b c} sum_a w_3^{a c} x[3 a + b]
+// with one exchange through the group's scratch between the two DFT_3s
+// (lane j = b first, then lane j = c). Complex64 only: the complex128 plane
+// kernel holds one workgroup per CU, and its extra exchange latency costs
+// more than the instructions saved (6.3 against 5.8 ms per launch).
+template <int N, typename T> struct Mid9Tail {
+  static constexpr bool on = Split<N>::N1 == 12 && Split<N>::N2 == 9 && sizeof(T) == 4;
+  static constexpr int RR = on ? 1 : (Split<N>::N1 + Split<N>::N2 - 1) / Split<N>::N2;
+  static constexpr int E1 = 0, E2 = 32;  // exchange tiles [t][.][.] in the group scratch
+};
+
 // One x-row of the plane pass: inverse transform from the compact row (only
 // active columns stored; column x sits at compact index act_of[x], or is
 // absent), V_eff multiply, forward transform back into the compact row, with
@@ -301,11 +324,13 @@ __device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
 // them per row (for launch shapes whose register budget is tight).
 template <int N, typename T> struct RowPlanRegs {
   static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
-  static constexpr int R = (N1 + N2 - 1) / N2;
+  static constexpr int RR = Mid9Tail<N, T>::RR;
   int in_idx[N1];
   int out_idx[N1];
   cplx<T> tw1[N1];
-  cplx<T> tw2[R][N2];
+  cplx<T> tw2[RR][N2];
+  cplx<T> tw9[2];  // Mid9Tail: w_9^{j c}, c = 1, 2
+  cplx<T> twt[3];  // Mid9Tail: w_N^{-k1 m2}, k1 = 9 + t, m2 = j + 3 q
   __device__ __forceinline__ RowPlanRegs(const int* __restrict__ act_of,
                                          const cplx<T>* __restrict__ tw, int lane) {
 #pragma unroll
@@ -315,17 +340,43 @@ template <int N, typename T> struct RowPlanRegs {
       tw1[n1] = twiddle<1>(tw, (lane * n1) % N);
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r)
+    for (int r = 0; r < RR; ++r)
 #pragma unroll
       for (int m2 = 0; m2 < N2; ++m2) {
         const int k1 = lane + r * N2;
         tw2[r][m2] = twiddle<-1>(tw, k1 < N1 ? (k1 * m2) % N : 0);
       }
+    if constexpr (Mid9Tail<N, T>::on) {
+      const int t = lane / 3, j = lane - 3 * t;
+#pragma unroll
+      for (int c = 1; c < 3; ++c) tw9[c - 1] = twiddle<1>(tw, (N / 9) * j * c);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) twt[q] = twiddle<-1>(tw, (N2 + t) * (j + 3 * q));
+    }
+  }
+  // Completes the twiddle loads at this point: an empty asm statement takes
+  // every twiddle in use as a register operand, so the compiler waits for
+  // them here and the waits it places in a later loop count only newer loads.
+  __device__ __forceinline__ void settle() {
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) asm volatile("" : "+v"(tw1[k1].x), "+v"(tw1[k1].y));
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+#pragma unroll
+      for (int m2 = 1; m2 < N2; ++m2) asm volatile("" : "+v"(tw2[r][m2].x), "+v"(tw2[r][m2].y));
+    if constexpr (Mid9Tail<N, T>::on) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) asm volatile("" : "+v"(tw9[c].x), "+v"(tw9[c].y));
+#pragma unroll
+      for (int q = 0; q < 3; ++q) asm volatile("" : "+v"(twt[q].x), "+v"(twt[q].y));
+    }
   }
   __device__ __forceinline__ int in(int n1) const { return in_idx[n1]; }
   __device__ __forceinline__ int out(int m1) const { return out_idx[m1]; }
   __device__ __forceinline__ cplx<T> w1(int k1) const { return tw1[k1]; }
   __device__ __forceinline__ cplx<T> w2(int r, int m2) const { return tw2[r][m2]; }
+  __device__ __forceinline__ cplx<T> w9(int c) const { return tw9[c - 1]; }
+  __device__ __forceinline__ cplx<T> wt(int q) const { return twt[q]; }
 };
 
 template <int N, typename T> struct RowPlanLoad {
@@ -340,6 +391,13 @@ template <int N, typename T> struct RowPlanLoad {
   __device__ __forceinline__ cplx<T> w1(int k1) const { return twiddle<1>(tw, lane * k1); }
   __device__ __forceinline__ cplx<T> w2(int r, int m2) const {
     return twiddle<-1>(tw, (lane + r * N2) * m2);
+  }
+  __device__ __forceinline__ cplx<T> w9(int c) const {
+    return twiddle<1>(tw, (N / 9) * (lane - 3 * (lane / 3)) * c);
+  }
+  __device__ __forceinline__ cplx<T> wt(int q) const {
+    const int t = lane / 3;
+    return twiddle<-1>(tw, (N2 + t) * (lane - 3 * t + 3 * q));
   }
 };
 
@@ -379,8 +437,9 @@ __device__ __forceinline__ void group_fft_line_x(cplx<T>* line, cplx<T>* sc, int
     for (int k1 = 0; k1 < N1; ++k1) sc[k1 * PA + lane] = a[k1];
   }
   wave_sync();
+  using MT = Mid9Tail<N, T>;
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < MT::RR; ++r) {
     const int k1 = lane + r * N2;
     if (k1 < N1) {
       cplx<T> b[N2];
@@ -392,14 +451,70 @@ __device__ __forceinline__ void group_fft_line_x(cplx<T>* line, cplx<T>* sc, int
         if (!(ZOUT && W::skip_out(k2))) line[(k1 + N1 * k2) * ES] = b[k2];
     }
   }
+  if constexpr (MT::on) {  // rows k1 = 9 + t by lanes 3 t + j (Mid9Tail)
+    const int t = lane / 3, j = lane - 3 * t, k1 = N2 + t;
+    cplx<T> y[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) y[a] = sc[k1 * PA + 3 * a + j];
+    Dft<3, S, T>::run(y);
+    y[1] = y[1] * twiddle<S>(tw, (N / 9) * j);
+    y[2] = y[2] * twiddle<S>(tw, (N / 9) * 2 * j);
+    wave_sync();  // every read of the [k1][n2] tile precedes the exchange writes
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sc[MT::E1 + 9 * t + 3 * j + c] = y[c];
+    wave_sync();
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + 9 * t + 3 * bb + j];
+    Dft<3, S, T>::run(y);  // X[k2 = j + 3 d]
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      // k2 in [3 d, 3 d + 3): skipped as a whole, kept as a whole, or tested
+      const bool all_skip = ZOUT && W::K2A <= 3 * d && 3 * d + 3 <= W::K2B;
+      const bool none_skip = !ZOUT || 3 * d + 3 <= W::K2A || 3 * d >= W::K2B;
+      if (all_skip) continue;
+      const int k2 = j + 3 * d;
+      if (none_skip || !W::skip_out(k2)) line[(k1 + N1 * k2) * ES] = y[d];
+    }
+  }
   wave_sync();  // the tile is reused by the group's next line
 }
 
+// V_eff of one row as a lane of a group uses it: v[r][k2] = V(x) at
+// x = k1 + N1 k2, k1 = lane + r N2 (entries of k1 >= N1 unused). The plane
+// pass loads the next row's values while it transforms the current one, so
+// the L2 latency of the loads is spent on the row's FFT work.
+template <int N, typename T> struct RowV {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static constexpr int RR = Mid9Tail<N, T>::RR;
+  T v[RR][N2];
+  T vt[3];  // Mid9Tail: V(x) at x = 9 + t + N1 (j + 3 d), d < 3
+  __device__ __forceinline__ void load(const T* __restrict__ vrow, int lane) {
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+      const int k1 = lane + r * N2;
+      if (k1 < N1) {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) v[r][k2] = vrow[k1 + N1 * k2];
+      }
+    }
+    if constexpr (Mid9Tail<N, T>::on) {
+      const int t = lane / 3, j = lane - 3 * t;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) vt[d] = vrow[N2 + t + N1 * (j + 3 * d)];
+    }
+  }
+};
+
 // PR: the stick table fits the zero window (Window<N>): the window's inputs
-// are not read and its outputs not stored.
-template <int N, typename T, bool PR, typename Plan>
+// are not read and its outputs not stored. PF: V_eff comes from vv (this
+// row's values in registers), and vv is refilled from next_vrow (when given)
+// once the multiply has used it, so the loads run during the rest of this
+// row and the start of the next; otherwise V_eff is read from vrow at the
+// multiply.
+template <int N, typename T, bool PR, bool PF, typename Plan>
 __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Plan& rp,
-                                            const T* __restrict__ vrow, int lane) {
+                                            RowV<N, T>& vv, const T* __restrict__ vrow,
+                                            const T* __restrict__ next_vrow, int lane) {
   constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
   constexpr int R = (N1 + N2 - 1) / N2;
   using W = Window<N>;
@@ -421,9 +536,11 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     for (int k1 = 0; k1 < N1; ++k1) sc[k1 * RowScratch<N, T>::PA + lane] = a[k1];
   }
   wave_sync();
-  cplx<T> b[R][N2];
+  using MT = Mid9Tail<N, T>;
+  constexpr int RR = MT::RR;
+  cplx<T> b[RR][N2];
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RR; ++r) {
     const int k1 = lane + r * N2;
     if (k1 < N1) {
 #pragma unroll
@@ -431,7 +548,11 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
       Dft<N2, 1, T>::run(b[r]);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const T v = vrow[k1 + N1 * k2];
+        T v;
+        if constexpr (PF)
+          v = vv.v[r][k2];
+        else
+          v = vrow[k1 + N1 * k2];
         b[r][k2] = mk(b[r][k2].x * v, b[r][k2].y * v);
       }
       Dft<N2, -1, T>::run(b[r]);
@@ -439,15 +560,63 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
       for (int m2 = 1; m2 < N2; ++m2) b[r][m2] = b[r][m2] * rp.w2(r, m2);
     }
   }
+  // Mid9Tail: rows k1 = 9 + t by lanes 3 t + j; y ends as Z[m2 = j + 3 q]
+  cplx<T> y[3];
+  int tk1 = 0, tj = 0;
+  if constexpr (MT::on) {
+    const int t = lane / 3, j = lane - 3 * t, k1 = N2 + t;
+    tk1 = k1;
+    tj = j;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) y[a] = sc[k1 * RowScratch<N, T>::PA + 3 * a + j];
+    Dft<3, 1, T>::run(y);
+    y[1] = y[1] * rp.w9(1);
+    y[2] = y[2] * rp.w9(2);
+    wave_sync();  // every read of the [k1][n2] tile precedes the exchange writes
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sc[MT::E1 + 9 * t + 3 * j + c] = y[c];
+    wave_sync();
+#pragma unroll
+    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + 9 * t + 3 * bb + j];
+    Dft<3, 1, T>::run(y);  // X[j + 3 d]
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      T v;
+      if constexpr (PF)
+        v = vv.vt[d];
+      else
+        v = vrow[k1 + N1 * (j + 3 * d)];
+      y[d] = mk(y[d].x * v, y[d].y * v);
+    }
+    // forward 3 x 3 with the roles swapped: DFT_3 over d, w_9^{-j p}, then
+    // (lane j = p) DFT_3 over the lanes' index c
+    Dft<3, -1, T>::run(y);
+    y[1] = y[1] * conj(rp.w9(1));
+    y[2] = y[2] * conj(rp.w9(2));
+#pragma unroll
+    for (int p = 0; p < 3; ++p) sc[MT::E2 + 9 * t + 3 * j + p] = y[p];
+    wave_sync();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] = sc[MT::E2 + 9 * t + 3 * c + j];
+    Dft<3, -1, T>::run(y);  // Z[j + 3 q]
+#pragma unroll
+    for (int q = 0; q < 3; ++q) y[q] = y[q] * rp.wt(q);
+  }
   wave_sync();
 #pragma unroll
-  for (int r = 0; r < R; ++r) {
+  for (int r = 0; r < RR; ++r) {
     const int k1 = lane + r * N2;
     if (k1 < N1) {
 #pragma unroll
       for (int m2 = 0; m2 < N2; ++m2) sc[m2 * RowScratch<N, T>::PB + k1] = b[r][m2];
     }
   }
+  if constexpr (MT::on) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) sc[(tj + 3 * q) * RowScratch<N, T>::PB + tk1] = y[q];
+  }
+  if constexpr (PF)
+    if (next_vrow) vv.load(next_vrow, lane);
   wave_sync();
   {
     cplx<T> c[N1];

@@ -131,13 +131,23 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   cplx<T>* rowbuf = cmp + N * RS;
   int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.ld;
+  int lane;
+  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
+  const bool rows = g >= 0 && g < nrb;  // the groups that transform rows
+  const T* vz = veff + (size_t)z * N * N;
+  // Rows t = g, g + nrb, ... With PF each row's V_eff is loaded during the
+  // work before it (the first row's from here on, the others' during the
+  // row before); that holds 18 more registers per lane across the row,
+  // which the 256-thread complex64 shape has (two wavefronts per SIMD, 256
+  // registers each) and the wider shapes and complex128 do not.
+  constexpr bool PF = sizeof(T) == 4 && TPB <= 256;
+  RowV<N, T> vv;
+  if (PF && rows) vv.load(vz + (PR ? W::row(g) : g) * N, lane);
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
   __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
   __syncthreads();
-  int lane;
-  const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
   cplx<T>* sc = rowbuf + (g >= 0 ? g : 0) * RowScratch<N, T>::STRIDE;
   if constexpr (PR) {
     if (g >= 0 && g < nrb)
@@ -148,19 +158,22 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
       for (int a = g; a < st.n_act; a += ng) group_fft_line<N, 1>(cmp + a, RS, lane, tw, PostId());
   }
   __syncthreads();
-  if (g >= 0 && g < nrb) {
-    const T* vz = veff + (size_t)z * N * N;
+  if (rows) {
     if (TPB <= 512) {
-      const RowPlanRegs<N, T> rp(act_of, tw, lane);
+      RowPlanRegs<N, T> rp(act_of, tw, lane);
+      // PF: the twiddle loads complete before the loop, whose waits then
+      // count only the V_eff loads
+      if (PF) rp.settle();
       for (int t = g; t < N; t += nrb) {
-        const int iy = PR ? W::row(t) : t;
-        row_apply_v<N, T, PR>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+        const int iy = PR ? W::row(t) : t, tn = t + nrb;
+        const T* vn = PF && tn < N ? vz + (PR ? W::row(tn) : tn) * N : nullptr;
+        row_apply_v<N, T, PR, PF>(cmp + iy * RS, sc, rp, vv, vz + iy * N, vn, lane);
       }
     } else {
       const RowPlanLoad<N, T> rp(act_of, tw, lane);
       for (int t = g; t < N; t += nrb) {
         const int iy = PR ? W::row(t) : t;
-        row_apply_v<N, T, PR>(cmp + iy * RS, sc, rp, vz + iy * N, lane);
+        row_apply_v<N, T, PR, false>(cmp + iy * RS, sc, rp, vv, vz + iy * N, nullptr, lane);
       }
     }
   }

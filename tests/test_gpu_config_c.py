@@ -1,8 +1,10 @@
 """The 216-atom USPP cell (3x3x3 diamond-cubic, 108^3 grid, 80,725 plane
 waves, 864 projectors) at operator level against the FP64 oracle: H|psi>
 and S|psi> in both precisions on a few bands (the largest plane and stick
-tables the kernels see), and the complex64 filter against the complex128
-one on the device."""
+tables the kernels see), every launch shape of the plane kernel, and the
+complex64 filter against the complex128 one on the device."""
+import os
+
 import numpy as np
 import pytest
 
@@ -53,3 +55,49 @@ def test_filter_c64_vs_c128_config_c(ctx, cell):
     b = chs.chebyshev_filter(ctx, chs.Gpu.from_host(ctx, psi.astype(np.complex64)), 12,
                              b4).sync_to_host().astype(np.complex128)
     assert np.abs(a - b).max() / np.abs(a).max() < 1e-4
+
+
+# Plane-kernel launch shapes (workgroup size, row-buffer divisor) the tuner
+# may choose; each is forced through a tuning-cache file.
+SHAPES = {np.complex64: [(256, 1), (256, 2), (512, 1), (512, 2), (512, 4), (1024, 1), (1024, 2)],
+          np.complex128: [(256, 1), (256, 2), (512, 1), (512, 2)]}
+
+
+@pytest.fixture(scope="module")
+def tune_header(tmp_path_factory):
+    """First line of a tuning cache written by this library on this device."""
+    path = tmp_path_factory.mktemp("tune") / "cache"
+    os.environ["CHS_TUNE_CACHE"] = str(path)
+    try:
+        c = chs.Context(0)
+        mil, idx, kin = chs.gsphere(c, 10.26, 15.0, 36)
+        veff = chs.Gpu.from_host(c, np.zeros(36 ** 3))
+        chs.Hamiltonian(c, 36, mil, kin, veff)
+        chs.autotune_hpsi(c, 4, chs.C64)
+        c.close()
+    finally:
+        del os.environ["CHS_TUNE_CACHE"]
+    return path.read_text().splitlines()[0]
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 3e-5)])
+def test_plane_kernel_every_launch_shape(cell, tune_header, tmp_path, dtype, tol):
+    psi = cell.initial_psi(3)
+    rh = cell.apply_h(psi)
+    prec = chs.C64 if dtype == np.complex64 else chs.C128
+    for threads, div in SHAPES[dtype]:
+        f = tmp_path / f"tune_{threads}_{div}"
+        f.write_text(f"{tune_header}\nvloc 108 {prec} 16 {div} {threads}\n")
+        os.environ["CHS_TUNE_CACHE"] = str(f)
+        c = chs.Context(0)
+        try:
+            H = bind(c, cell)
+            gh = chs.apply_full_hamiltonian(c, chs.Gpu.from_host(c, psi.astype(dtype)))
+            gh = gh.sync_to_host().astype(np.complex128)
+            err = np.abs(gh - rh).max() / np.abs(rh).max()
+            print(f"{threads} threads, row-buffer divisor {div}: {err:.1e}")
+            assert err < tol, (threads, div, err)
+            del H, gh
+        finally:
+            del os.environ["CHS_TUNE_CACHE"]
+            c.close()
