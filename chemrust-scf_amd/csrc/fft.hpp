@@ -304,7 +304,12 @@ __device__ __forceinline__ void group_fft_line(cplx<T>* base, int es, int lane,
 template <int N, typename T> struct Mid9Tail {
   static constexpr bool on = Split<N>::N1 == 12 && Split<N>::N2 == 9 && sizeof(T) == 4;
   static constexpr int RR = on ? 1 : (Split<N>::N1 + Split<N>::N2 - 1) / Split<N>::N2;
-  static constexpr int E1 = 0, E2 = 32;  // exchange tiles [t][.][.] in the group scratch
+  // Exchange tiles in the group scratch: the value lane (t, x) hands to lane
+  // (t, y) sits at slot E + 3 t + x + 17 y, so the writes (fixed y) and the
+  // reads (fixed x) of the 7 groups of a wavefront spread evenly over the
+  // LDS banks (with the group scratch stride of 137 slots).
+  static constexpr int E1 = 0, E2 = 48;
+  static __device__ __forceinline__ int at(int t, int x, int y) { return 3 * t + x + 17 * y; }
 };
 
 // One x-row of the plane pass: inverse transform from the compact row (only
@@ -461,10 +466,10 @@ __device__ __forceinline__ void group_fft_line_x(cplx<T>* line, cplx<T>* sc, int
     y[2] = y[2] * twiddle<S>(tw, (N / 9) * 2 * j);
     wave_sync();  // every read of the [k1][n2] tile precedes the exchange writes
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sc[MT::E1 + 9 * t + 3 * j + c] = y[c];
+    for (int c = 0; c < 3; ++c) sc[MT::E1 + MT::at(t, j, c)] = y[c];
     wave_sync();
 #pragma unroll
-    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + 9 * t + 3 * bb + j];
+    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + MT::at(t, bb, j)];
     Dft<3, S, T>::run(y);  // X[k2 = j + 3 d]
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -574,10 +579,10 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     y[2] = y[2] * rp.w9(2);
     wave_sync();  // every read of the [k1][n2] tile precedes the exchange writes
 #pragma unroll
-    for (int c = 0; c < 3; ++c) sc[MT::E1 + 9 * t + 3 * j + c] = y[c];
+    for (int c = 0; c < 3; ++c) sc[MT::E1 + MT::at(t, j, c)] = y[c];
     wave_sync();
 #pragma unroll
-    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + 9 * t + 3 * bb + j];
+    for (int bb = 0; bb < 3; ++bb) y[bb] = sc[MT::E1 + MT::at(t, bb, j)];
     Dft<3, 1, T>::run(y);  // X[j + 3 d]
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -594,10 +599,10 @@ __device__ __forceinline__ void row_apply_v(cplx<T>* crow, cplx<T>* sc, const Pl
     y[1] = y[1] * conj(rp.w9(1));
     y[2] = y[2] * conj(rp.w9(2));
 #pragma unroll
-    for (int p = 0; p < 3; ++p) sc[MT::E2 + 9 * t + 3 * j + p] = y[p];
+    for (int p = 0; p < 3; ++p) sc[MT::E2 + MT::at(t, j, p)] = y[p];
     wave_sync();
 #pragma unroll
-    for (int c = 0; c < 3; ++c) y[c] = sc[MT::E2 + 9 * t + 3 * c + j];
+    for (int c = 0; c < 3; ++c) y[c] = sc[MT::E2 + MT::at(t, c, j)];
     Dft<3, -1, T>::run(y);  // Z[j + 3 q]
 #pragma unroll
     for (int q = 0; q < 3; ++q) y[q] = y[q] * rp.wt(q);
