@@ -384,6 +384,68 @@ template <int N, typename T> struct RowPlanRegs {
   __device__ __forceinline__ cplx<T> wt(int q) const { return twt[q]; }
 };
 
+// Per-lane twiddle table of RowPlanLds in LDS: row `lane` (N2 rows, pitch P)
+// holds w_N^{lane k}, k < P - 1 (every w1 and, conjugated, every w2 of the
+// lane); the odd pitch spreads a group's rows over the banks.
+template <int N> struct LaneTw {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static constexpr int P = (N1 > N2 ? N1 : N2) + 1;
+  static constexpr int SIZE = N2 * P;
+  template <typename T>
+  static __device__ __forceinline__ void fill(cplx<T>* twl, const cplx<T>* __restrict__ tw) {
+    for (int i = threadIdx.x; i < SIZE; i += blockDim.x) {
+      const int l = i / P, k = i - l * P;
+      twl[i] = tw[(l * k) % N];
+    }
+  }
+};
+
+// Row plan with the row twiddles read from the LaneTw table (one base
+// register per lane, the entries at immediate offsets) instead of held in
+// registers: about 38 registers fewer than RowPlanRegs, which lets the
+// 512-thread complex64 plane kernel keep V_eff one row ahead within the 128
+// registers of four wavefronts per SIMD. The Mid9Tail twiddles stay in
+// registers. Single-round middle stages only (Mid9Tail<N, T>::RR == 1).
+template <int N, typename T> struct RowPlanLds {
+  static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
+  static_assert(Mid9Tail<N, T>::RR == 1, "RowPlanLds needs a single-round middle stage");
+  int in_idx[N1];
+  int out_idx[N1];
+  const cplx<T>* t1;
+  cplx<T> tw9[2];
+  cplx<T> twt[3];
+  __device__ __forceinline__ RowPlanLds(const int* __restrict__ act_of, const cplx<T>* twl,
+                                        const cplx<T>* __restrict__ tw, int lane) {
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) {
+      in_idx[n1] = act_of[n1 * N2 + lane];
+      out_idx[n1] = act_of[lane + N2 * n1];
+    }
+    t1 = twl + lane * LaneTw<N>::P;
+    if constexpr (Mid9Tail<N, T>::on) {
+      const int t = lane / 3, j = lane - 3 * t;
+#pragma unroll
+      for (int c = 1; c < 3; ++c) tw9[c - 1] = twiddle<1>(tw, (N / 9) * j * c);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) twt[q] = twiddle<-1>(tw, (N2 + t) * (j + 3 * q));
+    }
+  }
+  __device__ __forceinline__ void settle() {
+    if constexpr (Mid9Tail<N, T>::on) {
+#pragma unroll
+      for (int c = 0; c < 2; ++c) asm volatile("" : "+v"(tw9[c].x), "+v"(tw9[c].y));
+#pragma unroll
+      for (int q = 0; q < 3; ++q) asm volatile("" : "+v"(twt[q].x), "+v"(twt[q].y));
+    }
+  }
+  __device__ __forceinline__ int in(int n1) const { return in_idx[n1]; }
+  __device__ __forceinline__ int out(int m1) const { return out_idx[m1]; }
+  __device__ __forceinline__ cplx<T> w1(int k1) const { return t1[k1]; }
+  __device__ __forceinline__ cplx<T> w2(int, int m2) const { return conj(t1[m2]); }
+  __device__ __forceinline__ cplx<T> w9(int c) const { return tw9[c - 1]; }
+  __device__ __forceinline__ cplx<T> wt(int q) const { return twt[q]; }
+};
+
 template <int N, typename T> struct RowPlanLoad {
   static constexpr int N1 = Split<N>::N1, N2 = Split<N>::N2;
   const int* act_of;

@@ -130,6 +130,10 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   cplx<T>* cmp = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* rowbuf = cmp + N * RS;
   int* act_of = reinterpret_cast<int*>(rowbuf + nrb * RowScratch<N, T>::STRIDE);
+  // LT: the rows use RowPlanLds, whose twiddle table follows act_of (N is even,
+  // so the table starts 8-byte aligned)
+  constexpr bool LT = sizeof(T) == 4 && TPB == 512;
+  cplx<T>* twl = reinterpret_cast<cplx<T>*>(act_of + N);
   cplx<T>* plane = sbuf + ((size_t)b * N + z) * st.ld;
   int lane;
   const int g = Groups<N>::id(&lane), ng = Groups<N>::count();
@@ -137,14 +141,16 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   const T* vz = veff + (size_t)z * N * N;
   // Rows t = g, g + nrb, ... With PF each row's V_eff is loaded during the
   // work before it (the first row's from here on, the others' during the
-  // row before); that holds 18 more registers per lane across the row,
-  // which the 256-thread complex64 shape has (two wavefronts per SIMD, 256
-  // registers each) and the wider shapes and complex128 do not.
-  constexpr bool PF = sizeof(T) == 4 && TPB <= 256;
+  // row before); that holds 12 more registers per lane across the row, which
+  // the 256-thread complex64 shape has (two wavefronts per SIMD, 256
+  // registers each), the 512-thread one with the LDS twiddle plan, and the
+  // 1024-thread shape and complex128 do not.
+  constexpr bool PF = sizeof(T) == 4 && TPB <= 512;
   RowV<N, T> vv;
   if (PF && rows) vv.load(vz + (PR ? W::row(g) : g) * N, lane);
   for (int i = threadIdx.x; i < N * RS; i += blockDim.x) cmp[i] = zero_c<T>();
   for (int i = threadIdx.x; i < N; i += blockDim.x) act_of[i] = st.act_of[i];
+  if (LT) LaneTw<N>::fill(twl, tw);
   __syncthreads();
   for (int s = threadIdx.x; s < st.n_st; s += blockDim.x) cmp[st.iy[s] * RS + st.act[s]] = plane[s];
   __syncthreads();
@@ -159,16 +165,22 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
   }
   __syncthreads();
   if (rows) {
-    if (TPB <= 512) {
-      RowPlanRegs<N, T> rp(act_of, tw, lane);
-      // PF: the twiddle loads complete before the loop, whose waits then
-      // count only the V_eff loads
+    auto rows_pass = [&](auto& rp) {
+      // PF: the plan's global loads complete before the loop, whose waits
+      // then count only the V_eff loads
       if (PF) rp.settle();
       for (int t = g; t < N; t += nrb) {
         const int iy = PR ? W::row(t) : t, tn = t + nrb;
         const T* vn = PF && tn < N ? vz + (PR ? W::row(tn) : tn) * N : nullptr;
         row_apply_v<N, T, PR, PF>(cmp + iy * RS, sc, rp, vv, vz + iy * N, vn, lane);
       }
+    };
+    if constexpr (LT) {
+      RowPlanLds<N, T> rp(act_of, twl, tw, lane);
+      rows_pass(rp);
+    } else if (TPB <= 512) {
+      RowPlanRegs<N, T> rp(act_of, tw, lane);
+      rows_pass(rp);
     } else {
       const RowPlanLoad<N, T> rp(act_of, tw, lane);
       for (int t = g; t < N; t += nrb) {
@@ -517,15 +529,16 @@ struct PlaneLaunch {
   int threads = 256;
 };
 static int plane_launch(chs_context* ctx, int n, const StickArgs& sa, size_t csz, int n2,
-                        int nrb_div, int threads, int row_elems, PlaneLaunch* out) {
+                        int nrb_div, int threads, int row_elems, PlaneLaunch* out,
+                        size_t extra_bytes = 0) {
   const size_t cmp_bytes = (size_t)n * sa.rs * csz;
   const size_t row_bytes = (size_t)row_elems * csz;
   const int lim = max_lds_per_block(ctx);
-  if (cmp_bytes + row_bytes + (size_t)n * sizeof(int32_t) > (size_t)lim)
+  if (cmp_bytes + row_bytes + (size_t)n * sizeof(int32_t) + extra_bytes > (size_t)lim)
     return set_error(ctx, CHS_ERR_FFT, "plane of %d^3 grid does not fit in LDS (%zu > %d bytes)",
-                     n, cmp_bytes + row_bytes, lim);
+                     n, cmp_bytes + row_bytes + extra_bytes, lim);
   const int groups = (threads / 64) * (64 / n2);
-  const size_t tab_bytes = (size_t)n * sizeof(int32_t);  // act_of
+  const size_t tab_bytes = (size_t)n * sizeof(int32_t) + extra_bytes;  // act_of (+ twiddles)
   int nrb = (int)((lim - cmp_bytes - tab_bytes) / row_bytes);
   nrb = std::max(1, std::min(nrb, groups) / std::max(1, nrb_div));
   out->nrb = nrb;
@@ -641,8 +654,11 @@ static int vloc_pipeline_t(chs_context* ctx, int precision, const void* psi_v, i
   for (int r = 0; r < std::max(1, reps); ++r) {
     CHS_DISPATCH_N(n, {
       PlaneLaunch pl;
+      // the 512-thread complex64 plane kernel keeps its row twiddles in LDS
+      const size_t lanetw = tn.threads == 512 && sizeof(C) == 8
+                                ? (size_t)LaneTw<NN>::SIZE * sizeof(C) : 0;
       CHS_TRY(plane_launch(ctx, NN, sa, sizeof(C), Split<NN>::N2, tn.nrb_div, tn.threads,
-                           RowScratch<NN, T>::STRIDE, &pl));
+                           RowScratch<NN, T>::STRIDE, &pl, lanetw));
       hipEvent_t* evp = ms ? ev : nullptr;
       if (tn.sb == 16)
         CHS_TRY((vloc_launch<NN, T, 16>(ctx, sa, pl, n_b, psi, sbuf, veff, tw, H.kin, out, aux0,
