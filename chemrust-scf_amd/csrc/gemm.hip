@@ -168,10 +168,14 @@ __device__ __forceinline__ bool gemm_block(int mt, int nt, int kz, int nfast, in
   return true;
 }
 
-// CL: operand loads at clamped addresses with a select for the out-of-range
-// elements (no branch per element, so a stage's loads issue back to back);
-// otherwise each element's load sits behind its bounds test. Which is faster
-// depends on the shape (the autotuner decides).
+// CL: each thread keeps one 32-bit byte offset per staged element from a
+// uniform base, set once (rows and columns past the matrix edge clamped to
+// the last one: they only feed output elements that are never stored); the
+// bases advance by one K stage per stage, so a full stage's loads need no
+// address arithmetic or bounds tests, and only a slice's last, partial stage
+// tests k against its end. Usable while every offset fits in 32 bits
+// (cl_offsets_fit). Otherwise each element's load sits behind its bounds
+// test. Which is faster depends on the shape (the autotuner decides).
 template <typename T, int OPA, int OPB, int V, bool CL>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, int n, int k, cplx<T> alpha,
@@ -200,7 +204,13 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
   const int m0 = bm * BM, n0 = bn * BN;
   const int kb = bz * klen, ke = min(k, kb + klen);
   // three real products per complex one (Re = P1 - P2, Im = P3 - P1 - P2
-  // with P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar + ai)(br + bi))
+  // with P1 = sum ar br, P2 = sum ai bi, P3 = sum (ar + ai)(br + bi)).
+  // CL stages an op-C operand unconjugated and folds the conjugation into
+  // the products: with s_a, s_b = -1 for a conjugated operand (else 1) and
+  // P3 = sum (ar + s_a ai)(br + s_b bi), Re = P1 - s_a s_b P2 and
+  // Im = P3 - P1 - s_a s_b P2.
+  constexpr int SA = CL && OPA == 2 ? -1 : 1, SB = CL && OPB == 2 ? -1 : 1;
+  constexpr int OPA_LD = CL && OPA == 2 ? 1 : OPA, OPB_LD = CL && OPB == 2 ? 1 : OPB;
   acc_t p1[WTM][WTN], p2[WTM][WTN], p3[WTM][WTN];
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
@@ -217,34 +227,61 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
   auto b_pos = [](int e, int& c, int& kk) {
     if (OPB == 0) { kk = e % BK; c = e / BK; } else { c = e % BN; kk = e / BN; }
   };
-  auto load = [&](int k0) {
+  // CL: per-element byte offsets from the operands' stage bases (32-bit: the
+  // host takes the bounds-tested kernel when they would not fit), the bases
+  // advanced by one K stage after each stage's loads
+  uint32_t oa[CL ? NA : 1], ob[CL ? NB : 1];
+  const char* base_a = reinterpret_cast<const char*>(A + (OPA == 0 ? (size_t)kb * lda : kb));
+  const char* base_b = reinterpret_cast<const char*>(B + (OPB == 0 ? kb : (size_t)kb * ldb));
+  const size_t sa = (OPA == 0 ? (size_t)BK * lda : (size_t)BK) * sizeof(cplx<T>);
+  const size_t sb = (OPB == 0 ? (size_t)BK : (size_t)BK * ldb) * sizeof(cplx<T>);
+  if constexpr (CL) {
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       int r, kk;
       a_pos(threadIdx.x + 256 * j, r, kk);
-      const int gr = m0 + r, gk = k0 + kk;
-      if (CL) {
-        const cplx<T> v = load_op<T, OPA>(A, lda, min(gr, m - 1), min(gk, ke - 1));
-        ra[j] = (gr < m && gk < ke) ? v : mk((T)0, (T)0);
-      } else {
-        ra[j] = (gr < m && gk < ke) ? load_op<T, OPA>(A, lda, gr, gk) : mk((T)0, (T)0);
-      }
+      const size_t gr = (size_t)min(m0 + r, m - 1);
+      oa[j] = (uint32_t)((OPA == 0 ? gr + (size_t)kk * lda : kk + gr * lda) * sizeof(cplx<T>));
     }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       int c, kk;
       b_pos(threadIdx.x + 256 * j, c, kk);
-      const int gc = n0 + c, gk = k0 + kk;
-      if (CL) {
-        const cplx<T> v = load_op<T, OPB>(B, ldb, min(gk, ke - 1), min(gc, n - 1));
-        rb[j] = (gc < n && gk < ke) ? v : mk((T)0, (T)0);
-      } else {
-        rb[j] = (gc < n && gk < ke) ? load_op<T, OPB>(B, ldb, gk, gc) : mk((T)0, (T)0);
-      }
+      const size_t gc = (size_t)min(n0 + c, n - 1);
+      ob[j] = (uint32_t)((OPB == 0 ? kk + gc * ldb : gc + (size_t)kk * ldb) * sizeof(cplx<T>));
+    }
+  }
+  // Bounds-tested loads of stage k0 (the non-CL kernel, and the CL kernel's
+  // partial last stage of a slice: k >= ke zero, rows and columns past the
+  // edge clamped as the offsets are)
+  auto load_tested = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+      int r, kk;
+      a_pos(threadIdx.x + 256 * j, r, kk);
+      const int gr = CL ? min(m0 + r, m - 1) : m0 + r, gk = k0 + kk;
+      ra[j] = (gr < m && gk < ke) ? load_op<T, OPA_LD>(A, lda, gr, gk) : mk((T)0, (T)0);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      int c, kk;
+      b_pos(threadIdx.x + 256 * j, c, kk);
+      const int gc = CL ? min(n0 + c, n - 1) : n0 + c, gk = k0 + kk;
+      rb[j] = (gc < n && gk < ke) ? load_op<T, OPB_LD>(B, ldb, gk, gc) : mk((T)0, (T)0);
     }
   };
-  if (kb < ke) load(kb);
-  for (int k0 = kb; k0 < ke; k0 += BK) {
+  // CL, full stages: one plain load per element at its offset, the values
+  // untouched until the LDS store of the next stage, so they are waited for
+  // only after this stage's matrix-core products
+  auto load_plain = [&]() {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) ra[j] = *reinterpret_cast<const cplx<T>*>(base_a + oa[j]);
+#pragma unroll
+    for (int j = 0; j < NB; ++j) rb[j] = *reinterpret_cast<const cplx<T>*>(base_b + ob[j]);
+    base_a += sa;
+    base_b += sb;
+  };
+  auto store = [&]() {
 #pragma unroll
     for (int j = 0; j < NA; ++j) {
       int r, kk;
@@ -257,8 +294,8 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
       b_pos(threadIdx.x + 256 * j, c, kk);
       Bs[kk][c] = rb[j];
     }
-    __syncthreads();
-    if (k0 + BK < ke) load(k0 + BK);  // next stage in flight during the MFMAs
+  };
+  auto products = [&]() {
 #pragma unroll
     for (int s = 0; s < BK / 4; ++s) {
       const int kk = 4 * s + lk;
@@ -281,9 +318,35 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
       for (int i = 0; i < WTM; ++i)
 #pragma unroll
         for (int j = 0; j < WTN; ++j)
-          p3[i][j] = M::mac(a[i].x + a[i].y, b[j].x + b[j].y, p3[i][j]);
+          p3[i][j] = M::mac(a[i].x + (T)SA * a[i].y, b[j].x + (T)SB * b[j].y, p3[i][j]);
     }
-    __syncthreads();
+  };
+  if constexpr (CL) {
+    // full stages with the next one's loads in flight, then the partial one
+    const int kf = kb + max(0, (ke - kb) / BK) * BK;
+    if (kb < kf) load_plain();
+    for (int k0 = kb; k0 < kf; k0 += BK) {
+      store();
+      __syncthreads();
+      if (k0 + BK < kf) load_plain();
+      products();
+      __syncthreads();
+    }
+    if (kf < ke) {
+      load_tested(kf);
+      store();
+      __syncthreads();
+      products();
+    }
+  } else {
+    if (kb < ke) load_tested(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      store();
+      __syncthreads();
+      if (k0 + BK < ke) load_tested(k0 + BK);  // next stage in flight during the MFMAs
+      products();
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
@@ -295,7 +358,8 @@ __attribute__((amdgpu_waves_per_eu(MfmaTile<T, V>::WAVES))) k_gemm_mfma(int m, i
         const int c = n0 + wn0 + 16 * j + li;
         if (r < m && c < n) {
           const T r1 = (T)p1[i][j][v], r2 = (T)p2[i][j][v];
-          const cplx<T> acc = mk(r1 - r2, (T)p3[i][j][v] - r1 - r2);
+          const T r2s = SA * SB < 0 ? -r2 : r2;
+          const cplx<T> acc = mk(r1 - r2s, (T)p3[i][j][v] - r1 - r2s);
           if (part) {
             part[(size_t)bz * m * n + (size_t)r + (size_t)c * m] = acc;
           } else {
@@ -520,11 +584,29 @@ static void split_plan(int m, int n, int k, int bm, int bn, int bk, int target, 
   *ksplit = std::max(1, (k + kl - 1) / kl);
 }
 
+// Largest byte offset of a CL kernel's operand element from its stage base
+// (rows and columns clamped to the matrix, k within one stage) below 2^32.
+template <typename T>
+static bool cl_offsets_fit(int oa, int ob, int m, int n, int lda, int ldb, int bk) {
+  const double sz = (double)sizeof(cplx<T>);
+  const double a = oa == 0 ? (m - 1) + (double)(bk - 1) * lda : (bk - 1) + (double)(m - 1) * lda;
+  const double b = ob == 0 ? (bk - 1) + (double)(n - 1) * ldb : (n - 1) + (double)(bk - 1) * ldb;
+  return a * sz < 4294967296.0 && b * sz < 4294967296.0;
+}
+
 template <typename T>
 static int gemm_launch(chs_context* ctx, int choice, int oa, int ob, int m, int n, int k,
                        cplx<T> alpha, const cplx<T>* A, int lda, const cplx<T>* B, int ldb,
                        cplx<T> beta, cplx<T>* C, int ldc, DevBuf& ws, int tri = 0) {
-  const int variant = choice & 15, target = kSplitTargets[(choice >> 4) % kNumSplitTargets];
+  int variant = choice & 15;
+  const int target = kSplitTargets[(choice >> 4) % kNumSplitTargets];
+  // the offset-addressed kernels (3, 4, 5) only while their offsets fit 32 bits;
+  // otherwise the bounds-tested kernel of the same tile
+  if (variant >= 3) {
+    int bm0, bn0, bk0;
+    tile_dims<T>(variant, &bm0, &bn0, &bk0);
+    if (!cl_offsets_fit<T>(oa, ob, m, n, lda, ldb, bk0)) variant = variant == 4 ? 2 : 1;
+  }
   int bm, bn, bk;
   tile_dims<T>(variant, &bm, &bn, &bk);
   int ksplit, klen;

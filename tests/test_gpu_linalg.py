@@ -52,19 +52,23 @@ def test_zgemm_random(ctx, opa, opb, m, n, k):
 def test_zgemm_every_kernel_choice(monkeypatch, choice):
     """Each GEMM kernel the autotuner can pick (variant in the low bits, split-K
     target index in bits 4-5), pinned with CHS_GEMM_CHOICE, on ragged shapes
-    (partial tiles in m, n and the last K slice), both precisions."""
+    (partial tiles in m, n and the last K slice), every pairing of plain,
+    transposed and conjugated operands, both precisions."""
     monkeypatch.setenv("CHS_GEMM_CHOICE", str(choice))
     c = chs.Context(0)
     rng = np.random.default_rng(choice)
+    def op(x, o):
+        return x if o == "N" else (x.T if o == "T" else x.conj().T)
     for opa, opb, m, n, k in (("C", "N", 150, 77, 1303), ("N", "N", 1303, 77, 150),
-                              ("N", "C", 97, 130, 211)):
+                              ("N", "C", 97, 130, 211), ("C", "C", 61, 45, 517),
+                              ("T", "C", 70, 33, 190), ("C", "T", 45, 66, 271),
+                              ("T", "N", 40, 20, 9)):
         sa = (m, k) if opa == "N" else (k, m)
         sb = (k, n) if opb == "N" else (n, k)
         A = rng.standard_normal(sa) + 1j * rng.standard_normal(sa)
         B = rng.standard_normal(sb) + 1j * rng.standard_normal(sb)
         C0 = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
-        ref = (0.7 - 0.2j) * (A.conj().T if opa == "C" else A) @ \
-            (B.conj().T if opb == "C" else B) + (0.3 + 0.1j) * C0
+        ref = (0.7 - 0.2j) * op(A, opa) @ op(B, opb) + (0.3 + 0.1j) * C0
         for dt, tol in ((np.complex128, 1e-13), (np.complex64, 3e-6)):
             a, b = chs.Gpu.from_host(c, fmat(A).astype(dt)), chs.Gpu.from_host(c, fmat(B).astype(dt))
             cc = chs.Gpu.from_host(c, fmat(C0).astype(dt))
