@@ -134,10 +134,12 @@ template <> struct MfmaTile<float, 2> {
   static constexpr int WAVES = 1;
 };
 // 64 x 64 with a 32-deep K stage: half the barriers per matrix-core product
-// and twice the loads in flight per stage, for the long-K products
+// and twice the loads in flight per stage, for the long-K products (its
+// 16 staged elements per thread need the registers of one wavefront per
+// SIMD fewer than the 16-deep tile to stay out of scratch)
 template <typename T> struct MfmaTile<T, 3> {
   static constexpr int BM = 64, BN = 64, BK = 32, WTM = 2, WTN = 2;
-  static constexpr int WAVES = sizeof(T) == 8 ? 2 : 4;
+  static constexpr int WAVES = sizeof(T) == 8 ? 1 : 2;
 };
 template <> struct MfmaTile<double, 2> {
   static constexpr int BM = 128, BN = 64, BK = 8, WTM = 4, WTN = 2;
