@@ -133,8 +133,8 @@ def roofline(chs, ctx, cfg, precision, tune=False):
                 # per 4 cycles per SIMD (16 lanes, no packed FP32), 1,024 SIMDs
                 # at 2.4 GHz
                 rate = t["valu_wave_insts_per_launch"] / (dom["ms"] * 1e-3)
-                valu = {"valu_wave_insts_per_launch": t["valu_wave_insts_per_launch"],
-                        "valu_issue_frac": round(rate / VALU_ISSUE_PEAK, 4)}
+                valu.update({"valu_wave_insts_per_launch": t["valu_wave_insts_per_launch"],
+                             "valu_issue_frac": round(rate / VALU_ISSUE_PEAK, 4)})
     s_b = 16 if precision == "c128" else 8
     canonical = nb * (4 * s_b * n ** 3 + 3 * s_b * npw) + 8 * n ** 3
     hpsi_ms = float(prof["ms"].sum())
