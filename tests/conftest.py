@@ -18,3 +18,24 @@ def ctx():
     c = chs.Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(scope="session")
+def tune_header(tmp_path_factory):
+    """First line of a tuning cache written by this library on this device
+    (a cache file with this header and a chosen 'vloc' line forces the
+    H|psi> launch shape of a new context)."""
+    import numpy as np
+    import chs
+    path = tmp_path_factory.mktemp("tune") / "cache"
+    os.environ["CHS_TUNE_CACHE"] = str(path)
+    try:
+        c = chs.Context(0)
+        mil, idx, kin = chs.gsphere(c, 10.26, 15.0, 36)
+        veff = chs.Gpu.from_host(c, np.zeros(36 ** 3))
+        chs.Hamiltonian(c, 36, mil, kin, veff)
+        chs.autotune_hpsi(c, 4, chs.C64)
+        c.close()
+    finally:
+        del os.environ["CHS_TUNE_CACHE"]
+    return path.read_text().splitlines()[0]

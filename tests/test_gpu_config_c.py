@@ -63,23 +63,6 @@ SHAPES = {np.complex64: [(256, 1), (256, 2), (512, 1), (512, 2), (512, 4), (1024
           np.complex128: [(256, 1), (256, 2), (512, 1), (512, 2)]}
 
 
-@pytest.fixture(scope="module")
-def tune_header(tmp_path_factory):
-    """First line of a tuning cache written by this library on this device."""
-    path = tmp_path_factory.mktemp("tune") / "cache"
-    os.environ["CHS_TUNE_CACHE"] = str(path)
-    try:
-        c = chs.Context(0)
-        mil, idx, kin = chs.gsphere(c, 10.26, 15.0, 36)
-        veff = chs.Gpu.from_host(c, np.zeros(36 ** 3))
-        chs.Hamiltonian(c, 36, mil, kin, veff)
-        chs.autotune_hpsi(c, 4, chs.C64)
-        c.close()
-    finally:
-        del os.environ["CHS_TUNE_CACHE"]
-    return path.read_text().splitlines()[0]
-
-
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 3e-5)])
 def test_plane_kernel_every_launch_shape(cell, tune_header, tmp_path, dtype, tol):
     psi = cell.initial_psi(3)

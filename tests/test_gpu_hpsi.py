@@ -172,6 +172,38 @@ def test_apply_h_plane_window(monkeypatch, n_rep, n_grid, kf, window):
     del H
 
 
+@pytest.mark.parametrize("n_rep,n_grid,kf", [(1, 36, (0, 0, 0)), (2, 72, (0, 0, 0)),
+                                             (1, 36, (0.25, 0.25, 0.25))])
+def test_apply_h_every_plane_launch_shape(tune_header, tmp_path, n_rep, n_grid, kf):
+    """Every launch shape of the plane kernel (workgroup size, row-buffer
+    divisor, sticks per block of the z passes), forced through a tuning-cache
+    file, against the oracle in both precisions: the register and LDS row
+    plans and the V_eff prefetch on these grids' 4 x 9 and 8 x 9 splits."""
+    import os
+    nb = 5
+    m = model(n_rep, n_grid, nb, kfrac=kf)
+    psi = m.initial_psi(nb)
+    ref = m.apply_h(psi)
+    shapes = [(sb, thr, div) for sb in (16, 32) for thr in (256, 512, 1024) for div in (1, 2)]
+    for dt, prec, tol in ((np.complex128, chs.C128, 1e-12), (np.complex64, chs.C64, 2e-5)):
+        for sb, thr, div in shapes:
+            if thr == 1024 and dt == np.complex128:
+                continue
+            f = tmp_path / f"tune_{prec}_{sb}_{thr}_{div}"
+            f.write_text(f"{tune_header}\nvloc {n_grid} {prec} {sb} {div} {thr}\n")
+            os.environ["CHS_TUNE_CACHE"] = str(f)
+            c = chs.Context(0)
+            try:
+                H = bind(c, m)
+                got = chs.apply_full_hamiltonian(c, chs.Gpu.from_host(c, psi.astype(dt)))
+                err = np.abs(got.sync_to_host() - ref).max() / np.abs(ref).max()
+                assert err < tol, (dt, sb, thr, div, err)
+                del H, got
+            finally:
+                del os.environ["CHS_TUNE_CACHE"]
+                c.close()
+
+
 def test_apply_h_closed_forms(ctx):
     """V = 0 => H psi = 0.5|G|^2 psi; V = const => shift."""
     m = model(1, 36, 4)
