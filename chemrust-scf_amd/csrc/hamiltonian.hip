@@ -50,9 +50,18 @@ struct StickArgs {
 
 template <typename T> __device__ __forceinline__ cplx<T> zero_c() { return mk((T)0, (T)0); }
 
+// Workgroup of the z passes: whole wavefronts enough for one lane group per
+// stick of the block, so every stick's line is transformed in one round
+// (16, 32, 64 sticks of a 108-point grid: 3, 5, 10 wavefronts of 7 groups).
+template <int N, int SB> struct StickTpb {
+  static constexpr int W = (SB + Groups<N>::PER_WAVE - 1) / Groups<N>::PER_WAVE;
+  static constexpr int value = W * 64;
+  static_assert(value <= 1024, "z-pass workgroup too large");
+};
+
 // --- A: scatter + inverse z-FFT -------------------------------------------
 template <int N, typename T, int SB>
-__global__ void __launch_bounds__(256) k_stick_inv(const cplx<T>* __restrict__ psi, StickArgs st,
+__global__ void __launch_bounds__((StickTpb<N, SB>::value)) k_stick_inv(const cplx<T>* __restrict__ psi, StickArgs st,
                                                    cplx<T>* __restrict__ sbuf,
                                                    const cplx<T>* __restrict__ tw) {
   constexpr int LS = N + 1;
@@ -206,7 +215,7 @@ __global__ void __launch_bounds__(TPB) k_plane(cplx<T>* __restrict__ sbuf, Stick
 // c and inv_e come from `coef` (device, {c, inv_e}) when it is given, so a
 // captured graph of the filter replays with new spectral bounds.
 template <int N, typename T, int MODE, int SB>
-__global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ sbuf, StickArgs st,
+__global__ void __launch_bounds__((StickTpb<N, SB>::value)) k_stick_fwd(const cplx<T>* __restrict__ sbuf, StickArgs st,
                                                    const cplx<T>* __restrict__ psi,
                                                    const double* __restrict__ kin,
                                                    cplx<T>* out, const cplx<T>* aux0, T c, T inv_e,
@@ -217,7 +226,7 @@ __global__ void __launch_bounds__(256) k_stick_fwd(const cplx<T>* __restrict__ s
                                                    const double* __restrict__ ress) {
   constexpr int LS = N + 1;
   __shared__ cplx<T> lds[SB * LS];
-  __shared__ double red[2][4];
+  __shared__ double red[2][16];  // one slot per wavefront
   if (coef) {
     c = (T)coef[0];
     inv_e = (T)coef[1];
@@ -561,7 +570,8 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
                        hipEvent_t* ev, const cplx<T>* resw, const double* ress) {
   const int gz = ceil_div(sa.n_st, SB);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
-  hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream, psi, sa,
+  hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, n_b), dim3(StickTpb<NN, SB>::value), 0,
+                     ctx->stream, psi, sa,
                      sbuf, tw);
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
   const unsigned pgrid = (unsigned)(((size_t)NN * n_b + 7) / 8 * 8);
@@ -599,7 +609,8 @@ static int vloc_launch(chs_context* ctx, const StickArgs& sa, const PlaneLaunch&
 #undef CHS_PLANE
   if (ev) CHS_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
 #define CHS_FWD(M)                                                                            \
-  hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(256), 0, ctx->stream,   \
+  hipLaunchKernelGGL((k_stick_fwd<NN, T, M, SB>), dim3(gz, n_b), dim3(StickTpb<NN, SB>::value), \
+                     0, ctx->stream,                                                           \
                      sbuf, sa, psi, kin, out, aux0, (T)c, (T)inv_e, coef, (T)inv_n3, part, tw,  \
                      resw, ress)
   if (mode == EPI_HPSI)
@@ -794,7 +805,8 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
     if (sa.win && sa.rs != Window<NN>::RS)
       return set_error(ctx, CHS_ERR_FFT, "stick table row stride %d != window stride %d", sa.rs,
                        Window<NN>::RS);
-    hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, nb), dim3(256), 0, ctx->stream, psi,
+    hipLaunchKernelGGL((k_stick_inv<NN, T, SB>), dim3(gz, nb), dim3(StickTpb<NN, SB>::value), 0,
+                       ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
     auto planes = [&](int tpb) -> int {
