@@ -5,7 +5,7 @@
 // inverse, veff_multiply, a batched cuFFT forward and gather_add_kinetic over
 // n_b full N^3 grids. Here the same operator is three kernels that never
 // materialise a full grid:
-//   A  k_stick_inv : per (band, 32 sticks): scatter the sticks' plane waves
+//   A  k_stick_inv : per (band, SB sticks): scatter the sticks' plane waves
 //                    into LDS, inverse z-FFT, write the stick buffer
 //                    sbuf[b][z][stick] (coalesced over sticks);
 //   B  k_plane     : per (band, z-plane): sticks -> LDS plane in compact
@@ -14,13 +14,14 @@
 //                    compact row, x V_eff(r) and the forward x-FFT in
 //                    registers, active outputs back (row_apply_v); forward
 //                    y-FFT; sticks back to sbuf (in place);
-//   C  k_stick_fwd : per (band, 32 sticks): forward z-FFT, gather the plane
+//   C  k_stick_fwd : per (band, SB sticks): forward z-FFT, gather the plane
 //                    waves, x 1/N^3, + 0.5|k+G|^2 psi, and the Chebyshev
 //                    three-term update of chebyshev.rs:700-762 in the same
 //                    epilogue, with per-block norm partial sums for the
 //                    divergence guard (:546-563).
 // Every 1D line is transformed by a lane group inside one wavefront
-// (group_fft_line), so workgroup barriers are needed only between phases.
+// (group_fft_line), so workgroup barriers are needed only between phases;
+// SB (16, 32 or 64) and the plane kernel's launch shape are tuned per grid.
 #include <algorithm>
 #include <cmath>
 
