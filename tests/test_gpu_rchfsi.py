@@ -108,3 +108,38 @@ def test_scf_with_residual_refinement_matches_fp64(ctx, npa):
     assert abs(rr["energy"] - r64["energy"]) / 8 < 1e-7
     assert er[1] <= 3 * e64[1] + 1e-7
     assert er[2] <= 3 * e64[2] + 1e-6
+
+
+def test_kpoint_scf_with_residual_refinement_matches_fp64(ctx):
+    """The same on a k-point set: the 8-atom USPP cell on the Gamma-centred
+    {0, 1/2}^3 set (8 k-points with uneven plane-wave counts), the complex64
+    filter then the residual filter per k-point (fp64_refine = 1) against the
+    plain FP64 k-point SCF: energy within 1e-7 Ha/atom and the Fermi level
+    within 1e-6 Ha, eigenvalues and density no further from a tightly
+    converged FP64 solution than the plain FP64 run."""
+    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=100,
+              n_proj_per_atom=4, q0=0.25)
+    pos = O.positions(1)
+    kp = np.array([[i, j, k] for i in (0.0, 0.5) for j in (0.0, 0.5) for k in (0.0, 0.5)])
+    w = np.full(len(kp), 1.0 / len(kp))
+    r64 = chs.run_scf_k(ctx, chs.scf_params(**kw), pos, kp, w, want_rho=True)
+    rr = chs.run_scf_k(ctx, chs.scf_params(**kw, precision=chs.C64, fp64_refine=1), pos, kp, w,
+                       want_rho=True)
+    rt = chs.run_scf_k(ctx, chs.scf_params(**kw, etol=1e-8), pos, kp, w, want_rho=True)
+    assert r64["converged"] and rr["converged"] and rt["converged"]
+    assert rr["n_iter"] - rr["n_iter_fp32"] >= 3
+    dv = 10.26 ** 3 / 36 ** 3
+
+    def err(r):
+        return (abs(r["energy"] - rt["energy"]) / 8, np.abs(r["eps"] - rt["eps"]).max(),
+                np.abs(r["rho"] - rt["rho"]).sum() * dv)
+
+    e64, er = err(r64), err(rr)
+    print(f"8 k-points: residual-refined {rr['n_iter']} iterations ({rr['n_iter_fp32']} FP32) vs "
+          f"FP64 {r64['n_iter']}; distance to the tight solution (E/atom, eps, rho L1): "
+          f"FP64 {e64[0]:.2e} {e64[1]:.2e} {e64[2]:.2e}, refined {er[0]:.2e} {er[1]:.2e} {er[2]:.2e}")
+    assert rr["n_iter"] <= r64["n_iter"] + 3
+    assert abs(rr["energy"] - r64["energy"]) / 8 < 1e-7
+    assert abs(rr["mu"] - r64["mu"]) < 1e-6
+    assert er[1] <= 3 * e64[1] + 1e-7
+    assert er[2] <= 3 * e64[2] + 1e-6
