@@ -369,7 +369,12 @@ def run_form(chs, ctx, cfg, precision, steps, warmup, kpts=None, refine=0):
               f"converged={r['converged']}", file=sys.stderr, flush=True)
     runs = []
     for k in range(steps):
+        ctx.synchronize()
+        tw = time.perf_counter()
         r = run()
+        ctx.synchronize()
+        # wall clock of the whole timed run: system set-up, upload and SCF
+        r["wall_s"] = time.perf_counter() - tw
         runs.append(r)
         print(f"[{precision}] step {k}: {r['n_iter']} iterations, {r['seconds']:.3f} s, "
               f"E = {r['energy']:.10f}", file=sys.stderr, flush=True)
@@ -441,19 +446,17 @@ def main():
             ctx.synchronize()
         print(json.dumps(roofline(chs, ctx, cfg, precision, tune=True)))
         return
-    ctx.synchronize()
-    t0 = time.perf_counter()
     refine = 1 if args.refine == "residual" else 0
     runs = run_form(chs, ctx, cfg, precision, args.steps, args.warmup, kpts, refine)
-    ctx.synchronize()
-    t1 = time.perf_counter()
     value = sum(r["seconds"] for r in runs) / len(runs)
     last = runs[-1]
     n_atoms = 8 * cfg["n_rep"] ** 3
     line = {
         "metric": METRIC, "value": round(value, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round((t1 - t0) / max(1, args.steps + args.warmup) * 1e3, 2),
+        # one step = one timed SCF run, wall clock (set-up and upload included,
+        # warm-up runs and first-use tuning excluded); value is the SCF loop alone
+        "ms_per_step": round(sum(r["wall_s"] for r in runs) / len(runs) * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": precision, "data": "synthetic",
         "config": {"workload": WORKLOAD[args.config], "atoms": n_atoms,
