@@ -72,8 +72,8 @@ void tune_cache_load(chs_context* ctx) {
       int choice;
       if (ss >> k.tsize >> k.oa >> k.ob >> k.m >> k.n >> k.k >> choice) im->gemm_tune[k] = choice;
     } else if (kind == "dens") {
-      int key, tpb;
-      if (ss >> key >> tpb) im->density_tpb[key] = tpb;
+      int key, shape;
+      if (ss >> key >> shape) im->density_tpb[key] = shape;
     }
   }
 }

@@ -346,8 +346,11 @@ __global__ void __launch_bounds__(TPB) k_density_plane(const z64* __restrict__ s
 // y-lines through the groups' tiles (first nrb groups), then the rows in the
 // bank-spreading order, each an inverse x-transform read straight from the
 // compact row (row_density).
+// complex64: at most 128 registers, so two 512-thread workgroups share a CU
+// (with half the row buffers, which the 160 KB LDS then holds twice)
 template <int N, typename T, int TPB>
-__global__ void __launch_bounds__(TPB) k_density_plane_w(const cplx<T>* __restrict__ sbuf,
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+k_density_plane_w(const cplx<T>* __restrict__ sbuf,
                                                          StickArgs st,
                                                          const double* __restrict__ occ, int n_b,
                                                          int chunk, int nrb,
@@ -797,8 +800,11 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
   StickArgs sa = stick_args(st);
   constexpr int SB = 16;
   const int gz = ceil_div(st.n_st, SB);
-  // workgroup size of the density planes (256 or 512 threads), timed once per
-  // (grid, precision) on the first call (both give bit-identical partial planes)
+  // launch shape of the density planes, timed once per (grid, precision) on
+  // the first call: workgroup size (256 or 512 threads, low 16 bits) and the
+  // row-buffer divisor (high bits; rows go to fewer groups, so more
+  // workgroups fit a CU's LDS). Every shape gives bit-identical partial
+  // planes (each output element sums its bands in the same order).
   const int key = 2 * n + (sizeof(T) == 4 ? 1 : 0);
   auto tpb_it = im->density_tpb.find(key);
   const bool tune = tpb_it == im->density_tpb.end();
@@ -810,10 +816,12 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
                        ctx->stream, psi,
                        sa, sbuf, tw);
     CHS_LAUNCH_CHECK(ctx);
-    auto planes = [&](int tpb) -> int {
+    auto planes = [&](int shape) -> int {
+      const int tpb = (shape & 0xffff) == 512 ? 512 : 256;
+      const int div = std::min(8, std::max(1, shape >> 16));
       PlaneLaunch pl;
       if (sa.win) {
-        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(cplx<T>), Split<NN>::N2, 1, tpb,
+        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(cplx<T>), Split<NN>::N2, div, tpb,
                              RowScratch<NN, T>::STRIDE, &pl));
         if (tpb == 512)
           hipLaunchKernelGGL((k_density_plane_w<NN, T, 512>), dim3(NN, n_chunks), dim3(512),
@@ -822,7 +830,7 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
           hipLaunchKernelGGL((k_density_plane_w<NN, T, 256>), dim3(NN, n_chunks), dim3(256),
                              pl.lds, ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
       } else if constexpr (sizeof(T) == 8) {
-        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, 1, tpb, NN + 1, &pl));
+        CHS_TRY(plane_launch(ctx, NN, sa, sizeof(z64), Split<NN>::N2, div, tpb, NN + 1, &pl));
         if (tpb == 512)
           hipLaunchKernelGGL((k_density_plane<NN, 512>), dim3(NN, n_chunks), dim3(512), pl.lds,
                              ctx->stream, sbuf, sa, docc, nb, chunk, pl.nrb, part, tw);
@@ -839,16 +847,16 @@ static int density_t(chs_context* ctx, const cplx<T>* psi, int n_b, const double
     if (tune) {
       int best = 256;
       float best_t = 1e30f;
-      for (int tpb : {256, 512}) {
+      for (int shape : {256 | 1 << 16, 512 | 1 << 16, 512 | 2 << 16, 512 | 4 << 16}) {
         CHS_HIP(ctx, hipEventRecord(im->ev0, ctx->stream));
-        CHS_TRY(planes(tpb));
+        CHS_TRY(planes(shape));
         CHS_HIP(ctx, hipEventRecord(im->ev1, ctx->stream));
         CHS_HIP(ctx, hipEventSynchronize(im->ev1));
         float t = 0;
         CHS_HIP(ctx, hipEventElapsedTime(&t, im->ev0, im->ev1));
         if (t < best_t) {
           best_t = t;
-          best = tpb;
+          best = shape;
         }
       }
       im->density_tpb[key] = best;

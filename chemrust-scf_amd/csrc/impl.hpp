@@ -141,7 +141,7 @@ struct chs_scf;  // a caller-supplied SCF system (scf_driver.hip)
 struct chs_context::Impl {
   std::map<std::pair<int, int>, void*> twiddles;  // (N, precision) -> table
   std::map<std::pair<int, int>, chs::VlocTune> vloc_tune;  // (N, precision) -> launch shape
-  std::map<int, int> density_tpb;  // 2 N + (complex64) -> density-plane workgroup size
+  std::map<int, int> density_tpb;  // 2 N + (complex64) -> density-plane launch shape
   bool hpsi_autotune = true;  // chs_scf_run tunes new grids (CHS_HPSI_TUNE=0: off)
   // plane kernels that skip the zero window of each line when the stick
   // table allows (CHS_PLANE_WINDOW=0: the general kernels throughout)

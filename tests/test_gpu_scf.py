@@ -49,6 +49,67 @@ def test_density_plane_window(monkeypatch, n_rep, n_grid, nb, window):
     del H
 
 
+DENSITY_SHAPES = [(256, 1), (512, 1), (512, 2), (512, 4)]  # (threads, row-buffer divisor)
+
+
+@pytest.mark.parametrize("n_rep,n_grid,nb", [(1, 36, 20), (2, 72, 24)])
+def test_density_every_launch_shape(tune_header, tmp_path, n_rep, n_grid, nb):
+    """Every launch shape of the zero-window density planes (workgroup size,
+    row-buffer divisor), forced through a tuning-cache file, against the
+    oracle's density."""
+    import os
+    p = O.default_params(n_rep=n_rep, n_grid=n_grid, n_bands=nb)
+    m = O.Model(p, O.positions(n_rep))
+    mil, idx, kin = m.pw()
+    psi = m.initial_psi(nb)
+    occ = np.linspace(2.0, 0.1, nb)
+    occ[-3:] = 0.0
+    ref = m.density(psi, occ)
+    omega = (10.26 * n_rep) ** 3
+    for thr, div in DENSITY_SHAPES:
+        f = tmp_path / f"tune_{thr}_{div}"
+        f.write_text(f"{tune_header}\ndens {2 * n_grid} {thr | div << 16}\n")
+        os.environ["CHS_TUNE_CACHE"] = str(f)
+        c = chs.Context(0)
+        try:
+            H = chs.Hamiltonian(c, n_grid, mil, kin, m.vion())
+            got = chs.density(c, chs.Gpu.from_host(c, psi), occ, omega, n_grid).sync_to_host()
+            assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max() + 1e-16, (thr, div)
+            del H
+        finally:
+            del os.environ["CHS_TUNE_CACHE"]
+            c.close()
+
+
+def test_complex64_density_shapes_give_identical_scf(tune_header, tmp_path, monkeypatch):
+    """The complex64 density planes (the mixed SCF's density) in every launch
+    shape: with the GEMM kernel and the H|psi> launch shape pinned, the mixed
+    SCF of the 8-atom cell is bit-identical across density shapes (each
+    output element sums its bands in the same order) and converges."""
+    import os
+    monkeypatch.setenv("CHS_GEMM_CHOICE", "1")
+    kw = dict(n_rep=1, n_grid=36, n_bands=20, cheb_degree=10, max_iter=60, precision=chs.C64)
+    pos = O.positions(1)
+    runs = []
+    for thr, div in DENSITY_SHAPES:
+        f = tmp_path / f"tune64_{thr}_{div}"
+        f.write_text(f"{tune_header}\nvloc 36 {chs.C64} 16 1 512\nvloc 36 {chs.C128} 16 1 256\n"
+                     f"dens {2 * 36 + 1} {thr | div << 16}\ndens {2 * 36} 256\n")
+        os.environ["CHS_TUNE_CACHE"] = str(f)
+        c = chs.Context(0)
+        try:
+            r = chs.run_scf(c, chs.scf_params(**kw), pos, want_rho=True)
+        finally:
+            del os.environ["CHS_TUNE_CACHE"]
+            c.close()
+        assert r["converged"] and r["n_iter_fp32"] > 0, (thr, div)
+        runs.append(r)
+    for r, (thr, div) in zip(runs[1:], DENSITY_SHAPES[1:]):
+        assert r["n_iter"] == runs[0]["n_iter"], (thr, div)
+        assert r["energy"] == runs[0]["energy"], (thr, div)
+        assert np.array_equal(r["rho"], runs[0]["rho"]), (thr, div)
+
+
 def test_density_single_plane_wave(ctx):
     """One plane wave => |psi(r)|^2 is constant = 1/Omega."""
     p = O.default_params(n_rep=1, n_grid=36, n_bands=1)
