@@ -59,8 +59,12 @@ def test_filter_c64_vs_c128_config_c(ctx, cell):
 
 # Plane-kernel launch shapes (workgroup size, row-buffer divisor) the tuner
 # may choose; each is forced through a tuning-cache file.
-SHAPES = {np.complex64: [(256, 1), (256, 2), (512, 1), (512, 2), (512, 4), (1024, 1), (1024, 2)],
-          np.complex128: [(256, 1), (256, 2), (512, 1), (512, 2)]}
+# (sticks per z-pass block, plane-kernel threads, row-buffer divisor): every plane shape
+# with 16 sticks per block, and the larger z-pass blocks the tuner also tries where their
+# LDS fits (complex128 with 64 sticks does not: the library then takes 16)
+SHAPES = {np.complex64: [(16, 256, 1), (16, 256, 2), (16, 512, 1), (16, 512, 2), (16, 512, 4),
+                         (16, 1024, 1), (16, 1024, 2), (32, 512, 2), (64, 512, 2)],
+          np.complex128: [(16, 256, 1), (16, 256, 2), (16, 512, 1), (16, 512, 2), (32, 256, 2)]}
 
 
 @pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 3e-5)])
@@ -68,9 +72,9 @@ def test_plane_kernel_every_launch_shape(cell, tune_header, tmp_path, dtype, tol
     psi = cell.initial_psi(3)
     rh = cell.apply_h(psi)
     prec = chs.C64 if dtype == np.complex64 else chs.C128
-    for threads, div in SHAPES[dtype]:
-        f = tmp_path / f"tune_{threads}_{div}"
-        f.write_text(f"{tune_header}\nvloc 108 {prec} 16 {div} {threads}\n")
+    for sb, threads, div in SHAPES[dtype]:
+        f = tmp_path / f"tune_{sb}_{threads}_{div}"
+        f.write_text(f"{tune_header}\nvloc 108 {prec} {sb} {div} {threads}\n")
         os.environ["CHS_TUNE_CACHE"] = str(f)
         c = chs.Context(0)
         try:
@@ -78,8 +82,9 @@ def test_plane_kernel_every_launch_shape(cell, tune_header, tmp_path, dtype, tol
             gh = chs.apply_full_hamiltonian(c, chs.Gpu.from_host(c, psi.astype(dtype)))
             gh = gh.sync_to_host().astype(np.complex128)
             err = np.abs(gh - rh).max() / np.abs(rh).max()
-            print(f"{threads} threads, row-buffer divisor {div}: {err:.1e}")
-            assert err < tol, (threads, div, err)
+            print(f"{sb} sticks per z block, {threads} threads, row-buffer divisor {div}: "
+                  f"{err:.1e}")
+            assert err < tol, (sb, threads, div, err)
             del H, gh
         finally:
             del os.environ["CHS_TUNE_CACHE"]

@@ -184,7 +184,7 @@ def test_apply_h_every_plane_launch_shape(tune_header, tmp_path, n_rep, n_grid, 
     m = model(n_rep, n_grid, nb, kfrac=kf)
     psi = m.initial_psi(nb)
     ref = m.apply_h(psi)
-    shapes = [(sb, thr, div) for sb in (16, 32) for thr in (256, 512, 1024) for div in (1, 2)]
+    shapes = [(sb, thr, div) for sb in (16, 32, 64) for thr in (256, 512, 1024) for div in (1, 2)]
     for dt, prec, tol in ((np.complex128, chs.C128, 1e-12), (np.complex64, chs.C64, 2e-5)):
         for sb, thr, div in shapes:
             if thr == 1024 and dt == np.complex128:
