@@ -108,7 +108,8 @@ def collect_pairs_seeded(ctx, seeds, make, pairs, **kw):
 def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False, passes=3, degree=8):
     """Band-CG from the same ChFSI block (`passes` passes of degree `degree`)
     with TPA and with the network. Returns dict with total steps, converged
-    bands and max |eps - eps*| for both."""
+    bands, max |eps - eps*| over all bands and over the converged ones, for
+    both."""
     sample.bind()
     ref, eps_ref = sample.converged()
     blk, _ = sample.rough(passes, degree)
@@ -122,6 +123,9 @@ def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False, passes=
             table = net.scale_table(b)
             eps, steps, conv = chs.band_cg_minimize(ctx, b, max_steps, tol, precond=2,
                                                     mlp_scale=table)
+        err = np.abs(eps - eps_ref)
+        done = np.asarray(conv) != 0
         out[name] = {"steps": int(steps.sum()), "max_steps": int(steps.max()),
-                     "converged": int(conv.sum()), "eps_err": float(np.abs(eps - eps_ref).max())}
+                     "converged": int(done.sum()), "eps_err": float(err.max()),
+                     "eps_err_converged": float(err[done].max()) if done.any() else None}
     return out
