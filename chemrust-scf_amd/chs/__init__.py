@@ -827,16 +827,19 @@ def autotune_hpsi(ctx, n_b, precision=C128):
     return sb.value, rd.value, th.value, ms.value
 
 
-def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None):
+def band_cg_minimize(ctx, psi, max_steps, tol, precond=0, mlp_scale=None, full_overlap=False):
     """band_cg_minimize (src/eigensolver/band_cg.rs:157-437) for every band of the
     c128 device block psi in place; band i against bands j < i of the block as
     handed in. precond: 0 TPA, 1 TPA + USPP term, 2 MLP scale table (device f32,
-    n_b x n_pw). Returns (eps, n_steps, converged)."""
+    n_b x n_pw). full_overlap: remove the complex lower-band overlap instead of
+    its real part (cg_helpers.rs:188; not in the reference). Returns (eps,
+    n_steps, converged)."""
     n_b = psi.shape[0]
     eps = np.zeros(n_b)
     steps = np.zeros(n_b, np.int32)
     conv = np.zeros(n_b, np.int32)
-    ctx.check(lib().chs_band_cg(ctx.h, psi.ptr, n_b, max_steps, tol, precond,
+    ctx.check(lib().chs_band_cg(ctx.h, psi.ptr, n_b, max_steps, tol,
+                                precond | (256 if full_overlap else 0),
                                 mlp_scale.ptr if mlp_scale is not None else None, _dptr(eps),
                                 steps.ctypes.data_as(vp), conv.ctypes.data_as(vp)))
     return eps, steps, conv

@@ -105,24 +105,39 @@ def collect_pairs_seeded(ctx, seeds, make, pairs, **kw):
     return used
 
 
-def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False, passes=3, degree=8):
+def evaluate(ctx, sample, net, tol=1e-9, max_steps=200, uspp_term=False, passes=3, degree=8,
+             full_overlap=False):
     """Band-CG from the same ChFSI block (`passes` passes of degree `degree`)
-    with TPA and with the network. Returns dict with total steps, converged
-    bands, max |eps - eps*| over all bands and over the converged ones, for
-    both."""
+    with TPA and with the network (full_overlap: the lower-band
+    S-orthogonalisation removes the complex overlap, see band_cg_minimize).
+    Returns dict with total steps, converged bands, max |eps - eps*| over all
+    bands and over the converged ones, for both."""
+    return evaluate_settings(ctx, sample, net, [(tol, max_steps, full_overlap)], uspp_term,
+                             passes, degree)[0]
+
+
+def evaluate_settings(ctx, sample, net, settings, uspp_term=False, passes=3, degree=8):
+    """evaluate() for several (tol, max_steps, full_overlap) settings from
+    one converged reference and one start block of the sample."""
     sample.bind()
     ref, eps_ref = sample.converged()
     blk, _ = sample.rough(passes, degree)
+    return [_cg_pair(ctx, net, blk, eps_ref, tol, max_steps, full_overlap, uspp_term)
+            for tol, max_steps, full_overlap in settings]
+
+
+def _cg_pair(ctx, net, blk, eps_ref, tol, max_steps, full_overlap, uspp_term):
     out = {}
     for name in ("tpa", "mlp"):
         b = copy_block(ctx, blk)
         if name == "tpa":
             eps, steps, conv = chs.band_cg_minimize(ctx, b, max_steps, tol,
-                                                    precond=1 if uspp_term else 0)
+                                                    precond=1 if uspp_term else 0,
+                                                    full_overlap=full_overlap)
         else:
             table = net.scale_table(b)
             eps, steps, conv = chs.band_cg_minimize(ctx, b, max_steps, tol, precond=2,
-                                                    mlp_scale=table)
+                                                    mlp_scale=table, full_overlap=full_overlap)
         err = np.abs(eps - eps_ref)
         done = np.asarray(conv) != 0
         out[name] = {"steps": int(steps.sum()), "max_steps": int(steps.max()),

@@ -152,13 +152,17 @@ __global__ void k_precond_fix(z64* __restrict__ out, const z64* __restrict__ cor
   out[i] = mk(out[i].x + f * corr[i].x, out[i].y + f * corr[i].y);
 }
 
-// Masked real overlaps for the lower-band S-orthogonalisation:
-// OV[j, i] <- Re(OV[j, i]) for j < i, else 0 (cg_helpers.rs:188 uses the real part)
-__global__ void k_lower_real(z64* __restrict__ ov, int n_b, const BandState* __restrict__ st) {
+// Masked overlaps for the lower-band S-orthogonalisation:
+// OV[j, i] <- Re(OV[j, i]) for j < i, else 0 (cg_helpers.rs:188 uses the real
+// part); with `full` the complex OV[j, i] is kept, which removes the whole
+// component along band j (the real part alone leaves i Im(OV) psi_j behind,
+// so complex bands drift into lower ones over many steps)
+__global__ void k_lower_real(z64* __restrict__ ov, int n_b, const BandState* __restrict__ st,
+                             int full) {
   const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_b) return;
   z64& v = ov[(size_t)j + (size_t)i * n_b];
-  v = (j < i && st[i].active) ? mk(v.x, 0.0) : mk(0.0, 0.0);
+  v = (j < i && st[i].active) ? (full ? v : mk(v.x, 0.0)) : mk(0.0, 0.0);
 }
 
 // ---- per-band scalar logic ---------------------------------------------------
@@ -493,7 +497,7 @@ int BandCg::run(z64* psi, int max_steps, double tol, const float* mlp, double* e
     // lower-band S-orthogonalisation against the handed-in block (real overlaps)
     CHS_TRY(zgemm(ctx, 'C', 'N', n_b, n_b, n_pw, one, CONV, n_pw, SG, n_pw, mk(0.0, 0.0), OV, n_b));
     hipLaunchKernelGGL(k_lower_real, dim3(ceil_div(n_b, 256), n_b), dim3(256), 0, ctx->stream, OV,
-                       n_b, (const BandState*)st);
+                       n_b, (const BandState*)st, full_overlap ? 1 : 0);
     CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, mone, CONV, n_pw, OV, n_b, one, G, n_pw));
     CHS_TRY(zgemm(ctx, 'N', 'N', n_pw, n_b, n_b, mone, SCONV, n_pw, OV, n_b, one, SG, n_pw));
     bdot(G, SG, d1);
@@ -569,12 +573,15 @@ using namespace chs;
 extern "C" int chs_band_cg(chs_context* ctx, double* d_psi, int n_b, int max_steps, double tol,
                            int precond, const float* d_mlp_scale, double* eps, int* n_steps,
                            int* converged) {
+  const bool full = precond >= 0 && (precond & 256) != 0;
+  if (precond >= 0) precond &= ~256;
   if (!ctx || !d_psi || n_b <= 0 || max_steps < 0 || !(tol > 0) || precond < 0 || precond > 2)
     return CHS_ERR_INVALID_ARG;
   if (precond == 2 && !d_mlp_scale) return CHS_ERR_INVALID_ARG;
   CHS_TRY(ensure_precision_ready(ctx, CHS_C128));
   Hamiltonian& H = ctx->impl->ham;
   BandCg cg;
+  cg.full_overlap = full;
   CHS_TRY(cg.ensure_ws(ctx, H.n_pw, n_b));
   CHS_TRY(cg.setup_precond(precond >= 1));
   return cg.run((z64*)d_psi, max_steps, tol, precond == 2 ? d_mlp_scale : nullptr, eps, n_steps,

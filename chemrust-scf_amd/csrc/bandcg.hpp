@@ -10,6 +10,9 @@ struct BandCg {
   chs_context* ctx = nullptr;
   int n_pw = 0, n_b = 0;
   bool use_r = false;
+  // lower-band S-orthogonalisation with the complex overlap (the reference,
+  // cg_helpers.rs:188, keeps its real part only)
+  bool full_overlap = false;
   DevBuf vecs;  // 12 band blocks
   DevBuf scal;  // per-band scalars, states, the n_b x n_b overlap
   DevBuf tpa;   // t(g)

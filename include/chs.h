@@ -222,8 +222,11 @@ int chs_cholqr(chs_context* ctx, double* d_y, int n_b);
  * (cg_helpers.rs:49-216) and stopping rules. precond: 0 = TPA
  * (uspp_preconditioner.rs:148-180), 1 = TPA + USPP Woodbury term
  * (:56-131), 2 = a per-(band, plane wave) scale table d_mlp_scale (f32,
- * n_b x n_pw) in place of the TPA diagonal. Outputs per band (host):
- * eps, n_steps, converged. */
+ * n_b x n_pw) in place of the TPA diagonal. precond + 256: the lower-band
+ * S-orthogonalisation removes the complex overlap <psi_j|S g> instead of its
+ * real part (cg_helpers.rs:188 keeps the real part, which lets complex bands
+ * drift into lower ones; this option has no counterpart in the reference).
+ * Outputs per band (host): eps, n_steps, converged. */
 int chs_band_cg(chs_context* ctx, double* d_psi, int n_b, int max_steps, double tol, int precond,
                 const float* d_mlp_scale, double* eps, int* n_steps, int* converged);
 

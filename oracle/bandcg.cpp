@@ -125,13 +125,20 @@ int s_normalize(const cd* psi, const cd* spsi, int n, cd* out) {
 
 // cg_helpers.rs:161-216: modified Gram-Schmidt in the S metric using the
 // real part of the overlap, then S-normalise. Returns -1 when the final
-// <psi|S|psi> is not positive.
+// <psi|S|psi> is not positive. With complex vectors the real part leaves
+// i Im<psi_j|S psi> psi_j behind; full_overlap removes the whole overlap.
 int s_orthogonalize_against(const cd* psi, const cd* spsi, int n, int n_conv, const cd* conv_psi,
-                            const cd* conv_spsi, cd* out_psi, cd* out_spsi) {
+                            const cd* conv_spsi, cd* out_psi, cd* out_spsi, bool full_overlap) {
   std::vector<cd> p(psi, psi + n), s(spsi, spsi + n);
   for (int j = 0; j < n_conv; ++j) {
     const cd* pj = conv_psi + (size_t)j * n;
     const cd* sj = conv_spsi + (size_t)j * n;
+    if (full_overlap) {
+      const cd ov = dot(pj, s.data(), n);
+      for (int i = 0; i < n; ++i) p[i] -= ov * pj[i];
+      for (int i = 0; i < n; ++i) s[i] -= ov * sj[i];
+      continue;
+    }
     const double ov = dot(pj, s.data(), n).real();
     for (int i = 0; i < n; ++i) p[i] -= ov * pj[i];
     for (int i = 0; i < n; ++i) s[i] -= ov * sj[i];
@@ -252,7 +259,7 @@ void UsppPrecond::apply(const cd* res, cd* out) const {
 // band; the preconditioner is `pre(res, out)`.
 CgResult band_cg_minimize(const cd* psi_init, int n, const PrecondFn& pre, int n_conv,
                           const cd* conv_psi, const cd* conv_spsi, int max_steps, double tol,
-                          const HsFn& apply_hs) {
+                          const HsFn& apply_hs, bool full_overlap) {
   std::vector<cd> h0(n), s0(n);
   apply_hs(psi_init, h0.data(), s0.data());
   CgResult out;
@@ -289,7 +296,7 @@ CgResult band_cg_minimize(const cd* psi_init, int n, const PrecondFn& pre, int n
     std::vector<cd> htmp(n);
     apply_hs(g.data(), htmp.data(), sg.data());
     if (s_orthogonalize_against(g.data(), sg.data(), n, n_conv, conv_psi, conv_spsi, go.data(),
-                                sgo.data()) != 0) {
+                                sgo.data(), full_overlap) != 0) {
       out.status = -2;
       return out;
     }

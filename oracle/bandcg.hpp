@@ -16,8 +16,11 @@ LineSearchResult line_search_2d_quadratic(double dot_hpsi_psi, double dot_hpsi_d
 int compute_residual(const cd* psi, const cd* hpsi, const cd* spsi, int n, cd* r, double* eps);
 double residual_bare_norm(const cd* r, int n);
 int s_normalize(const cd* psi, const cd* spsi, int n, cd* out);
+// full_overlap: project out the complex overlap instead of its real part
+// (an option of this build; the reference keeps the real part)
 int s_orthogonalize_against(const cd* psi, const cd* spsi, int n, int n_conv, const cd* conv_psi,
-                            const cd* conv_spsi, cd* out_psi, cd* out_spsi);
+                            const cd* conv_spsi, cd* out_psi, cd* out_spsi,
+                            bool full_overlap = false);
 void tpa_diag(const double* kin, int n, double* t);
 
 struct UsppPrecond {
@@ -43,6 +46,6 @@ using PrecondFn = std::function<void(const cd*, cd*)>;
 
 CgResult band_cg_minimize(const cd* psi_init, int n, const PrecondFn& pre, int n_conv,
                           const cd* conv_psi, const cd* conv_spsi, int max_steps, double tol,
-                          const HsFn& apply_hs);
+                          const HsFn& apply_hs, bool full_overlap = false);
 
 }  // namespace orc

@@ -416,10 +416,13 @@ int orc_uspp_precond_apply(const double* beta, int n_pw, int n_proj, const doubl
 // band_cg_minimize for every band of a block, band i against the converged
 // set {(psi_j, S psi_j), j < i} of the block as handed in, with the model's
 // H and S. precond 0: TPA only (beta = 0, Q = I); 1: the full USPP
-// preconditioner with the model's projectors and Q.
+// preconditioner with the model's projectors and Q; + 256: the lower-band
+// S-orthogonalisation removes the complex overlap (not only its real part).
 int orc_band_cg_model(void* mm, double* psi_io, int n_b, int max_steps, double tol, int precond,
                       double* eps, int* n_steps, int* converged) {
   auto& m = static_cast<OrcModel*>(mm)->m;
+  const bool full = (precond & 256) != 0;
+  precond &= 255;
   const int n = m.pw.n_pw;
   const Projectors* pj = m.pj.n_proj ? &m.pj : nullptr;
   std::vector<cd> in(C(psi_io), C(psi_io) + (size_t)n_b * n), sin((size_t)n_b * n), tmp(n);
@@ -447,7 +450,7 @@ int orc_band_cg_model(void* mm, double* psi_io, int n_b, int max_steps, double t
   PrecondFn pre = [&](const cd* r, cd* o) { pc.apply(r, o); };
   for (int b = 0; b < n_b; ++b) {
     CgResult res = band_cg_minimize(in.data() + (size_t)b * n, n, pre, b, in.data(), sin.data(),
-                                    max_steps, tol, hs);
+                                    max_steps, tol, hs, full);
     if (res.status) return res.status;
     std::copy(res.psi.begin(), res.psi.end(), C(psi_io) + (size_t)b * n);
     eps[b] = res.eigenvalue;

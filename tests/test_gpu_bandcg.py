@@ -20,7 +20,9 @@ def start_block(m, nb, passes=2, degree=6):
     return psi
 
 
-@pytest.mark.parametrize("uspp,precond", [(False, 0), (True, 0), (True, 1)])
+# precond + 256: lower-band orthogonalisation with the complex overlap
+@pytest.mark.parametrize("uspp,precond", [(False, 0), (True, 0), (True, 1), (False, 256),
+                                          (True, 257)])
 def test_band_cg_vs_oracle(ctx, uspp, precond):
     kw = dict(n_rep=1, n_grid=36, n_bands=12)
     if uspp:
@@ -33,7 +35,8 @@ def test_band_cg_vs_oracle(ctx, uspp, precond):
     psi0 = start_block(m, 12)
     rpsi, reps, rsteps, rconv = m.band_cg(psi0, 6, 1e-9, precond)
     g = chs.Gpu.from_host(ctx, psi0)
-    eps, steps, conv = chs.band_cg_minimize(ctx, g, 6, 1e-9, precond)
+    eps, steps, conv = chs.band_cg_minimize(ctx, g, 6, 1e-9, precond & 255,
+                                            full_overlap=bool(precond & 256))
     assert np.array_equal(steps, rsteps)
     assert np.array_equal(conv, rconv)
     assert np.abs(eps - reps).max() < 1e-10

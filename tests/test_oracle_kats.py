@@ -651,6 +651,32 @@ def test_band_cg_converges_to_ground_state():
     assert abs(r["eig"] - 0.5) < 1e-8
 
 
+def test_band_cg_lower_band_overlap():
+    """cg_helpers.rs:188 removes only the real part of <psi_j|S g>. With
+    complex bands (the 8-atom cell at Gamma, 12 bands after three degree-8
+    filter passes) band CG then lets bands leave the orthogonal complement
+    of the lower bands it was handed. With the complex overlap removed
+    (precond + 256, an option of this build) every band stays orthogonal
+    to them, and the empty bands stay above the eighth Ritz value."""
+    m = O.Model(O.default_params(n_rep=1, n_grid=36, n_bands=12), O.positions(1))
+    psi = m.initial_psi(12)
+    b4 = O.spectral_bounds(None, m.lanczos_bound(10, 4)[0])
+    for _ in range(3):
+        _, psi, _ = m.chebyshev_filter(psi, 8, b4)
+        psi, eps0 = m.rayleigh_ritz(psi)
+        b4 = O.spectral_bounds(eps0, b4[0])
+
+    def lower_overlap(p):
+        return max((np.abs(psi[:i].conj() @ p[i]) / np.linalg.norm(p[i])).max()
+                   for i in range(1, 12))
+
+    p_re, _, _, _ = m.band_cg(psi, 20, 1e-9, 0)
+    p_cx, e_cx, _, _ = m.band_cg(psi, 20, 1e-9, 256)
+    assert lower_overlap(p_re) > 1e-2
+    assert lower_overlap(p_cx) < 1e-10
+    assert e_cx[8:].min() > eps0[7]
+
+
 # ---------------------------------------------------------------------------
 # fine -> wave grid resampling (downsample_array_to_wave_grid, scf.rs:413-454)
 # ---------------------------------------------------------------------------
