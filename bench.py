@@ -135,6 +135,16 @@ def roofline(chs, ctx, cfg, precision, tune=False):
                 rate = t["valu_wave_insts_per_launch"] / (dom["ms"] * 1e-3)
                 valu.update({"valu_wave_insts_per_launch": t["valu_wave_insts_per_launch"],
                              "valu_issue_frac": round(rate / VALU_ISSUE_PEAK, 4)})
+    # every kernel of the step against its own roofline (algorithmic bytes / HBM peak for
+    # the H|psi> kernels, matrix-core flops / dense peak for the projector GEMMs)
+    step_roofline = {}
+    for it in items:
+        if it["bound"] == "hbm":
+            a, pk, u = it["work"] / (it["ms"] * 1e-3) / 1e9, HBM_PEAK_GBS, "GB/s"
+        else:
+            a, pk, u = it["work"] / (it["ms"] * 1e-3) / 1e12, PEAK_TFLOPS[precision], "TFLOP/s"
+        step_roofline[it["kernel"]] = {"bound": it["bound"], "achieved": round(a, 2), "unit": u,
+                                       "frac": round(a / pk, 4)}
     s_b = 16 if precision == "c128" else 8
     canonical = nb * (4 * s_b * n ** 3 + 3 * s_b * npw) + 8 * n ** 3
     hpsi_ms = float(prof["ms"].sum())
@@ -143,6 +153,7 @@ def roofline(chs, ctx, cfg, precision, tune=False):
         "frac": round(achieved / peak, 4), "traffic": traffic, "kernel": dom["kernel"],
         "kernel_ms": round(dom["ms"], 4), "algorithmic_work_per_launch": dom["work"],
         "step_kernels_ms": {it["kernel"]: round(it["ms"], 4) for it in items},
+        "step_kernels_roofline": step_roofline,
         "hpsi_canonical_bytes": float(canonical),
         "hpsi_canonical_gbs": round(canonical / (hpsi_ms * 1e-3) / 1e9, 1),
         "sticks": prof["sticks"], "active_x": prof["active_x"], "precision": precision,
