@@ -28,3 +28,14 @@ def test_library_exports_every_declared_symbol():
 def test_oracle_is_not_linked_into_the_product():
     so = open(os.path.join(ROOT, "chemrust-scf_amd", "libchs_gpu.so"), "rb").read()
     assert b"liboracle" not in so and b"orc_" not in so
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU: --gpus above 1 exits with status 2 and says so,
+    before anything touches a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
