@@ -203,9 +203,9 @@ def test_cholqr(ctx):
 
 def test_tuning_cache_on_disk(tmp_path, monkeypatch):
     """CHS_TUNE_CACHE: one context's tuning decisions (H|psi> launch shape,
-    GEMM variants) are written to the file and read back by the next
-    context, keyed by this library build and device; a file of another
-    build (different header) is ignored."""
+    GEMM variants, density launch shape) are written to the file and read
+    back by the next context, keyed by this library build and device; a
+    file of another build (different header) is ignored."""
     path = tmp_path / "tune.txt"
     monkeypatch.setenv("CHS_TUNE_CACHE", str(path))
     c1 = chs.Context(0)
@@ -218,15 +218,17 @@ def test_tuning_cache_on_disk(tmp_path, monkeypatch):
     a = chs.Gpu.from_host(c1, np.ones((40, 300), complex))
     out = chs.Gpu(c1, (40, 40), np.complex128)
     chs.zgemm(c1, "C", "N", 40, 40, 300, 1.0, a, 300, a, 300, 0.0, out, 40)
+    psi = chs.Gpu.from_host(c1, m.initial_psi(8))
+    chs.density(c1, psi, np.full(8, 2.0), 10.26 ** 3, 36)
     c1.synchronize()
     n1 = c1.tuning_entries()
-    assert n1[0] == 1 and n1[1] >= 1
-    del H
+    assert n1[0] == 1 and n1[1] >= 1 and n1[2] == 1
+    del H, psi
     c1.close()
     lines = path.read_text().splitlines()
-    assert lines[0].startswith("chs-tune 1 ") and len(lines) >= 1 + n1[0] + n1[1]
+    assert lines[0].startswith("chs-tune 1 ") and len(lines) >= 1 + sum(n1)
     c2 = chs.Context(0)
-    assert c2.tuning_entries()[:2] == n1[:2]
+    assert c2.tuning_entries() == n1
     c2.close()
     path.write_text("chs-tune 1 0000000000000000 other\nvloc 36 0 16 1 256\n")
     c3 = chs.Context(0)
