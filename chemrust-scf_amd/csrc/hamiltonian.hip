@@ -351,11 +351,11 @@ __global__ void __launch_bounds__(TPB) k_density_plane(const z64* __restrict__ s
 template <int N, typename T, int TPB>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 k_density_plane_w(const cplx<T>* __restrict__ sbuf,
-                                                         StickArgs st,
-                                                         const double* __restrict__ occ, int n_b,
-                                                         int chunk, int nrb,
-                                                         double* __restrict__ part,
-                                                         const cplx<T>* __restrict__ tw) {
+                  StickArgs st,
+                  const double* __restrict__ occ, int n_b,
+                  int chunk, int nrb,
+                  double* __restrict__ part,
+                  const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
   using W = Window<N>;
   constexpr int RS = W::RS;
